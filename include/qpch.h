@@ -1,0 +1,143 @@
+/*
+ * qpch: the C ABI of the MI355X (gfx950) reconstruction engine.
+ *
+ * Everything the C++ host (qpcu::Backend::hip) and the Python/torch adapter need from the
+ * GPU goes through these entry points: plain pointers and sizes, integer status codes, no
+ * C++ types and no exceptions across the boundary.  Arrays are FFT order (numpy
+ * ifftshift of the centre-origin arrays), row-major, slowest axis first; complex values
+ * are interleaved float pairs (complex64).
+ *
+ * Reference interfaces replaced (NaNIsaac/QoraPhaseCU v0.1.0):
+ *   qpch_solver_*          the CudaSolver behind detail::make_cuda_solver
+ *                          (src/cuda/cuda_solver.cu:58-500, src/core/backends.hpp:9-10)
+ *   qpch_solver_iterate    CudaSolver::iterate/graph/step_device (cuda_solver.cu:300-421)
+ *   qpch_solver_shrinkwrap CudaSolver::apply(Shrinkwrap) (cuda_solver.cu:240-254)
+ *   qpch_solver_*_support  the device side of apply(DetwinBegin/End) (cuda_solver.cu:256-276)
+ *   qpch_solver_read_*     collect_errors/report/results (cuda_solver.cu:165-191,423-453)
+ *   qpch_fft / qpch_modulus_project / qpch_support_project / qpch_gaussian_blur
+ *                          qpcu::cuda operators (include/qoraphasecu/operators.hpp:25-33,
+ *                          src/cuda/operators.cu:62-130) and the cuFFT plans they use
+ *                          (src/cuda/common.cuh:84-140)
+ *   qpch_device_count / qpch_device_info
+ *                          cuda_available / cuda_device_info (src/cuda/operators.cu:13-36)
+ *   qpch_amplitude_loss    no counterpart: the adjoint of the amplitude loss for the torch
+ *                          autograd Function.
+ *
+ * Conventions: every function returns QPCH_OK or an error code and leaves a message for
+ * qpch_last_error() (thread-local).  One solver handle must not be used from two threads
+ * at once.  Solver calls run on the solver's own non-blocking stream and return after
+ * the work is queued, except those that hand data back to the host.
+ */
+#ifndef QPCH_H
+#define QPCH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QPCH_OK 0
+#define QPCH_EINVAL 1  /* bad argument: maps to std::invalid_argument / ValueError */
+#define QPCH_EDEVICE 2 /* no device or a device failure: std::runtime_error */
+
+/* storage precision of the iterate and the wave between passes */
+#define QPCH_F32 0
+#define QPCH_F16 1
+#define QPCH_BF16 2
+
+/* execution schedules, the reference's Variant ladder (include/qoraphasecu/solver.hpp:21-29) */
+#define QPCH_UNFUSED 0 /* one kernel per operation: transforms per axis, modulus, update */
+#define QPCH_FUSED 1   /* two fused passes per iteration (three or four in 3D), eager launches */
+#define QPCH_GRAPH 2   /* fused passes replayed from hipGraphs */
+
+typedef struct qpch_solver qpch_solver;
+
+typedef struct {
+  int ndim;          /* 2 or 3 */
+  size_t shape[3];   /* slowest first */
+  const float* amplitude;  /* n, FFT order: sqrt of the clipped intensity */
+  const uint8_t* mask;     /* n, FFT order, 1 = unusable pixel */
+  const uint8_t* support;  /* n, FFT order, the initial support of every start */
+  int starts;
+  int batch;           /* starts per launch group, must divide starts (0 = all) */
+  uint64_t seed;       /* random starting phases */
+  const float* start;  /* starts * n complex, FFT order, or NULL for random starts */
+  int storage;         /* QPCH_F32 / QPCH_F16 / QPCH_BF16 */
+  int mixed;           /* with F16/BF16: run refine (ER) iterations in F32 */
+  int schedule;        /* QPCH_UNFUSED / QPCH_FUSED / QPCH_GRAPH */
+} qpch_problem;
+
+typedef struct {
+  char name[256];
+  char arch[64];            /* gcnArchName, e.g. gfx950:sramecc+:xnack- */
+  int compute_units;
+  size_t memory_bytes;
+  size_t l2_bytes;
+  double peak_bandwidth_gbs; /* 2 x memory clock x bus width */
+} qpch_device_info_t;
+
+const char* qpch_last_error(void);
+
+int qpch_device_count(int* count);
+int qpch_device_info(qpch_device_info_t* info);
+
+/* --- solver ---------------------------------------------------------------------------- */
+
+int qpch_solver_create(const qpch_problem* problem, qpch_solver** out);
+void qpch_solver_destroy(qpch_solver* solver);
+
+/* Sizes the error history for `capacity` more iterations and resets its slot counter.
+ * Must precede qpch_solver_iterate; the history holds the FP64 numerators
+ * sum (|F x| - A)^2 of every start per iteration. */
+int qpch_solver_begin_run(qpch_solver* solver, int capacity);
+
+/* `iterations` iterations of x <- S ? P_M x : a x + b P_M x for every start.  refine = 1
+ * marks an ER stage (run in F32 by a mixed solver). */
+int qpch_solver_iterate(qpch_solver* solver, int iterations, float a, float b, int refine);
+
+/* method 0 = max, 1 = rms over the current support */
+int qpch_solver_shrinkwrap(qpch_solver* solver, float threshold, float sigma, int method);
+
+int qpch_solver_get_support(qpch_solver* solver, uint8_t* host /* starts * n */);
+int qpch_solver_set_support(qpch_solver* solver, const uint8_t* host);
+int qpch_solver_save_support(qpch_solver* solver);
+int qpch_solver_restore_support(qpch_solver* solver);
+
+/* errors[t * starts + s], t in [first, first + count) of the current run */
+int qpch_solver_read_errors(qpch_solver* solver, int first, int count, double* host);
+/* objects in FFT order, rescaled to the data (starts * n complex) */
+int qpch_solver_read_state(qpch_solver* solver, float* host);
+/* usable energy sum A^2 in the solver's scale: the error denominator */
+double qpch_solver_energy(const qpch_solver* solver);
+int qpch_solver_synchronize(qpch_solver* solver);
+
+/* --- operators on host arrays (synchronous) --------------------------------------------- */
+
+/* unitary n-D transform in place (complex, FFT order) */
+int qpch_fft(float* host, int ndim, const size_t* shape, int forward);
+/* x <- P_M x on an FFT-order object; *numerator = sum over usable pixels (|F x| - A)^2.
+ * amplitude is FFT order with masked pixels marked by mask. */
+int qpch_modulus_project(float* host_x, const float* amplitude, const uint8_t* mask, int ndim,
+                         const size_t* shape, double* numerator);
+int qpch_support_project(float* host_x, const uint8_t* support, size_t n);
+/* Gaussian blur with transfer exp(-2 pi^2 sigma^2 |f|^2) (real image, any origin) */
+int qpch_gaussian_blur(const float* host_in, float* host_out, int ndim, const size_t* shape,
+                       float sigma);
+
+/* --- device-pointer entry points (caller's stream) --------------------------------------- */
+
+/* Amplitude loss and its gradient for `batch` objects x (device, FFT order complex):
+ *   loss[b] = sum over usable pixels (|F x_b| - A)^2   (F unitary, FP64 accumulation)
+ *   grad_b  = 2 (x_b - P_M x_b)                         (the Wirtinger gradient, d/dRe + i d/dIm)
+ * precision 0: complex64 x/grad, float amplitude; 1: complex128 x/grad, double amplitude.
+ * amplitude is signed: a negative value marks a masked pixel.  stream: hipStream_t or NULL. */
+int qpch_amplitude_loss(const void* x, const void* amplitude, void* grad, double* loss,
+                        int ndim, const size_t* shape, int batch, int precision, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QPCH_H */

@@ -1,0 +1,252 @@
+"""ctypes binding of libqpch.so, the C ABI of the MI355X engine (include/qpch.h).
+
+The library is built in-tree by ``qoraphasecu_amd.build``.  Loading fails loudly when it
+is missing: there is no CPU fallback behind these functions.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqpch.so")
+
+QPCH_OK, QPCH_EINVAL, QPCH_EDEVICE = 0, 1, 2
+STORAGE = {"f32": 0, "f16": 1, "bf16": 2}
+SCHEDULE = {"unfused": 0, "fused": 1, "graph": 2}
+
+
+class DeviceError(RuntimeError):
+    """A device failure or a missing device (std::runtime_error in the C++ API)."""
+
+
+class QpchProblem(ctypes.Structure):
+    _fields_ = [
+        ("ndim", ctypes.c_int),
+        ("shape", ctypes.c_size_t * 3),
+        ("amplitude", ctypes.c_void_p),
+        ("mask", ctypes.c_void_p),
+        ("support", ctypes.c_void_p),
+        ("starts", ctypes.c_int),
+        ("batch", ctypes.c_int),
+        ("seed", ctypes.c_uint64),
+        ("start", ctypes.c_void_p),
+        ("storage", ctypes.c_int),
+        ("mixed", ctypes.c_int),
+        ("schedule", ctypes.c_int),
+    ]
+
+
+class QpchDeviceInfo(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 256),
+        ("arch", ctypes.c_char * 64),
+        ("compute_units", ctypes.c_int),
+        ("memory_bytes", ctypes.c_size_t),
+        ("l2_bytes", ctypes.c_size_t),
+        ("peak_bandwidth_gbs", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libqpch.so (raises when it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DeviceError(
+                f"{LIB_PATH} is missing: build it with qoraphasecu_amd.build.build() "
+                "(there is no CPU fallback)"
+            )
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i, f, d, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_double, ctypes.c_size_t
+        szp = ctypes.POINTER(ctypes.c_size_t)
+        L.qpch_last_error.restype = ctypes.c_char_p
+        L.qpch_device_count.argtypes = [ctypes.POINTER(i)]
+        L.qpch_device_info.argtypes = [ctypes.POINTER(QpchDeviceInfo)]
+        L.qpch_solver_create.argtypes = [ctypes.POINTER(QpchProblem), ctypes.POINTER(vp)]
+        L.qpch_solver_destroy.argtypes = [vp]
+        L.qpch_solver_destroy.restype = None
+        L.qpch_solver_begin_run.argtypes = [vp, i]
+        L.qpch_solver_iterate.argtypes = [vp, i, f, f, i]
+        L.qpch_solver_shrinkwrap.argtypes = [vp, f, f, i]
+        L.qpch_solver_get_support.argtypes = [vp, vp]
+        L.qpch_solver_set_support.argtypes = [vp, vp]
+        L.qpch_solver_save_support.argtypes = [vp]
+        L.qpch_solver_restore_support.argtypes = [vp]
+        L.qpch_solver_read_errors.argtypes = [vp, i, i, vp]
+        L.qpch_solver_read_state.argtypes = [vp, vp]
+        L.qpch_solver_energy.argtypes = [vp]
+        L.qpch_solver_energy.restype = d
+        L.qpch_solver_synchronize.argtypes = [vp]
+        L.qpch_fft.argtypes = [vp, i, szp, i]
+        L.qpch_modulus_project.argtypes = [vp, vp, vp, i, szp, ctypes.POINTER(d)]
+        L.qpch_support_project.argtypes = [vp, vp, sz]
+        L.qpch_gaussian_blur.argtypes = [vp, vp, i, szp, f]
+        L.qpch_amplitude_loss.argtypes = [vp, vp, vp, vp, i, szp, i, i, vp]
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    if status == QPCH_OK:
+        return
+    msg = lib().qpch_last_error().decode(errors="replace")
+    if status == QPCH_EINVAL:
+        raise ValueError(msg)
+    raise DeviceError(msg)
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _shape(shape):
+    shape = tuple(int(s) for s in shape)
+    return len(shape), (ctypes.c_size_t * len(shape))(*shape)
+
+
+def device_count() -> int:
+    c = ctypes.c_int(0)
+    check(lib().qpch_device_count(ctypes.byref(c)))
+    return c.value
+
+
+def device_info() -> dict:
+    info = QpchDeviceInfo()
+    check(lib().qpch_device_info(ctypes.byref(info)))
+    return {
+        "name": info.name.decode(),
+        "arch": info.arch.decode(),
+        "compute_units": info.compute_units,
+        "memory_bytes": info.memory_bytes,
+        "l2_bytes": info.l2_bytes,
+        "peak_bandwidth_gbs": info.peak_bandwidth_gbs,
+    }
+
+
+# --- operators on host arrays ------------------------------------------------------------
+
+
+def fft(x: np.ndarray, forward: bool = True) -> np.ndarray:
+    """Unitary n-D transform (FFT order, complex64) on the device."""
+    y = np.array(x, dtype=np.complex64, order="C", copy=True)
+    nd, sh = _shape(y.shape)
+    check(lib().qpch_fft(_ptr(y), nd, sh, int(forward)))
+    return y
+
+
+def modulus_project(x, amplitude, mask):
+    """(P_M x, numerator) on FFT-order arrays."""
+    y = np.array(x, dtype=np.complex64, order="C", copy=True)
+    a = np.ascontiguousarray(amplitude, dtype=np.float32)
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    nd, sh = _shape(y.shape)
+    num = ctypes.c_double(0.0)
+    check(lib().qpch_modulus_project(_ptr(y), _ptr(a), _ptr(m), nd, sh, ctypes.byref(num)))
+    return y, num.value
+
+
+def support_project(x, support):
+    y = np.array(x, dtype=np.complex64, order="C", copy=True)
+    s = np.ascontiguousarray(support, dtype=np.uint8)
+    check(lib().qpch_support_project(_ptr(y), _ptr(s), y.size))
+    return y
+
+
+def gaussian_blur(image, sigma):
+    im = np.ascontiguousarray(image, dtype=np.float32)
+    out = np.empty_like(im)
+    nd, sh = _shape(im.shape)
+    check(lib().qpch_gaussian_blur(_ptr(im), _ptr(out), nd, sh, float(sigma)))
+    return out
+
+
+# --- solver ------------------------------------------------------------------------------
+
+
+class Solver:
+    """A device solver over FFT-order inputs (what the C++ HipSolver drives)."""
+
+    def __init__(self, amplitude, mask, support, starts=1, batch=0, seed=0, start=None,
+                 storage="f32", mixed=False, schedule="graph"):
+        self.shape = tuple(amplitude.shape)
+        self.n = int(np.prod(self.shape))
+        self.starts = starts
+        self._keep = [
+            np.ascontiguousarray(amplitude, dtype=np.float32),
+            np.ascontiguousarray(mask, dtype=np.uint8),
+            np.ascontiguousarray(support, dtype=np.uint8),
+            None if start is None else np.ascontiguousarray(start, dtype=np.complex64),
+        ]
+        p = QpchProblem()
+        p.ndim = len(self.shape)
+        for k, s in enumerate(self.shape):
+            p.shape[k] = s
+        p.amplitude = _ptr(self._keep[0])
+        p.mask = _ptr(self._keep[1])
+        p.support = _ptr(self._keep[2])
+        p.starts = starts
+        p.batch = batch
+        p.seed = seed
+        p.start = _ptr(self._keep[3])
+        p.storage = STORAGE[storage]
+        p.mixed = int(mixed)
+        p.schedule = SCHEDULE[schedule]
+        h = ctypes.c_void_p()
+        check(lib().qpch_solver_create(ctypes.byref(p), ctypes.byref(h)))
+        self._h = h
+        self.energy = lib().qpch_solver_energy(self._h)
+
+    def begin_run(self, capacity):
+        check(lib().qpch_solver_begin_run(self._h, capacity))
+
+    def iterate(self, n, a, b, refine=False):
+        check(lib().qpch_solver_iterate(self._h, n, a, b, int(refine)))
+
+    def shrinkwrap(self, threshold=0.3, sigma=1.0, method="max"):
+        check(lib().qpch_solver_shrinkwrap(self._h, threshold, sigma, 0 if method == "max" else 1))
+
+    def get_support(self):
+        out = np.empty((self.starts, *self.shape), np.uint8)
+        check(lib().qpch_solver_get_support(self._h, _ptr(out)))
+        return out
+
+    def set_support(self, s):
+        s = np.ascontiguousarray(s, dtype=np.uint8)
+        check(lib().qpch_solver_set_support(self._h, _ptr(s)))
+
+    def save_support(self):
+        check(lib().qpch_solver_save_support(self._h))
+
+    def restore_support(self):
+        check(lib().qpch_solver_restore_support(self._h))
+
+    def read_errors(self, first, count):
+        out = np.empty((count, self.starts), np.float64)
+        check(lib().qpch_solver_read_errors(self._h, first, count, _ptr(out)))
+        return out
+
+    def read_state(self):
+        out = np.empty((self.starts, *self.shape), np.complex64)
+        check(lib().qpch_solver_read_state(self._h, _ptr(out)))
+        return out
+
+    def synchronize(self):
+        check(lib().qpch_solver_synchronize(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().qpch_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,944 @@
+// The MI355X reconstruction engine behind the qpch C ABI: device buffers, the per-shape
+// pass schedule, hipGraph replay of iteration chunks, Shrinkwrap and the operators.
+//
+// Data layout in HBM (per solver): x and the wave are [starts][n] in FFT order, row-major
+// (complex64, or half2 / bfloat162 storage); the support is [starts][n] uint8; the signed
+// amplitude (-1 = masked) and the true amplitude are [n] float shared by every start;
+// the error history is [slot][starts] FP64 numerators.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../../include/qpch.h"
+#include "generic.hpp"
+#include "misc.hpp"
+#include "passes.hpp"
+
+namespace qpch {
+
+using dev::Mode;
+using dev::PassArgs;
+
+namespace {
+
+thread_local std::string g_error;
+
+struct InvalidArgument : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct DeviceError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define QPCH_HIP(expr)                                                                       \
+  do {                                                                                     \
+    const hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) {                                                                \
+      throw ::qpch::DeviceError(std::string(#expr) + ": " + hipGetErrorString(e_) + " (" +       \
+                        __FILE__ + ":" + std::to_string(__LINE__) + ")");                 \
+    }                                                                                      \
+  } while (0)
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return QPCH_OK;
+  } catch (const InvalidArgument& e) {
+    g_error = e.what();
+    return QPCH_EINVAL;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return QPCH_EDEVICE;
+  } catch (...) {
+    g_error = "unknown error";
+    return QPCH_EDEVICE;
+  }
+}
+
+template <class T>
+class DevBuf {
+ public:
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { resize(count); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    std::swap(p_, o.p_);
+    std::swap(n_, o.n_);
+    return *this;
+  }
+  ~DevBuf() { if (p_) (void)hipFree(p_); }
+  void resize(size_t count) {
+    if (p_) { (void)hipFree(p_); p_ = nullptr; }
+    n_ = count;
+    if (count) QPCH_HIP(hipMalloc(&p_, count * sizeof(T)));
+  }
+  T* get() const { return p_; }
+  size_t size() const { return n_; }
+
+ private:
+  T* p_ = nullptr;
+  size_t n_ = 0;
+};
+
+template <class T>
+DevBuf<T> upload(const T* host, size_t count) {
+  DevBuf<T> d(count);
+  if (count) QPCH_HIP(hipMemcpy(d.get(), host, count * sizeof(T), hipMemcpyHostToDevice));
+  return d;
+}
+
+bool is_pow2(long long v) { return v >= 1 && (v & (v - 1)) == 0; }
+
+// ---------------------------------------------------------------------------------------
+// geometry
+
+struct Dims {
+  int nd = 0;
+  int d[3] = {1, 1, 1};
+  long long n = 0;
+  // axis a: lines of length d[a], `inner` apart, `outer` blocks of them
+  long long inner(int a) const {
+    long long v = 1;
+    for (int b = a + 1; b < nd; ++b) v *= d[b];
+    return v;
+  }
+  long long outer(int a) const {
+    long long v = 1;
+    for (int b = 0; b < a; ++b) v *= d[b];
+    return v;
+  }
+};
+
+Dims make_dims(int ndim, const size_t* shape) {
+  if (ndim != 2 && ndim != 3) throw InvalidArgument("shape must be 2D or 3D");
+  Dims d;
+  d.nd = ndim;
+  d.n = 1;
+  for (int a = 0; a < ndim; ++a) {
+    if (shape[a] == 0) throw InvalidArgument("shape has a zero-length axis");
+    if (shape[a] > (1u << 20)) throw InvalidArgument("axis too long");
+    d.d[a] = static_cast<int>(shape[a]);
+    d.n *= static_cast<long long>(shape[a]);
+  }
+  return d;
+}
+
+constexpr int kMaxFused = 4096;
+
+bool fusable(const Dims& d) {
+  for (int a = 0; a < d.nd; ++a) {
+    if (!is_pow2(d.d[a]) || d.d[a] < 2 || d.d[a] > kMaxFused) return false;
+  }
+  return true;
+}
+
+int grid_for(long long work, int threads = 256, int cap = 2048) {
+  long long g = (work + threads - 1) / threads;
+  return static_cast<int>(std::max<long long>(1, std::min<long long>(g, cap)));
+}
+
+// ---------------------------------------------------------------------------------------
+// fused line-pass dispatch
+
+template <int N, bool COL, Mode M, class St>
+void launch_pass_t(const PassArgs& a, int starts, hipStream_t s) {
+  using G = dev::PassGeom<N, COL>;
+  const int gx = COL ? a.outer * ((a.inner + G::kB - 1) / G::kB) : (a.outer + G::kB - 1) / G::kB;
+  hipLaunchKernelGGL((dev::line_pass<N, COL, M, St>), dim3(gx, starts), dim3(G::kThreads), 0, s,
+                     a);
+  QPCH_HIP(hipGetLastError());
+}
+
+template <int N, bool COL>
+int pass_blocks_t(int outer, int inner) {
+  using G = dev::PassGeom<N, COL>;
+  return COL ? outer * ((inner + G::kB - 1) / G::kB) : (outer + G::kB - 1) / G::kB;
+}
+
+template <int N, class St>
+void launch_pass_n(bool col, Mode m, const PassArgs& a, int starts, hipStream_t s) {
+  if (col) {
+    switch (m) {
+      case Mode::kFwd: return launch_pass_t<N, true, Mode::kFwd, St>(a, starts, s);
+      case Mode::kInv: return launch_pass_t<N, true, Mode::kInv, St>(a, starts, s);
+      case Mode::kModulus: return launch_pass_t<N, true, Mode::kModulus, St>(a, starts, s);
+      case Mode::kUpdate: return launch_pass_t<N, true, Mode::kUpdate, St>(a, starts, s);
+    }
+  } else {
+    switch (m) {
+      case Mode::kFwd: return launch_pass_t<N, false, Mode::kFwd, St>(a, starts, s);
+      case Mode::kInv: return launch_pass_t<N, false, Mode::kInv, St>(a, starts, s);
+      case Mode::kModulus: return launch_pass_t<N, false, Mode::kModulus, St>(a, starts, s);
+      case Mode::kUpdate: return launch_pass_t<N, false, Mode::kUpdate, St>(a, starts, s);
+    }
+  }
+}
+
+#define QPCH_FOR_EACH_N(X) \
+  X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
+
+template <class St>
+void launch_pass(int N, bool col, Mode m, const PassArgs& a, int starts, hipStream_t s) {
+  switch (N) {
+#define QPCH_CASE(V) \
+  case V:            \
+    return launch_pass_n<V, St>(col, m, a, starts, s);
+    QPCH_FOR_EACH_N(QPCH_CASE)
+#undef QPCH_CASE
+    default:
+      throw InvalidArgument("no fused pass for axis length " + std::to_string(N));
+  }
+}
+
+int pass_blocks(int N, bool col, int outer, int inner) {
+  switch (N) {
+#define QPCH_CASE(V) \
+  case V:            \
+    return col ? pass_blocks_t<V, true>(outer, inner) : pass_blocks_t<V, false>(outer, inner);
+    QPCH_FOR_EACH_N(QPCH_CASE)
+#undef QPCH_CASE
+    default:
+      throw InvalidArgument("no fused pass for axis length " + std::to_string(N));
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// generic (any length) transforms
+
+dev::GenericPlan generic_plan(int n) {
+  dev::GenericPlan p{};
+  p.n = n;
+  int rest = n, nf = 0;
+  bool ok = true;
+  while (rest % 4 == 0) { p.fac[nf++] = 4; rest /= 4; }
+  while (rest % 2 == 0) { p.fac[nf++] = 2; rest /= 2; }
+  for (int f = 3; rest > 1 && ok; f += 2) {
+    while (rest % f == 0) {
+      if (f > 64 || nf >= 32) { ok = false; break; }
+      p.fac[nf++] = f;
+      rest /= f;
+    }
+    if (ok && rest > 1 && f * f > rest) {
+      if (rest > 64 || nf >= 32) { ok = false; break; }
+      p.fac[nf++] = rest;
+      rest = 1;
+    }
+  }
+  p.nfac = ok ? nf : 0;
+  return p;
+}
+
+void launch_generic(float2* data, long long n_per_start, int batch, int N, long long outer,
+                    long long inner, bool forward, float scale, hipStream_t s) {
+  if (N > dev::kGenericMaxN) {
+    throw InvalidArgument("axis length " + std::to_string(N) +
+                          " is neither a power of two <= 4096 nor <= 2048");
+  }
+  const dev::GenericPlan plan = generic_plan(N);
+  const long long lines = outer * inner;
+  const size_t lds = sizeof(float2) * 3 * static_cast<size_t>(N);
+  hipLaunchKernelGGL(dev::generic_fft, dim3(static_cast<unsigned>(lines), batch),
+                     dim3(dev::kGenericThreads), lds, s, data, n_per_start,
+                     static_cast<int>(inner), plan, forward ? 1 : 0, scale);
+  QPCH_HIP(hipGetLastError());
+}
+
+void scale_inplace(float2* data, long long total, float scale, hipStream_t s) {
+  hipLaunchKernelGGL((dev::convert<float2, float2>), dim3(grid_for(total)), dim3(256), 0, s, data,
+                     data, total, scale, nullptr);
+  QPCH_HIP(hipGetLastError());
+}
+
+// Unitary (scale = 1/sqrt(n)) or unnormalised (scale = 1) n-D transform in place, fp32.
+void device_fft(float2* data, const Dims& d, int batch, bool forward, float scale,
+                hipStream_t s) {
+  bool scaled = false;
+  for (int a = d.nd - 1; a >= 0; --a) {
+    const int N = d.d[a];
+    if (N == 1) continue;
+    const bool last_axis = [&] {
+      for (int b = a - 1; b >= 0; --b)
+        if (d.d[b] > 1) return false;
+      return true;
+    }();
+    const float sc = last_axis ? scale : 1.0f;
+    scaled = scaled || last_axis;
+    const long long inner = d.inner(a), outer = d.outer(a);
+    if (is_pow2(N) && N <= kMaxFused) {
+      PassArgs p{};
+      p.outer = static_cast<int>(outer);
+      p.inner = static_cast<int>(inner);
+      p.n = d.n;
+      p.w = data;
+      p.out_scale = sc;
+      launch_pass<float2>(N, inner > 1, forward ? Mode::kFwd : Mode::kInv, p, batch, s);
+    } else {
+      launch_generic(data, d.n, batch, N, outer, inner, forward, sc, s);
+    }
+  }
+  if (!scaled && scale != 1.0f) scale_inplace(data, d.n * batch, scale, s);
+}
+
+// ---------------------------------------------------------------------------------------
+// Shrinkwrap blur taps: h[d] = (1/N) sum_j exp(k f_j^2) cos(2 pi j d / N), the inverse
+// DFT of the axis's Gaussian transfer (real and symmetric), k = -2 pi^2 sigma^2.
+
+struct Taps {
+  std::vector<float> taps;
+  int J = 0;  // offset of tap 0
+};
+
+Taps blur_taps(int N, float sigma) {
+  const double pi = 3.14159265358979323846;
+  const double k = -2.0 * pi * pi * static_cast<double>(sigma) * static_cast<double>(sigma);
+  std::vector<double> g(N), h(N);
+  for (int j = 0; j < N; ++j) {
+    const int s = j < (N + 1) / 2 ? j : j - N;
+    const double f = static_cast<double>(s) / N;
+    g[j] = std::exp(k * f * f);
+  }
+  for (int dd = 0; dd < N; ++dd) {
+    double acc = 0.0;
+    for (int j = 0; j < N; ++j) acc += g[j] * std::cos(2.0 * pi * static_cast<double>((static_cast<long long>(j) * dd) % N) / N);
+    h[dd] = acc / N;
+  }
+  // keep |h[d]| > 1e-12 h[0] around d = 0; the rest cannot change a float sum
+  int J = 0;
+  for (int dd = 1; dd <= N / 2; ++dd) {
+    if (std::fabs(h[dd]) > 1e-12 * std::fabs(h[0]) || std::fabs(h[N - dd]) > 1e-12 * std::fabs(h[0])) J = dd;
+  }
+  Taps t;
+  if (2 * J + 1 >= N) {  // the kernel spans the axis: full circular convolution
+    t.J = 0;
+    t.taps.resize(N);
+    for (int dd = 0; dd < N; ++dd) t.taps[dd] = static_cast<float>(h[(N - dd) % N]);
+    // tap t multiplies in[(k + t) mod N]: weight h[(k - (k + t)) mod N] = h[(-t) mod N]
+  } else {
+    t.J = J;
+    t.taps.resize(2 * J + 1);
+    for (int q = 0; q < 2 * J + 1; ++q) t.taps[q] = static_cast<float>(h[(J - q + N) % N]);
+  }
+  return t;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// the solver
+
+struct SolverImpl {
+  Dims dims;
+  long long n = 0;
+  int starts = 1, batch = 1, groups = 1;
+  int storage = QPCH_F32;
+  bool mixed = false;
+  int schedule = QPCH_GRAPH;
+  bool fused = true;
+  hipStream_t stream = nullptr;
+  float amp_scale = 1.0f;
+  double energy = 0.0;
+
+  DevBuf<float> amp_signed, amp_true;
+  DevBuf<std::uint8_t> supp, saved;
+  DevBuf<float2> x32, w32;
+  DevBuf<std::uint32_t> x16, w16;  // half2 / bfloat162 storage, 4 bytes per element
+  DevBuf<double> partials, errs, sw_part;
+  DevBuf<int> counters;
+  DevBuf<float> sw_a, sw_b;
+  int partial_stride = 1;
+  int err_cap = 0;
+  bool low_active = false;
+  bool wave_valid = false;
+  std::map<std::tuple<int, float, float>, std::pair<int, DevBuf<float>>> taps_cache;  // (N, sigma) -> (J, taps)
+  std::map<std::tuple<int, int, float, float, bool>, hipGraphExec_t> graphs;
+
+  ~SolverImpl() {
+    clear_graphs();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void clear_graphs() {
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
+  }
+
+  bool low() const { return storage != QPCH_F32; }
+
+  template <class St>
+  St* xp(int first) const {
+    if constexpr (std::is_same_v<St, float2>) return x32.get() + first * n;
+    else return reinterpret_cast<St*>(x16.get()) + first * n;
+  }
+  template <class St>
+  St* wp(int first) const {
+    if constexpr (std::is_same_v<St, float2>) return w32.get() + first * n;
+    else return reinterpret_cast<St*>(w16.get()) + first * n;
+  }
+
+  // --- fused schedule: the passes of one iteration and of the prologue -------------------
+  struct PassSpec {
+    int axis;
+    Mode mode;
+    bool from_x;  // prologue: read x instead of the wave
+    float in_scale, out_scale;
+  };
+
+  std::vector<PassSpec> iteration_passes(bool lowp) const {
+    const int nd = dims.nd;
+    std::vector<PassSpec> v;
+    const float s = static_cast<float>(1.0 / std::sqrt(static_cast<double>(n)));
+    auto ax = [&](int a) { return lowp ? static_cast<float>(1.0 / std::sqrt(static_cast<double>(dims.d[a]))) : 1.0f; };
+    const float ms = lowp ? ax(0) : s;
+    v.push_back({0, Mode::kModulus, false, ms, ms});
+    if (nd == 3) v.push_back({1, Mode::kInv, false, 1.0f, ax(1)});
+    v.push_back({nd - 1, Mode::kUpdate, false, ax(nd - 1), ax(nd - 1)});
+    if (nd == 3) v.push_back({1, Mode::kFwd, false, 1.0f, ax(1)});
+    return v;
+  }
+
+  std::vector<PassSpec> prologue_passes(bool lowp) const {
+    const int nd = dims.nd;
+    auto ax = [&](int a) { return lowp ? static_cast<float>(1.0 / std::sqrt(static_cast<double>(dims.d[a]))) : 1.0f; };
+    std::vector<PassSpec> v;
+    v.push_back({nd - 1, Mode::kFwd, true, 1.0f, ax(nd - 1)});
+    if (nd == 3) v.push_back({1, Mode::kFwd, false, 1.0f, ax(1)});
+    return v;
+  }
+
+  template <class St>
+  void run_pass(const PassSpec& ps, int g, int count, float a, float b) {
+    const int first = g * batch;
+    PassArgs p{};
+    const long long inner = dims.inner(ps.axis), outer = dims.outer(ps.axis);
+    p.outer = static_cast<int>(outer);
+    p.inner = static_cast<int>(inner);
+    p.n = n;
+    p.w = wp<St>(first);
+    p.src = ps.from_x ? static_cast<const void*>(xp<St>(first)) : nullptr;
+    p.x = xp<St>(first);
+    p.support = supp.get() + first * n;
+    p.amp = amp_signed.get();
+    p.partials = partials.get();
+    p.partial_stride = partial_stride;
+    p.errs = errs.get();
+    p.err_stride = starts;
+    p.err_offset = first;
+    p.err_cap = err_cap;
+    p.counter = counters.get() + g;
+    p.a = a;
+    p.b = b;
+    p.in_scale = ps.in_scale;
+    p.out_scale = ps.out_scale;
+    launch_pass<St>(dims.d[ps.axis], inner > 1, ps.mode, p, count, stream);
+  }
+
+  template <class St>
+  void fused_prologue(int g) {
+    for (const auto& ps : prologue_passes(!std::is_same_v<St, float2>)) run_pass<St>(ps, g, batch, 0, 0);
+  }
+
+  template <class St>
+  void fused_iterations(int g, int len, float a, float b) {
+    const auto passes = iteration_passes(!std::is_same_v<St, float2>);
+    for (int it = 0; it < len; ++it) {
+      for (const auto& ps : passes) run_pass<St>(ps, g, batch, a, b);
+    }
+  }
+
+  // --- unfused schedule (any shape, fp32) -------------------------------------------------
+  void unfused_iterations(int g, int len, float a, float b) {
+    const int first = g * batch;
+    float2* x = x32.get() + first * n;
+    float2* w = w32.get() + first * n;
+    const float s = static_cast<float>(1.0 / std::sqrt(static_cast<double>(n)));
+    const int mgrid = grid_for(n, 256, 1024);
+    for (int it = 0; it < len; ++it) {
+      QPCH_HIP(hipMemcpyAsync(w, x, sizeof(float2) * n * batch, hipMemcpyDeviceToDevice, stream));
+      device_fft(w, dims, batch, true, 1.0f, stream);
+      hipLaunchKernelGGL(dev::modulus_elementwise, dim3(mgrid, batch), dim3(256), 0, stream, w,
+                         amp_signed.get(), n, s, s, partials.get(), counters.get() + g);
+      QPCH_HIP(hipGetLastError());
+      device_fft(w, dims, batch, false, 1.0f, stream);
+      hipLaunchKernelGGL(dev::update_elementwise, dim3(grid_for(n, 256, 1024), batch), dim3(256), 0,
+                         stream, x, w, supp.get() + first * n, n, a, b, 1.0f, partials.get(), mgrid,
+                         errs.get(), starts, first, err_cap, counters.get() + g);
+      QPCH_HIP(hipGetLastError());
+    }
+  }
+
+  // --- precision switching -----------------------------------------------------------------
+  void convert_x(bool to_low) {
+    const long long total = n * starts;
+    if (to_low) {
+      if (storage == QPCH_F16) {
+        hipLaunchKernelGGL((dev::convert<float2, __half2>), dim3(grid_for(total)), dim3(256), 0,
+                           stream, x32.get(), reinterpret_cast<__half2*>(x16.get()), total, 1.0f,
+                           nullptr);
+      } else {
+        hipLaunchKernelGGL((dev::convert<float2, __hip_bfloat162>), dim3(grid_for(total)),
+                           dim3(256), 0, stream, x32.get(),
+                           reinterpret_cast<__hip_bfloat162*>(x16.get()), total, 1.0f, nullptr);
+      }
+    } else {
+      if (storage == QPCH_F16) {
+        hipLaunchKernelGGL((dev::convert<__half2, float2>), dim3(grid_for(total)), dim3(256), 0,
+                           stream, reinterpret_cast<const __half2*>(x16.get()), x32.get(), total,
+                           1.0f, nullptr);
+      } else {
+        hipLaunchKernelGGL((dev::convert<__hip_bfloat162, float2>), dim3(grid_for(total)),
+                           dim3(256), 0, stream,
+                           reinterpret_cast<const __hip_bfloat162*>(x16.get()), x32.get(), total,
+                           1.0f, nullptr);
+      }
+    }
+    QPCH_HIP(hipGetLastError());
+  }
+
+  void set_precision(bool lowp) {
+    if (lowp == low_active) return;
+    convert_x(lowp);
+    low_active = lowp;
+    wave_valid = false;
+  }
+
+  // --- iteration chunk ---------------------------------------------------------------------
+  template <class St>
+  void chunk(int g, int len, float a, float b, bool need_prologue) {
+    if (need_prologue) fused_prologue<St>(g);
+    if (schedule != QPCH_GRAPH) {
+      fused_iterations<St>(g, len, a, b);
+      return;
+    }
+    const auto key = std::make_tuple(g, len, a, b, !std::is_same_v<St, float2>);
+    auto it = graphs.find(key);
+    if (it == graphs.end()) {
+      QPCH_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      try {
+        fused_iterations<St>(g, len, a, b);
+      } catch (...) {
+        hipGraph_t dummy{};
+        (void)hipStreamEndCapture(stream, &dummy);
+        if (dummy) (void)hipGraphDestroy(dummy);
+        throw;
+      }
+      hipGraph_t graph{};
+      QPCH_HIP(hipStreamEndCapture(stream, &graph));
+      hipGraphExec_t exec{};
+      const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      QPCH_HIP(e);
+      it = graphs.emplace(key, exec).first;
+    }
+    QPCH_HIP(hipGraphLaunch(it->second, stream));
+  }
+
+  void iterate(int len, float a, float b, bool refine) {
+    if (len < 1) throw InvalidArgument("iterations must be >= 1");
+    if (err_cap <= 0) throw InvalidArgument("qpch_solver_begin_run must precede iterate");
+    const bool lowp = low() && !(mixed && refine);
+    set_precision(lowp);
+    const bool need_prologue = !wave_valid;
+    for (int g = 0; g < groups; ++g) {
+      if (!fused) {
+        unfused_iterations(g, len, a, b);
+        continue;
+      }
+      if (!lowp) {
+        chunk<float2>(g, len, a, b, need_prologue);
+      } else if (storage == QPCH_F16) {
+        chunk<__half2>(g, len, a, b, need_prologue);
+      } else {
+        chunk<__hip_bfloat162>(g, len, a, b, need_prologue);
+      }
+    }
+    wave_valid = fused;
+  }
+
+  void begin_run(int capacity) {
+    if (capacity < 1) throw InvalidArgument("capacity must be >= 1");
+    const size_t need = static_cast<size_t>(capacity) * starts;
+    if (errs.size() < need) {
+      clear_graphs();  // captured graphs hold the history's address
+      errs.resize(need);
+    }
+    err_cap = capacity;
+    hipLaunchKernelGGL(dev::fill_double, dim3(grid_for(static_cast<long long>(errs.size()))), dim3(256), 0,
+                       stream, errs.get(), static_cast<long long>(errs.size()), 0.0);
+    hipLaunchKernelGGL(dev::fill_int, dim3(1), dim3(256), 0, stream, counters.get(), groups, 0);
+    QPCH_HIP(hipGetLastError());
+  }
+
+  // --- Shrinkwrap ------------------------------------------------------------------------
+  const std::pair<int, DevBuf<float>>& taps_for(int N, float sigma) {
+    const auto key = std::make_tuple(N, sigma, 0.0f);
+    auto it = taps_cache.find(key);
+    if (it == taps_cache.end()) {
+      Taps t = blur_taps(N, sigma);
+      DevBuf<float> d = upload(t.taps.data(), t.taps.size());
+      it = taps_cache.emplace(key, std::make_pair(t.J, std::move(d))).first;
+    }
+    return it->second;
+  }
+
+  void shrinkwrap(float threshold, float sigma, int method) {
+    if (!(threshold > 0.0f) || !(sigma >= 0.0f)) throw InvalidArgument("Shrinkwrap needs threshold > 0, sigma >= 0");
+    const long long total = n * starts;
+    if (sw_a.size() < static_cast<size_t>(total)) {
+      sw_a.resize(total);
+      sw_b.resize(total);
+    }
+    const int nd = dims.nd;
+    float* cur = nullptr;
+    bool first = true;
+    for (int a = nd - 1; a >= 0; --a) {
+      const int N = dims.d[a];
+      const auto& tp = taps_for(N, sigma);
+      dev::BlurAxis ax{N, static_cast<int>(dims.inner(a)), static_cast<int>(tp.second.size()), tp.first,
+                       tp.second.get()};
+      float* out = (cur == sw_a.get()) ? sw_b.get() : sw_a.get();
+      const dim3 grid(grid_for(n, 256, 2048), starts);
+      if (first) {
+        if (!low_active) {
+          hipLaunchKernelGGL((dev::blur_axis<float2, true>), grid, dim3(256), 0, stream, x32.get(), out, n, ax);
+        } else if (storage == QPCH_F16) {
+          hipLaunchKernelGGL((dev::blur_axis<__half2, true>), grid, dim3(256), 0, stream, x16.get(), out, n, ax);
+        } else {
+          hipLaunchKernelGGL((dev::blur_axis<__hip_bfloat162, true>), grid, dim3(256), 0, stream, x16.get(), out, n, ax);
+        }
+      } else {
+        hipLaunchKernelGGL((dev::blur_axis<float2, false>), grid, dim3(256), 0, stream, cur, out, n, ax);
+      }
+      QPCH_HIP(hipGetLastError());
+      cur = out;
+      first = false;
+    }
+    const int blocks = grid_for(n, 256, 512);
+    if (sw_part.size() < static_cast<size_t>(blocks) * starts * 2) sw_part.resize(static_cast<size_t>(blocks) * starts * 2);
+    hipLaunchKernelGGL(dev::shrinkwrap_level, dim3(blocks, starts), dim3(256), 0, stream, cur, supp.get(), n,
+                       method, sw_part.get());
+    hipLaunchKernelGGL(dev::shrinkwrap_threshold, dim3(grid_for(n, 256, 2048), starts), dim3(256), 0, stream,
+                       cur, sw_part.get(), blocks, n, method, threshold, supp.get());
+    QPCH_HIP(hipGetLastError());
+  }
+
+  // --- set-up ------------------------------------------------------------------------------
+  void init(const qpch_problem& p) {
+    dims = make_dims(p.ndim, p.shape);
+    n = dims.n;
+    if (p.starts < 1) throw InvalidArgument("starts must be >= 1");
+    starts = p.starts;
+    batch = p.batch ? p.batch : p.starts;
+    if (batch < 1 || starts % batch != 0) throw InvalidArgument("batch must divide starts");
+    groups = starts / batch;
+    storage = p.storage;
+    if (storage != QPCH_F32 && storage != QPCH_F16 && storage != QPCH_BF16) throw InvalidArgument("unknown storage");
+    mixed = p.mixed != 0 && storage != QPCH_F32;
+    schedule = p.schedule;
+    if (schedule < QPCH_UNFUSED || schedule > QPCH_GRAPH) throw InvalidArgument("unknown schedule");
+    fused = fusable(dims) && schedule != QPCH_UNFUSED;
+    if (storage != QPCH_F32 && !fusable(dims)) {
+      throw InvalidArgument("fp16/bf16 storage needs power-of-two extents between 2 and 4096");
+    }
+    if (storage != QPCH_F32 && schedule == QPCH_UNFUSED) throw InvalidArgument("fp16/bf16 storage needs a fused schedule");
+    for (int a = 0; a < dims.nd; ++a) {
+      if (!(is_pow2(dims.d[a]) && dims.d[a] <= kMaxFused) && dims.d[a] > dev::kGenericMaxN) {
+        throw InvalidArgument("axis length " + std::to_string(dims.d[a]) + " is not supported");
+      }
+    }
+    if (!p.amplitude || !p.mask || !p.support) throw InvalidArgument("amplitude, mask and support are required");
+    QPCH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+
+    std::vector<float> amp(p.amplitude, p.amplitude + n);
+    if (storage != QPCH_F32) {
+      // keep the unnormalised spectra inside fp16 range: peak amplitude 1
+      const float peak = *std::max_element(amp.begin(), amp.end());
+      amp_scale = peak > 0.0f ? peak : 1.0f;
+      for (float& v : amp) v /= amp_scale;
+    }
+    std::vector<float> signed_amp(n);
+    double e = 0.0;
+    for (long long i = 0; i < n; ++i) {
+      signed_amp[i] = p.mask[i] ? -1.0f : amp[i];
+      if (!p.mask[i]) e += static_cast<double>(amp[i]) * amp[i];
+    }
+    energy = e;
+    amp_true = upload(amp.data(), n);
+    amp_signed = upload(signed_amp.data(), n);
+    std::vector<std::uint8_t> sup(static_cast<size_t>(n) * starts);
+    for (int s = 0; s < starts; ++s) {
+      for (long long i = 0; i < n; ++i) sup[s * n + i] = p.support[i] != 0;
+    }
+    supp = upload(sup.data(), sup.size());
+    saved.resize(sup.size());
+    x32.resize(static_cast<size_t>(n) * starts);
+    w32.resize(static_cast<size_t>(n) * starts);
+    if (storage != QPCH_F32) {
+      x16.resize(static_cast<size_t>(n) * starts);
+      w16.resize(static_cast<size_t>(n) * starts);
+    }
+    counters.resize(groups);
+    // partial sums of the error: one per work-group of the modulus pass
+    int pb = grid_for(n, 256, 1024);
+    if (fused) pb = std::max(pb, pass_blocks(dims.d[0], dims.inner(0) > 1, static_cast<int>(dims.outer(0)), static_cast<int>(dims.inner(0))));
+    partial_stride = fused ? pass_blocks(dims.d[0], dims.inner(0) > 1, static_cast<int>(dims.outer(0)),
+                                         static_cast<int>(dims.inner(0)))
+                           : grid_for(n, 256, 1024);
+    partials.resize(static_cast<size_t>(pb) * batch);
+
+    // starting objects
+    if (p.start) {
+      std::vector<float> host(p.start, p.start + 2 * n * starts);
+      if (amp_scale != 1.0f) for (float& v : host) v /= amp_scale;
+      QPCH_HIP(hipMemcpy(w32.get(), host.data(), sizeof(float2) * n * starts, hipMemcpyHostToDevice));
+    } else {
+      for (int g = 0; g < groups; ++g) {
+        float2* w = w32.get() + static_cast<long long>(g) * batch * n;
+        hipLaunchKernelGGL(dev::random_wave, dim3(grid_for(n, 256, 2048), batch), dim3(256), 0, stream, w,
+                           amp_true.get(), n, p.seed, static_cast<std::uint64_t>(g) * batch);
+        QPCH_HIP(hipGetLastError());
+        device_fft(w, dims, batch, false, static_cast<float>(1.0 / std::sqrt(static_cast<double>(n))), stream);
+      }
+    }
+    hipLaunchKernelGGL((dev::convert<float2, float2>), dim3(grid_for(n * starts)), dim3(256), 0, stream,
+                       w32.get(), x32.get(), n * starts, 1.0f, supp.get());
+    QPCH_HIP(hipGetLastError());
+    low_active = false;
+    wave_valid = false;
+    QPCH_HIP(hipStreamSynchronize(stream));
+  }
+
+  void read_state(float* host) {
+    if (low_active) convert_x(false);
+    QPCH_HIP(hipStreamSynchronize(stream));
+    if (low_active) {
+      // x32 now mirrors the low-precision authority; it stays valid until the next
+      // low-precision iteration converts back.
+    }
+    std::vector<float> tmp;
+    QPCH_HIP(hipMemcpy(host, x32.get(), sizeof(float2) * n * starts, hipMemcpyDeviceToHost));
+    if (amp_scale != 1.0f) {
+      for (long long i = 0; i < 2 * n * starts; ++i) host[i] *= amp_scale;
+    }
+  }
+};
+
+}  // namespace qpch
+
+struct qpch_solver {
+  qpch::SolverImpl impl;
+};
+
+using qpch::guarded;
+
+extern "C" {
+
+const char* qpch_last_error(void) { return qpch::g_error.c_str(); }
+
+int qpch_device_count(int* count) {
+  return guarded([&] {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *count = c;
+  });
+}
+
+int qpch_device_info(qpch_device_info_t* info) {
+  return guarded([&] {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess || c == 0) throw qpch::DeviceError("no HIP device available");
+    int dev = 0;
+    QPCH_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop{};
+    QPCH_HIP(hipGetDeviceProperties(&prop, dev));
+    std::memset(info, 0, sizeof(*info));
+    std::snprintf(info->name, sizeof(info->name), "%s", prop.name);
+    std::snprintf(info->arch, sizeof(info->arch), "%s", prop.gcnArchName);
+    info->compute_units = prop.multiProcessorCount;
+    info->memory_bytes = prop.totalGlobalMem;
+    info->l2_bytes = static_cast<size_t>(prop.l2CacheSize);
+    int clock_khz = 0;
+    QPCH_HIP(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeMemoryClockRate, dev));
+    info->peak_bandwidth_gbs = 2.0 * clock_khz * 1e3 * (prop.memoryBusWidth / 8.0) / 1e9;
+  });
+}
+
+int qpch_solver_create(const qpch_problem* problem, qpch_solver** out) {
+  *out = nullptr;
+  return guarded([&] {
+    if (!problem) throw qpch::InvalidArgument("null problem");
+    auto s = std::make_unique<qpch_solver>();
+    s->impl.init(*problem);
+    *out = s.release();
+  });
+}
+
+void qpch_solver_destroy(qpch_solver* solver) {
+  if (!solver) return;
+  if (solver->impl.stream) (void)hipStreamSynchronize(solver->impl.stream);
+  delete solver;
+}
+
+int qpch_solver_begin_run(qpch_solver* s, int capacity) {
+  return guarded([&] { s->impl.begin_run(capacity); });
+}
+
+int qpch_solver_iterate(qpch_solver* s, int iterations, float a, float b, int refine) {
+  return guarded([&] { s->impl.iterate(iterations, a, b, refine != 0); });
+}
+
+int qpch_solver_shrinkwrap(qpch_solver* s, float threshold, float sigma, int method) {
+  return guarded([&] { s->impl.shrinkwrap(threshold, sigma, method); });
+}
+
+int qpch_solver_get_support(qpch_solver* s, uint8_t* host) {
+  return guarded([&] {
+    auto& m = s->impl;
+    QPCH_HIP(hipStreamSynchronize(m.stream));
+    QPCH_HIP(hipMemcpy(host, m.supp.get(), m.supp.size(), hipMemcpyDeviceToHost));
+  });
+}
+
+int qpch_solver_set_support(qpch_solver* s, const uint8_t* host) {
+  return guarded([&] {
+    auto& m = s->impl;
+    QPCH_HIP(hipStreamSynchronize(m.stream));
+    QPCH_HIP(hipMemcpy(m.supp.get(), host, m.supp.size(), hipMemcpyHostToDevice));
+  });
+}
+
+int qpch_solver_save_support(qpch_solver* s) {
+  return guarded([&] {
+    auto& m = s->impl;
+    QPCH_HIP(hipMemcpyAsync(m.saved.get(), m.supp.get(), m.supp.size(), hipMemcpyDeviceToDevice, m.stream));
+  });
+}
+
+int qpch_solver_restore_support(qpch_solver* s) {
+  return guarded([&] {
+    auto& m = s->impl;
+    QPCH_HIP(hipMemcpyAsync(m.supp.get(), m.saved.get(), m.supp.size(), hipMemcpyDeviceToDevice, m.stream));
+  });
+}
+
+int qpch_solver_read_errors(qpch_solver* s, int first, int count, double* host) {
+  return guarded([&] {
+    auto& m = s->impl;
+    if (first < 0 || count < 0 || first + count > m.err_cap) throw qpch::InvalidArgument("error slots out of range");
+    QPCH_HIP(hipStreamSynchronize(m.stream));
+    if (count) {
+      QPCH_HIP(hipMemcpy(host, m.errs.get() + static_cast<size_t>(first) * m.starts,
+                         sizeof(double) * count * m.starts, hipMemcpyDeviceToHost));
+    }
+  });
+}
+
+int qpch_solver_read_state(qpch_solver* s, float* host) {
+  return guarded([&] { s->impl.read_state(host); });
+}
+
+double qpch_solver_energy(const qpch_solver* s) { return s->impl.energy; }
+
+int qpch_solver_synchronize(qpch_solver* s) {
+  return guarded([&] { QPCH_HIP(hipStreamSynchronize(s->impl.stream)); });
+}
+
+// --- operators ---------------------------------------------------------------------------
+
+int qpch_fft(float* host, int ndim, const size_t* shape, int forward) {
+  return guarded([&] {
+    const qpch::Dims d = qpch::make_dims(ndim, shape);
+    qpch::DevBuf<float2> buf = qpch::upload(reinterpret_cast<const float2*>(host), d.n);
+    hipStream_t st{};
+    QPCH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    qpch::device_fft(buf.get(), d, 1, forward != 0, static_cast<float>(1.0 / std::sqrt(static_cast<double>(d.n))), st);
+    const hipError_t e = hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+    QPCH_HIP(e);
+    QPCH_HIP(hipMemcpy(host, buf.get(), sizeof(float2) * d.n, hipMemcpyDeviceToHost));
+  });
+}
+
+int qpch_modulus_project(float* host_x, const float* amplitude, const uint8_t* mask, int ndim,
+                         const size_t* shape, double* numerator) {
+  return guarded([&] {
+    const qpch::Dims d = qpch::make_dims(ndim, shape);
+    std::vector<float> signed_amp(d.n);
+    for (long long i = 0; i < d.n; ++i) signed_amp[i] = mask[i] ? -1.0f : amplitude[i];
+    qpch::DevBuf<float> amp = qpch::upload(signed_amp.data(), d.n);
+    qpch::DevBuf<float2> buf = qpch::upload(reinterpret_cast<const float2*>(host_x), d.n);
+    const int grid = qpch::grid_for(d.n, 256, 1024);
+    qpch::DevBuf<double> part(grid);
+    qpch::DevBuf<int> counter(1);
+    hipStream_t st{};
+    QPCH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    const float s = static_cast<float>(1.0 / std::sqrt(static_cast<double>(d.n)));
+    qpch::device_fft(buf.get(), d, 1, true, 1.0f, st);
+    hipLaunchKernelGGL(qpch::dev::modulus_elementwise, dim3(grid, 1), dim3(256), 0, st, buf.get(), amp.get(),
+                       d.n, s, s, part.get(), counter.get());
+    qpch::device_fft(buf.get(), d, 1, false, 1.0f, st);
+    const hipError_t e = hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+    QPCH_HIP(e);
+    std::vector<double> hp(grid);
+    QPCH_HIP(hipMemcpy(hp.data(), part.get(), sizeof(double) * grid, hipMemcpyDeviceToHost));
+    double num = 0.0;
+    for (double v : hp) num += v;
+    *numerator = num;
+    QPCH_HIP(hipMemcpy(host_x, buf.get(), sizeof(float2) * d.n, hipMemcpyDeviceToHost));
+  });
+}
+
+int qpch_support_project(float* host_x, const uint8_t* support, size_t n) {
+  return guarded([&] {
+    const long long nn = static_cast<long long>(n);
+    qpch::DevBuf<float2> buf = qpch::upload(reinterpret_cast<const float2*>(host_x), n);
+    qpch::DevBuf<std::uint8_t> sup = qpch::upload(support, n);
+    hipLaunchKernelGGL((qpch::dev::convert<float2, float2>), dim3(qpch::grid_for(nn)), dim3(256), 0, nullptr,
+                       buf.get(), buf.get(), nn, 1.0f, sup.get());
+    QPCH_HIP(hipGetLastError());
+    QPCH_HIP(hipDeviceSynchronize());
+    QPCH_HIP(hipMemcpy(host_x, buf.get(), sizeof(float2) * n, hipMemcpyDeviceToHost));
+  });
+}
+
+int qpch_gaussian_blur(const float* host_in, float* host_out, int ndim, const size_t* shape, float sigma) {
+  return guarded([&] {
+    if (!(sigma >= 0.0f)) throw qpch::InvalidArgument("sigma must be >= 0");
+    const qpch::Dims d = qpch::make_dims(ndim, shape);
+    qpch::DevBuf<float> a = qpch::upload(host_in, d.n);
+    qpch::DevBuf<float> b(d.n);
+    float* cur = a.get();
+    float* other = b.get();
+    std::vector<qpch::DevBuf<float>> keep;
+    for (int ax = d.nd - 1; ax >= 0; --ax) {
+      const qpch::Taps t = qpch::blur_taps(d.d[ax], sigma);
+      keep.push_back(qpch::upload(t.taps.data(), t.taps.size()));
+      qpch::dev::BlurAxis bx{d.d[ax], static_cast<int>(d.inner(ax)), static_cast<int>(t.taps.size()), t.J,
+                             keep.back().get()};
+      hipLaunchKernelGGL((qpch::dev::blur_axis<float2, false>), dim3(qpch::grid_for(d.n), 1), dim3(256), 0,
+                         nullptr, cur, other, d.n, bx);
+      QPCH_HIP(hipGetLastError());
+      std::swap(cur, other);
+    }
+    QPCH_HIP(hipDeviceSynchronize());
+    QPCH_HIP(hipMemcpy(host_out, cur, sizeof(float) * d.n, hipMemcpyDeviceToHost));
+  });
+}
+
+int qpch_amplitude_loss(const void*, const void*, void*, double*, int, const size_t*, int, int, void*) {
+  return guarded([&] { throw qpch::InvalidArgument("qpch_amplitude_loss is not available yet"); });
+}
+
+}  // extern "C"

@@ -1,0 +1,149 @@
+// Fallback kernels for axes that are not powers of two (any length up to 2048), and the
+// element-wise kernels of the unfused iteration that goes with them.  These keep every
+// shape the reference accepts reconstructible; the fused passes are the fast path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "fft_engine.hpp"
+#include "passes.hpp"
+
+namespace qpch::dev {
+
+constexpr int kGenericMaxN = 2048;
+constexpr int kGenericThreads = 256;
+
+struct GenericPlan {
+  int n;
+  int nfac;  // 0: one plain DFT of the whole line
+  int fac[32];
+};
+
+// One line per work-group, autosort mixed-radix stages ping-ponging between two LDS
+// buffers; radix-r butterflies are evaluated as O(r^2) sums with the line's own W_N table.
+// Lines are addressed like the fused passes: element k of line (o, c) of start s at
+// s*n + o*N*inner + k*inner + c, line = o*inner + c = blockIdx.x.
+__global__ void __launch_bounds__(kGenericThreads)
+    generic_fft(float2* data, long long n_per_start, int inner, GenericPlan plan, int forward,
+                float scale) {
+  extern __shared__ float2 sm[];
+  const int N = plan.n;
+  float2* bufa = sm;
+  float2* bufb = sm + N;
+  float2* tw = sm + 2 * N;
+  for (int j = threadIdx.x; j < N; j += blockDim.x) {
+    double s, c;
+    sincospi(2.0 * static_cast<double>(j) / static_cast<double>(N), &s, &c);
+    tw[j] = {static_cast<float>(c), static_cast<float>(forward ? -s : s)};
+  }
+  const int o = blockIdx.x / inner;
+  const int c = blockIdx.x % inner;
+  const long long base = static_cast<long long>(blockIdx.y) * n_per_start +
+                         static_cast<long long>(o) * N * inner + c;
+  for (int k = threadIdx.x; k < N; k += blockDim.x) bufa[k] = data[base + static_cast<long long>(k) * inner];
+  __syncthreads();
+  float2* src = bufa;
+  float2* dst = bufb;
+  if (plan.nfac == 0) {
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+      float2 acc = {0.0f, 0.0f};
+      for (int q = 0; q < N; ++q) {
+        const float2 w = tw[static_cast<int>((static_cast<long long>(j) * q) % N)];
+        acc = cadd(acc, cmul(src[q], w));
+      }
+      dst[j] = acc;
+    }
+    __syncthreads();
+    float2* t = src;
+    src = dst;
+    dst = t;
+  } else {
+    int L = 1;
+    for (int s = 0; s < plan.nfac; ++s) {
+      const int r = plan.fac[s];
+      const int m = N / (L * r);
+      const int nr = N / r;
+      for (int t = threadIdx.x; t < nr; t += blockDim.x) {
+        const int k = t % L;
+        const int outb = (t - k) * r + k;
+        for (int j = 0; j < r; ++j) {
+          float2 acc = {0.0f, 0.0f};
+          for (int q = 0; q < r; ++q) {
+            float2 v = src[t + q * nr];
+            const int tq = q * k * m;  // stage twiddle W_{L r}^{q k}
+            if (tq) v = cmul(v, tw[tq]);
+            const int e = ((j * q) % r) * nr;  // W_r^{j q} = W_N^{(j q mod r) N / r}
+            acc = cadd(acc, e ? cmul(v, tw[e]) : v);
+          }
+          dst[outb + j * L] = acc;
+        }
+      }
+      __syncthreads();
+      float2* tt = src;
+      src = dst;
+      dst = tt;
+      L *= r;
+    }
+  }
+  for (int k = threadIdx.x; k < N; k += blockDim.x) {
+    const float2 v = src[k];
+    data[base + static_cast<long long>(k) * inner] = {v.x * scale, v.y * scale};
+  }
+}
+
+// Element-wise modulus projection over whole starts (unfused iteration).
+__global__ void __launch_bounds__(256)
+    modulus_elementwise(float2* w, const float* amp, long long n, float in_scale,
+                        float out_scale, double* partials, int* counter) {
+  __shared__ double red[16];
+  const long long soff = static_cast<long long>(blockIdx.y) * n;
+  double acc = 0.0;
+  for (long long k = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; k < n;
+       k += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const float2 v = w[soff + k];
+    const float2 q = project({v.x * in_scale, v.y * in_scale}, amp[k], acc);
+    w[soff + k] = {q.x * out_scale, q.y * out_scale};
+  }
+  const double t = block_sum(acc, red);
+  if (threadIdx.x == 0) {
+    partials[static_cast<long long>(blockIdx.y) * gridDim.x + blockIdx.x] = t;
+    if (blockIdx.x == 0 && blockIdx.y == 0) atomicAdd(counter, 1);
+  }
+}
+
+// Element-wise update (unfused iteration), plus the fixed-order error reduction.
+__global__ void __launch_bounds__(256)
+    update_elementwise(float2* x, const float2* pm, const std::uint8_t* support, long long n,
+                       float a, float b, float in_scale, const double* partials,
+                       int partial_stride, double* errs, int err_stride, int err_offset,
+                       int err_cap, const int* counter) {
+  const long long soff = static_cast<long long>(blockIdx.y) * n;
+  for (long long k = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; k < n;
+       k += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const float2 v = pm[soff + k];
+    const float2 pv = {v.x * in_scale, v.y * in_scale};
+    if (support[soff + k]) {
+      x[soff + k] = pv;
+    } else {
+      const float2 xo = x[soff + k];
+      x[soff + k] = {a * xo.x + b * pv.x, a * xo.y + b * pv.y};
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 64 && errs) {
+    double s = 0.0;
+    const double* part = partials + static_cast<long long>(blockIdx.y) * partial_stride;
+    for (int k = threadIdx.x; k < partial_stride; k += 64) s += part[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if (threadIdx.x == 0) {
+      const int slot = *counter - 1;
+      if (slot >= 0 && slot < err_cap) {
+        errs[static_cast<long long>(slot) * err_stride + err_offset + blockIdx.y] = s;
+      }
+    }
+  }
+}
+
+}  // namespace qpch::dev

@@ -1,0 +1,184 @@
+// Set-up, conversion and Shrinkwrap kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "passes.hpp"
+#include "storage.hpp"
+
+namespace qpch::dev {
+
+// splitmix64 finaliser and the counter-based uniform draw of the reference
+// (src/core/pointwise.hpp:19-32): any element of any start can be drawn on its own.
+__device__ __forceinline__ std::uint64_t mix64(std::uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ float uniform01(std::uint64_t seed, std::uint64_t start,
+                                           std::uint64_t index) {
+  const std::uint64_t key = mix64(seed ^ mix64(start + 1));
+  return static_cast<float>(mix64(key + index) >> 40) * 0x1p-24f;
+}
+
+// w = A e^{i 2 pi u}: the spectrum of a random start before its unitary inverse
+// transform (src/cpu/kernels.cpp:104-111).
+__global__ void random_wave(float2* w, const float* amp, long long n, std::uint64_t seed,
+                            std::uint64_t first_start) {
+  const std::uint64_t start = first_start + blockIdx.y;
+  const long long soff = static_cast<long long>(blockIdx.y) * n;
+  for (long long k = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; k < n;
+       k += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const float phase = 6.28318530717958647692f * uniform01(seed, start, static_cast<std::uint64_t>(k));
+    const float a = amp[k];
+    w[soff + k] = {a * cosf(phase), a * sinf(phase)};
+  }
+}
+
+// out = scale * in, zeroed outside the support when one is given.
+template <class Sin, class Sout>
+__global__ void convert(const Sin* in, Sout* out, long long total, float scale,
+                        const std::uint8_t* support) {
+  for (long long k = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; k < total;
+       k += static_cast<long long>(gridDim.x) * blockDim.x) {
+    float2 v = Storage<Sin>::ld(in, k);
+    v = {v.x * scale, v.y * scale};
+    if (support && !support[k]) v = {0.0f, 0.0f};
+    Storage<Sout>::st(out, k, v);
+  }
+}
+
+__global__ void fill_int(int* p, int count, int value) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) {
+    p[k] = value;
+  }
+}
+
+__global__ void fill_double(double* p, long long count, double value) {
+  for (long long k = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; k < count;
+       k += static_cast<long long>(gridDim.x) * blockDim.x) {
+    p[k] = value;
+  }
+}
+
+// --- Shrinkwrap ------------------------------------------------------------------------
+// The Gaussian transfer exp(-2 pi^2 sigma^2 |f|^2) of the reference (src/cpu/kernels.cpp:
+// 52-75) factorises over the axes, so the blur is a separable circular convolution with
+// the exact per-axis kernel h[d] = IDFT_N(exp(-2 pi^2 sigma^2 f^2))[d] (computed on the
+// host in double).  taps[t] multiplies in[(k + t - J) mod N] (J = (T - 1) / 2), or
+// in[(k + t) mod N] when the kernel spans the whole axis (full = 1, T = N).
+
+struct BlurAxis {
+  int N;        // axis length
+  int inner;    // element stride of the axis
+  int T;        // number of taps
+  int J;        // offset of tap 0
+  const float* taps;
+};
+
+template <class St, bool kFirst>
+__global__ void blur_axis(const void* in_, float* out, long long n, BlurAxis ax) {
+  const long long soff = static_cast<long long>(blockIdx.y) * n;
+  for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < n;
+       e += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const int k = static_cast<int>((e / ax.inner) % ax.N);
+    const long long line0 = e - static_cast<long long>(k) * ax.inner;
+    float acc = 0.0f;
+    for (int t = 0; t < ax.T; ++t) {
+      int kk = k + t - ax.J;
+      kk %= ax.N;
+      if (kk < 0) kk += ax.N;
+      const long long at = soff + line0 + static_cast<long long>(kk) * ax.inner;
+      float v;
+      if constexpr (kFirst) {
+        const float2 c = Storage<St>::ld(static_cast<const St*>(in_), at);
+        v = sqrtf(c.x * c.x + c.y * c.y);
+      } else {
+        v = static_cast<const float*>(in_)[at];
+      }
+      acc += ax.taps[t] * v;
+    }
+    out[soff + e] = acc;
+  }
+}
+
+// Per-block statistic of the blurred magnitude: max (method 0) or the FP64 sum of squares
+// and count over the current support (method 1, the rms of src/cpu/kernels.cpp:84-98).
+__global__ void __launch_bounds__(256)
+    shrinkwrap_level(const float* blurred, const std::uint8_t* support, long long n, int method,
+                     double* partials /* [start][blocks][2] */) {
+  __shared__ double red[16];
+  __shared__ float redf[16];
+  const long long soff = static_cast<long long>(blockIdx.y) * n;
+  double sum = 0.0, count = 0.0;
+  float peak = 0.0f;
+  for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < n;
+       e += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const float v = blurred[soff + e];
+    if (method == 0) {
+      peak = fmaxf(peak, v);
+    } else if (support[soff + e]) {
+      sum += static_cast<double>(v) * static_cast<double>(v);
+      count += 1.0;
+    }
+  }
+  double* out = partials + (static_cast<long long>(blockIdx.y) * gridDim.x + blockIdx.x) * 2;
+  if (method == 0) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) peak = fmaxf(peak, __shfl_down(peak, o, 64));
+    if ((threadIdx.x & 63) == 0) redf[threadIdx.x >> 6] = peak;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float m = redf[0];
+      for (int k = 1; k < static_cast<int>((blockDim.x + 63) >> 6); ++k) m = fmaxf(m, redf[k]);
+      out[0] = m;
+      out[1] = 0.0;
+    }
+  } else {
+    const double s = block_sum(sum, red);
+    __syncthreads();
+    const double c = block_sum(count, red);
+    if (threadIdx.x == 0) {
+      out[0] = s;
+      out[1] = c;
+    }
+  }
+}
+
+// support = blurred > threshold * level, the level reduced from the partials in a fixed
+// order by every block.
+__global__ void __launch_bounds__(256)
+    shrinkwrap_threshold(const float* blurred, const double* partials, int blocks, long long n,
+                         int method, float threshold, std::uint8_t* support) {
+  __shared__ float cut;
+  if (threadIdx.x == 0) {
+    const double* part = partials + static_cast<long long>(blockIdx.y) * blocks * 2;
+    float level;
+    if (method == 0) {
+      float m = 0.0f;
+      for (int k = 0; k < blocks; ++k) m = fmaxf(m, static_cast<float>(part[2 * k]));
+      level = m;
+    } else {
+      double s = 0.0, c = 0.0;
+      for (int k = 0; k < blocks; ++k) {
+        s += part[2 * k];
+        c += part[2 * k + 1];
+      }
+      level = static_cast<float>(sqrt(s / fmax(c, 1.0)));
+    }
+    cut = threshold * level;
+  }
+  __syncthreads();
+  const long long soff = static_cast<long long>(blockIdx.y) * n;
+  const float c = cut;
+  for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < n;
+       e += static_cast<long long>(gridDim.x) * blockDim.x) {
+    support[soff + e] = blurred[soff + e] > c;
+  }
+}
+
+}  // namespace qpch::dev

@@ -1,0 +1,245 @@
+// Fused line passes of the projection iteration (power-of-two axes).
+//
+// One launch transforms every line of one axis of a batch of starts.  A line is N
+// elements spaced `inner` apart; lines are numbered (o, c) with o in [0, outer) and c in
+// [0, inner), element k of line (o, c) of start s at s*n + o*N*inner + k*inner + c.
+// inner == 1 is a row pass (lines contiguous, B lines per work-group); inner > 1 is a
+// column pass (B adjacent columns per work-group, so every load of the group covers B
+// consecutive elements of a row).
+//
+// Modes (what a work-group does with each line):
+//   kFwd / kInv  transform (optionally reading `src` instead of `w`), scale, store to w
+//   kModulus     forward, Fourier-modulus projection with the error partial, inverse:
+//                the frequency-axis pass; reference src/cuda/device.cuh:17-29 and
+//                src/cpu/kernels.cpp:25-42
+//   kUpdate      inverse -> P_M x, x = S ? P_M x : a x + b P_M x, forward of the new x:
+//                the real-axis pass; reference src/cpu/kernels.cpp:44-50,
+//                src/core/pointwise.hpp:34-45
+// The error numerator of a start is reduced in a fixed order: the modulus pass writes one
+// FP64 partial per work-group, the next update pass (work-group 0 of each start) sums
+// them and stores errs[slot][start]; slot = *counter - 1, the modulus pass having
+// advanced the counter.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "fft_engine.hpp"
+#include "storage.hpp"
+
+namespace qpch::dev {
+
+enum class Mode : int { kFwd = 0, kInv = 1, kModulus = 2, kUpdate = 3 };
+
+struct PassArgs {
+  int outer;
+  int inner;
+  long long n;  // elements per start
+  void* w;      // wave, St*
+  const void* src;  // kFwd/kInv input when not null (St_in = St)
+  void* x;          // kUpdate: iterate, St*
+  const std::uint8_t* support;  // kUpdate: [start][n]
+  const float* amp;             // kModulus: signed amplitude [n], -1 = masked
+  double* partials;             // [start][partial_stride]
+  int partial_stride;
+  double* errs;                 // [slot][err_stride]
+  int err_stride;
+  int err_offset;               // global index of this launch's first start
+  int err_cap;                  // slots available
+  int* counter;
+  float a, b;                   // update coefficients
+  float in_scale;               // applied after the first transform of the pass
+  float out_scale;              // applied before the store (kModulus: after projection)
+};
+
+// Work-group geometry of a pass, shared by the kernel and its launcher.
+template <int N, bool COL>
+struct PassGeom {
+  using S = Shape1<N, 16>;
+  static constexpr int kT = S::kT;
+  static constexpr int kR = S::kR;
+  static constexpr int kLdsLimit = 64 * 1024 / 8 - N;  // float2 slots for the line tiles
+  static constexpr int pow2_floor(int v) { return v <= 1 ? 1 : 2 * pow2_floor(v / 2); }
+  static constexpr int clampB(int b) {
+    int lim = kLdsLimit / S::kLine;
+    if (1024 / kT < lim) lim = 1024 / kT;
+    return pow2_floor(b < lim ? b : lim);
+  }
+  static constexpr int kB = clampB(COL ? (256 / kT > 16 ? 256 / kT : 16) : (256 / kT > 1 ? 256 / kT : 1));
+  static constexpr int kThreads = kB * kT;
+  static constexpr int kTile = kB * S::kLine;  // float2
+};
+
+// Sum over the work-group in a fixed order; the result is valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double* scratch) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  if (lane == 0) scratch[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) {
+    const int waves = (blockDim.x + 63) >> 6;
+    for (int k = 0; k < waves; ++k) t += scratch[k];
+  }
+  return t;
+}
+
+// W_N^j = exp(-2 pi i j / N) into LDS, evaluated in double and rounded once.
+template <int N>
+__device__ __forceinline__ void load_twiddles(float2* tw) {
+  for (int j = threadIdx.x; j < N; j += blockDim.x) {
+    double s, c;
+    sincospi(2.0 * static_cast<double>(j) / static_cast<double>(N), &s, &c);
+    tw[j] = {static_cast<float>(c), static_cast<float>(-s)};
+  }
+}
+
+// One element of the modulus projection: w is the (scaled) spectrum value, a its measured
+// amplitude or -1 when masked.  Usable pixels add (|w| - a)^2 to acc and take magnitude
+// a; masked ones keep their own.  Reference semantics: src/cpu/kernels.cpp:29-39.
+__device__ __forceinline__ float2 project(float2 w, float a, double& acc) {
+  const float mag = sqrtf(w.x * w.x + w.y * w.y);
+  float target = mag;
+  if (a >= 0.0f) {
+    const float d = mag - a;
+    acc += static_cast<double>(d) * static_cast<double>(d);
+    target = a;
+  }
+  const float f = target / (mag + 1e-12f);
+  return {w.x * f, w.y * f};
+}
+
+template <int N, bool COL, Mode M, class St>
+__global__ void __launch_bounds__((PassGeom<N, COL>::kThreads))
+    line_pass(PassArgs p) {
+  using G = PassGeom<N, COL>;
+  using Io = Storage<St>;
+  constexpr int kT = G::kT;
+  constexpr int kR = G::kR;
+  constexpr int kB = G::kB;
+  __shared__ float2 lds[G::kTile + N];
+  __shared__ double red[16];
+  float2* tw = lds + G::kTile;
+  load_twiddles<N>(tw);
+
+  const int tid = threadIdx.x;
+  int b, i;
+  if constexpr (COL) {
+    b = tid % kB;
+    i = tid / kB;
+  } else {
+    i = tid % kT;
+    b = tid / kT;
+  }
+  const int start = blockIdx.y;
+  long long base;  // element 0 of this thread's line, relative to its start
+  bool valid;
+  if constexpr (COL) {
+    const int groups = (p.inner + kB - 1) / kB;
+    const int o = blockIdx.x / groups;
+    const int c = (blockIdx.x % groups) * kB + b;
+    valid = c < p.inner;
+    base = static_cast<long long>(o) * N * p.inner + c;
+  } else {
+    const int o = blockIdx.x * kB + b;
+    valid = o < p.outer;
+    base = static_cast<long long>(o) * N;
+  }
+  const long long es = COL ? p.inner : 1;
+  const long long soff = static_cast<long long>(start) * p.n;
+  float2* line = lds + b * Shape1<N, 16>::kLine;
+
+  St* w = static_cast<St*>(p.w);
+  const St* src = p.src ? static_cast<const St*>(p.src) : w;
+  float2 v[kR];
+  __syncthreads();  // twiddles
+
+  if constexpr (M == Mode::kFwd || M == Mode::kInv) {
+#pragma unroll
+    for (int m = 0; m < kR; ++m) {
+      v[m] = valid ? Io::ld(src, soff + base + (i + m * kT) * es) : float2{0.0f, 0.0f};
+    }
+    fft_line<N, 16, M == Mode::kFwd>(v, i, line, tw);
+    if (valid) {
+#pragma unroll
+      for (int m = 0; m < kR; ++m) {
+        Io::st(w, soff + base + (i + m * kT) * es,
+               float2{v[m].x * p.out_scale, v[m].y * p.out_scale});
+      }
+    }
+  } else if constexpr (M == Mode::kModulus) {
+#pragma unroll
+    for (int m = 0; m < kR; ++m) {
+      v[m] = valid ? Io::ld(w, soff + base + (i + m * kT) * es) : float2{0.0f, 0.0f};
+    }
+    fft_line<N, 16, true>(v, i, line, tw);
+    double acc = 0.0;
+    if (valid) {
+#pragma unroll
+      for (int m = 0; m < kR; ++m) {
+        const float a = p.amp[base + (i + m * kT) * es];
+        const float2 u = {v[m].x * p.in_scale, v[m].y * p.in_scale};
+        const float2 q = project(u, a, acc);
+        v[m] = {q.x * p.out_scale, q.y * p.out_scale};
+      }
+    }
+    fft_line<N, 16, false>(v, i, line, tw);
+    if (valid) {
+#pragma unroll
+      for (int m = 0; m < kR; ++m) Io::st(w, soff + base + (i + m * kT) * es, v[m]);
+    }
+    const double total = block_sum(acc, red);
+    if (tid == 0) {
+      p.partials[static_cast<long long>(start) * p.partial_stride + blockIdx.x] = total;
+      if (blockIdx.x == 0 && start == 0) atomicAdd(p.counter, 1);
+    }
+  } else {  // kUpdate
+    St* x = static_cast<St*>(p.x);
+#pragma unroll
+    for (int m = 0; m < kR; ++m) {
+      v[m] = valid ? Io::ld(w, soff + base + (i + m * kT) * es) : float2{0.0f, 0.0f};
+    }
+    fft_line<N, 16, false>(v, i, line, tw);
+    if (valid) {
+#pragma unroll
+      for (int m = 0; m < kR; ++m) {
+        const long long at = soff + base + (i + m * kT) * es;
+        const float2 pm = {v[m].x * p.in_scale, v[m].y * p.in_scale};
+        float2 nx = pm;
+        if (!p.support[at]) {
+          const float2 xo = Io::ld(x, at);
+          nx = {p.a * xo.x + p.b * pm.x, p.a * xo.y + p.b * pm.y};
+        }
+        Io::st(x, at, nx);
+        v[m] = nx;
+      }
+    }
+    fft_line<N, 16, true>(v, i, line, tw);
+    if (valid) {
+#pragma unroll
+      for (int m = 0; m < kR; ++m) {
+        Io::st(w, soff + base + (i + m * kT) * es,
+               float2{v[m].x * p.out_scale, v[m].y * p.out_scale});
+      }
+    }
+    if (blockIdx.x == 0 && tid < 64 && p.errs) {
+      // fixed-order sum of the previous modulus pass's partials of this start
+      double s = 0.0;
+      const double* part = p.partials + static_cast<long long>(start) * p.partial_stride;
+      for (int k = tid; k < p.partial_stride; k += 64) s += part[k];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+      if (tid == 0) {
+        const int slot = *p.counter - 1;
+        if (slot >= 0 && slot < p.err_cap) {
+          p.errs[static_cast<long long>(slot) * p.err_stride + p.err_offset + start] = s;
+        }
+      }
+    }
+  }
+}
+
+}  // namespace qpch::dev

@@ -1,0 +1,132 @@
+"""The C ABI on the device against the oracle: operators, one teacher-forced iteration,
+and short recipes (the reference's CUDA-vs-CPU bar, tests/test_cuda.cpp)."""
+
+import numpy as np
+import pytest
+
+import oracle
+import recipes as R
+import synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_device_info(qpch):
+    info = qpch.device_info()
+    print(info)
+    assert info["name"]
+    assert "gfx950" in info["arch"]
+    assert info["peak_bandwidth_gbs"] > 10.0
+
+
+@pytest.mark.parametrize(
+    "shape",
+    [(64, 64), (256, 256), (512, 512), (128, 64), (8, 4096), (17, 24), (21, 15), (16, 20, 12),
+     (32, 32, 32), (128, 128, 16), (15, 16, 17), (2, 2)],
+)
+@pytest.mark.parametrize("forward", [True, False])
+def test_fft_matches_oracle(qpch, shape, forward):
+    rng = np.random.default_rng(1)
+    x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    got = qpch.fft(x, forward)
+    want = oracle.fft(x.astype(np.complex128), forward, "d")
+    assert rel(got, want) < 2e-6, (shape, rel(got, want))
+
+
+def test_modulus_projection_matches_oracle(qpch):
+    # tests/test_cuda.cpp:26-39 on a pow2 and a non-pow2 shape
+    for shape, m in [((48, 40), 12), ((64, 64), 16), ((16, 16, 16), 8)]:
+        obj, sup, inten = synthetic.triangle(shape, m)
+        amp, mask, sup = oracle.make_problem(inten, sup, None)
+        mask = mask.copy()
+        mask.flat[5] = 1
+        x = synthetic.random_field(obj.size, 5).reshape(shape)
+        af, mf, xf = oracle.to_fft_order(amp), oracle.to_fft_order(mask), oracle.to_fft_order(x)
+        w = oracle.fft(xf, True)
+        w, err_o = oracle.modulus(w, af, mf)
+        p_o = oracle.fft(w, False)
+        p_g, num = qpch.modulus_project(xf, af, mf)
+        energy = oracle.usable_energy(af, mf)
+        err_g = float(np.float32(np.sqrt(num / (energy + 1e-12))))
+        assert abs(err_g - err_o) <= 1e-5 * err_o, (shape, err_g, err_o)
+        assert rel(p_g, p_o) < 1e-5, (shape, rel(p_g, p_o))
+        # idempotence at the projection
+        p2, num2 = qpch.modulus_project(p_g, af, mf)
+        assert np.sqrt(num2 / energy) < 1e-5
+
+
+def test_support_and_blur_match_oracle(qpch):
+    # tests/test_cuda.cpp:41-54
+    obj, sup, _ = synthetic.triangle((16, 20, 12), 6)
+    x = synthetic.random_field(obj.size, 6).reshape(obj.shape)
+    a = np.where(sup != 0, x, 0).astype(np.complex64)
+    b = qpch.support_project(x, sup)
+    assert np.array_equal(a, b)
+    image = np.abs(a).astype(np.float32)
+    for sigma in (1.0, 0.5, 2.5):
+        ca = oracle.blur(image, sigma)
+        cb = qpch.gaussian_blur(image, sigma)
+        assert np.max(np.abs(ca - cb)) < 1e-5, (sigma, np.max(np.abs(ca - cb)))
+
+
+def _problem(shape, m, masked=False):
+    obj, sup, inten = synthetic.triangle(shape, m)
+    amp, mask, sup = oracle.make_problem(inten, sup, None)
+    if masked:
+        mask = mask.copy()
+        mask.flat[3] = 1
+        mask.flat[obj.size // 2 + 7] = 1
+    return obj, amp, mask, sup
+
+
+@pytest.mark.parametrize("shape", [(64, 64), (48, 40), (32, 32, 32), (256, 256), (16, 8, 32)])
+@pytest.mark.parametrize("schedule", ["graph", "fused", "unfused"])
+def test_teacher_forced_iteration(qpch, shape, schedule):
+    """One iteration from the same start: error within 1e-5 relative, object within 1e-5."""
+    obj, amp, mask, sup = _problem(shape, max(4, shape[-1] // 4), masked=True)
+    start = np.stack([synthetic.random_field(obj.size, 11).reshape(shape),
+                      synthetic.random_field(obj.size, 12).reshape(shape)])
+    for kind, ops in [("hio", [R.hio(1)]), ("raar", [R.raar(1)]), ("er", [R.er(1)])]:
+        o = oracle.Solver(amp, mask, sup, starts=2, start=start)
+        o.run(ops)
+        ores = o.results()
+        sf = np.stack([oracle.to_fft_order(s) for s in start])
+        s = qpch.Solver(oracle.to_fft_order(amp), oracle.to_fft_order(mask),
+                        oracle.to_fft_order(sup), starts=2, start=sf, schedule=schedule)
+        errs = R.run_abi(s, ops)
+        xs = s.read_state()
+        for k in range(2):
+            assert abs(errs[0, k] - ores[k][2][0]) <= 1e-5 * ores[k][2][0], (kind, k)
+            xo = oracle.to_fft_order(ores[k][0])
+            assert rel(xs[k], xo) < 1e-5, (kind, k, rel(xs[k], xo))
+        s.close()
+
+
+@pytest.mark.parametrize("shape,m,schedule", [((64, 48), 16, "graph"), ((64, 64), 16, "graph"),
+                                               ((64, 64), 16, "fused"), ((64, 48), 16, "unfused")])
+def test_agrees_with_oracle_short_recipe(qpch, shape, m, schedule):
+    """tests/test_cuda.cpp:66-87: HIO15, Shrinkwrap, RAAR10, detwin(HIO5), ER10, 2 starts."""
+    obj, amp, mask, sup = _problem(shape, m)
+    ops = R.flat(R.hio(15), R.shrinkwrap(), R.raar(10), R.detwin(R.hio(5)), R.er(10))
+    o = oracle.Solver(amp, mask, sup, starts=2, seed=3)
+    o.run(ops)
+    ores = o.results()
+    s = qpch.Solver(oracle.to_fft_order(amp), oracle.to_fft_order(mask), oracle.to_fft_order(sup),
+                    starts=2, seed=3, schedule=schedule)
+    errs = R.run_abi(s, ops, sync_every=4)
+    xs = s.read_state()
+    sups = s.get_support()
+    worst = 0.0
+    for k in range(2):
+        e_o = ores[k][2]
+        d = np.abs(errs[:, k] - e_o) / e_o
+        worst = max(worst, float(d.max()))
+        assert np.all(np.abs(errs[:, k] - e_o) <= 1e-3 * e_o + 1e-6), (k, d.max())
+        assert rel(oracle.to_centered(xs[k]), ores[k][0]) < 1e-3
+        assert np.array_equal(oracle.to_centered(sups[k]).astype(bool), ores[k][1])
+    print(f"{shape} {schedule}: worst per-iteration relative error difference {worst:.2e}")
+    s.close()

@@ -1,0 +1,267 @@
+#!/usr/bin/env python3
+"""RAAR(+ER) start-iterations per second on one MI355X, batched reduced-precision storage
+against the plain fp32 one-start-per-launch path, with the HBM roofline of the dominant
+kernel and the CPU oracle timed on the host beside it.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--configs c2,c4] [--no-cpu]
+
+A step is one full recipe run over every start of the workload (the reference's
+bench/bench.cpp:33-45 protocol: construct, warm up with a run of the same length so the
+hipGraphs exist, then time).  The headline value is the 256^2 workload (32 starts,
+repeat(10, RAAR(50, beta 0.9), Shrinkwrap) then ER(200), Poisson data with a beamstop)
+on the batched fp16-storage path; the 128^3 workload (8 starts, repeat(8, RAAR(50),
+Shrinkwrap rms) then ER(200)) is reported under "details".  Prints one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E, public spec
+
+# Algorithmic bytes per element per start of each pass (the two-pass model): reads and
+# writes of the wave, the iterate, the amplitude and the support.
+PASS_BYTES = {
+    ("modulus", "f32"): 8 + 4 + 8,
+    ("update", "f32"): 8 + 8 + 1 + 8 + 8,
+    ("inv", "f32"): 16,
+    ("fwd", "f32"): 16,
+    ("modulus", "f16"): 4 + 4 + 4,
+    ("update", "f16"): 4 + 4 + 1 + 4 + 4,
+    ("inv", "f16"): 8,
+    ("fwd", "f16"): 8,
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(name):
+    from qoraphasecu_amd import workloads
+
+    if name == "c2":
+        obj, inten, mask = workloads.problem_2d(256, seed=1000, photons=1e7, beamstop_radius=6)
+        recipe = workloads.raar_shrinkwrap_recipe(10, 50, 200, method="max")
+        return dict(name="c2", shape=(256, 256), starts=32, intensity=inten, mask=mask, recipe=recipe,
+                    desc="256^2, 32 starts, repeat(10, RAAR(50, 0.9), Shrinkwrap(max)) + ER(200), "
+                         "Poisson 1e7 photons, 6 px beamstop, autocorrelation support")
+    if name == "c4":
+        obj, inten, mask = workloads.problem_3d(128, seed=2000)
+        recipe = workloads.raar_shrinkwrap_recipe(8, 50, 200, method="rms")
+        return dict(name="c4", shape=(128, 128, 128), starts=8, intensity=inten, mask=mask, recipe=recipe,
+                    desc="128^3, 8 starts, repeat(8, RAAR(50, 0.9), Shrinkwrap(rms)) + ER(200), "
+                         "noiseless, 1% random-pixel mask, autocorrelation support")
+    raise ValueError(name)
+
+
+def time_variant(w, variant, batch, steps, warmup):
+    from qoraphasecu_amd import _core
+
+    ops = w["recipe"].ops()
+    iters = w["recipe"].iterations()
+    s = _core.Solver(w["intensity"], None, w["mask"], w["starts"], batch, 0, None, 0.1, "hip", variant, 10)
+    t0 = time.perf_counter()
+    s.run(ops)  # first run: graph capture included
+    first = time.perf_counter() - t0
+    final = [float(r[2][-1]) for r in s.results()]
+    for _ in range(max(0, warmup - 1)):
+        s.run(ops)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.run(ops)  # run() returns after the solver's stream has drained
+    wall = time.perf_counter() - t0
+    ms = wall / steps * 1e3
+    rate = w["starts"] * iters * steps / wall
+    log(f"  {w['name']} {variant:>8} batch={batch}: {ms:9.2f} ms/run  {rate:12.4g} start-it/s  "
+        f"(first run {first * 1e3:.1f} ms, best final error {min(final):.3e})")
+    return dict(ms_per_run=ms, start_iterations_per_s=rate, first_run_ms=first * 1e3,
+                best_final_error=min(final), final_errors=final)
+
+
+def pass_roofline(w, storage, reps=50):
+    """Average duration of each pass of one batched iteration, HIP events on the solver stream."""
+    from qoraphasecu_amd import _core, _qpch
+
+    amp, mask, sup = _core.make_problem(w["intensity"], None, w["mask"], 0.1)
+    f = np.fft.ifftshift
+    s = _qpch.Solver(f(amp).copy(), f(mask).copy(), f(sup).copy(), starts=w["starts"], batch=0,
+                     storage=storage, mixed=storage != "f32", schedule="graph")
+    low = storage != "f32"
+    passes = s.time_passes(low=low, reps=reps)
+    s.close()
+    n = int(np.prod(w["shape"]))
+    key = "f16" if low else "f32"
+    out = []
+    for axis, mode, ms in passes:
+        b = PASS_BYTES[(mode, key)] * n * w["starts"]
+        out.append(dict(axis=axis, mode=mode, ms=ms, algorithmic_bytes=b, achieved_gbs=b / (ms * 1e-3) / 1e9))
+    return out
+
+
+def cpu_baseline(w, threads):
+    """The oracle (float, OpenMP) on a bounded sample: max(5, iters/20) RAAR iterations for
+    every start of the workload, after one untimed iteration (bench/bench.cpp:148)."""
+    import oracle
+
+    oracle.set_threads(threads)
+    iters = max(5, w["recipe"].iterations() // 20)
+    amp, mask, sup = oracle.make_problem(w["intensity"], None, w["mask"])
+    starts = w["starts"]
+    solver = oracle.Solver(amp, mask, sup, starts=starts, seed=0)
+    solver.run([("raar", 1, 0.9, 0.0, 0.0, "max", 0)])
+    t0 = time.perf_counter()
+    solver.run([("raar", iters, 0.9, 0.0, 0.0, "max", 0)])
+    wall = time.perf_counter() - t0
+    return dict(value=starts * iters / wall, starts=starts, iterations=iters, seconds=wall, threads=threads)
+
+
+def numpy_fft_pair(shape, reps=20):
+    x = (np.random.default_rng(0).standard_normal(shape) + 0j).astype(np.complex64)
+    np.fft.ifftn(np.fft.fftn(x))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        np.fft.ifftn(np.fft.fftn(x))
+    return (time.perf_counter() - t0) / reps * 1e3
+
+
+def traffic_for(kernel_key):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, when present."""
+    path = os.path.join(HERE, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        table = json.load(fh)
+    v = table.get(kernel_key)
+    return None if v is None else v.get("hbm_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--configs", default="c2,c4")
+    ap.add_argument("--variants", default="graph,mixed,mixed_bf16")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (starts shard across GPUs "
+              "with no per-iteration exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import qoraphasecu_amd as qpc
+
+    info = qpc.hip_device_info()
+    log(f"device: {info}")
+    details = {"device": info}
+    headline = None
+    roof = None
+    for name in [c for c in args.configs.split(",") if c]:
+        w = workload(name)
+        log(f"{name}: {w['desc']}")
+        res = {"workload": w["desc"], "iterations": w["recipe"].iterations(), "starts": w["starts"]}
+        for variant in [v for v in args.variants.split(",") if v]:
+            batch = 1 if variant in ("graph", "fused", "naive", "callback") else 0
+            res[variant] = time_variant(w, variant, batch, args.steps, args.warmup)
+        if "graph" in res:
+            for v in ("mixed", "mixed_bf16", "half", "bf16", "batched"):
+                if v in res:
+                    res[f"speedup_{v}_vs_plain"] = res[v]["start_iterations_per_s"] / res["graph"]["start_iterations_per_s"]
+        n = int(np.prod(w["shape"]))
+        for v in ("mixed", "mixed_bf16", "batched", "graph"):
+            if v not in res:
+                continue
+            # whole-run HBM model: RAAR iterations at the storage's bytes, ER at 53 B
+            iters_total = w["recipe"].iterations()
+            er = 200
+            low_b = 29 if v in ("mixed", "mixed_bf16", "half", "bf16") else 53
+            model_bytes = n * w["starts"] * ((iters_total - er) * low_b + er * 53)
+            gbs = model_bytes / (res[v]["ms_per_run"] * 1e-3) / 1e9
+            res[v]["model_gbs"] = gbs
+            res[v]["model_hbm_frac"] = gbs / PEAK_HBM_GBS
+        passes16 = pass_roofline(w, "f16")
+        passes32 = pass_roofline(w, "f32")
+        res["passes_f16"] = passes16
+        res["passes_f32"] = passes32
+        for p in passes16:
+            log(f"    f16 pass axis {p['axis']} {p['mode']:>8}: {p['ms'] * 1e3:8.1f} us  {p['achieved_gbs']:8.0f} GB/s")
+        for p in passes32:
+            log(f"    f32 pass axis {p['axis']} {p['mode']:>8}: {p['ms'] * 1e3:8.1f} us  {p['achieved_gbs']:8.0f} GB/s")
+        if not args.no_cpu:
+            threads = min(16, os.cpu_count() or 1)
+            cb = cpu_baseline(w, threads)
+            res["cpu_oracle"] = cb
+            res["numpy_fft_pair_ms"] = numpy_fft_pair(w["shape"])
+            log(f"    cpu oracle ({threads} threads): {cb['value']:.4g} start-it/s; numpy fft pair "
+                f"{res['numpy_fft_pair_ms']:.2f} ms")
+        details[name] = res
+        if name == "c2":
+            headline = res
+            dom = max(passes16, key=lambda p: p["ms"])
+            roof = {
+                "bound": "hbm",
+                "achieved": dom["achieved_gbs"],
+                "peak": PEAK_HBM_GBS,
+                "unit": "GB/s",
+                "frac": dom["achieved_gbs"] / PEAK_HBM_GBS,
+                "traffic": traffic_for(f"c2_f16_{dom['mode']}"),
+                "kernel": f"line_pass {dom['mode']} (axis {dom['axis']}, fp16 storage, 32 starts per launch)",
+                "algorithmic_bytes_per_launch": dom["algorithmic_bytes"],
+                "avg_launch_ms": dom["ms"],
+            }
+
+    if headline is None:
+        print(json.dumps({"details": details}))
+        return
+    best = "mixed" if "mixed" in headline else [k for k in ("mixed_bf16", "batched", "graph") if k in headline][0]
+    h = headline[best]
+    cpu = headline.get("cpu_oracle")
+    out = {
+        "metric": "RAAR iters/s (all starts) at 256² & 128³, batched/fp16 vs plain fp32; %HBM",
+        "value": h["start_iterations_per_s"],
+        "unit": "start-iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": h["ms_per_run"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "storage": "f16 (ER in f32)" if best == "mixed" else best,
+        "data": "synthetic (seeded ellipse objects, Poisson counts)",
+        "config": {
+            "workload": headline["workload"] + f"; variant {best}, all starts in one launch",
+            "starts": headline["starts"],
+            "iterations": headline["iterations"],
+            "shape": [256, 256],
+        },
+        "speedup_vs_plain_fp32": headline.get(f"speedup_{best}_vs_plain"),
+        "model_hbm_frac": h.get("model_hbm_frac"),
+        "roofline": roof,
+        "cpu_baseline": None if cpu is None else {
+            "value": cpu["value"],
+            "unit": "start-iterations/s",
+            "cores": cpu["threads"],
+            "kind": "port",
+            "sample": f"oracle (C, float, OpenMP) RAAR x{cpu['iterations']} on all {cpu['starts']} starts "
+                      f"of the 256^2 workload ({cpu['seconds']:.1f} s)",
+        },
+        "details": details,
+    }
+    print(json.dumps(out, default=float))
+
+
+if __name__ == "__main__":
+    main()

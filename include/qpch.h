@@ -113,6 +113,15 @@ int qpch_solver_read_state(qpch_solver* solver, float* host);
 double qpch_solver_energy(const qpch_solver* solver);
 int qpch_solver_synchronize(qpch_solver* solver);
 
+/* Measurement: the average duration (ms) of each pass of one iteration of the first
+ * launch group, every pass replayed `reps` times between two hipEvents on the solver's
+ * stream (low = 1: the reduced-precision passes).  Changes the iterate: time a solver
+ * whose state is not needed afterwards.  No reference counterpart; the reference's
+ * equivalent is its qpcu_profile tool (bench/profile.cu:15-36). */
+int qpch_solver_time_passes(qpch_solver* solver, int low, int reps, float* ms, int cap, int* count);
+/* the passes of one iteration: axis and mode (0 fwd, 1 inv, 2 modulus, 3 update) */
+int qpch_solver_pass_info(qpch_solver* solver, int low, int* axes, int* modes, int cap, int* count);
+
 /* --- operators on host arrays (synchronous) --------------------------------------------- */
 
 /* unitary n-D transform in place (complex, FFT order) */

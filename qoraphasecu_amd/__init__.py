@@ -1,0 +1,42 @@
+"""QoraPhase-MI355X: the phase-retrieval reconstruction path of QoraPhaseCU on AMD
+Instinct MI355X (gfx950), behind the same ``reconstruct`` API.
+
+    import qoraphasecu_amd as qpc
+    result = qpc.reconstruct(intensity, support=support, backend="hip")
+
+Arrays are centre-origin, ``mask=True`` marks an unusable pixel, ``intensity`` is a
+measured intensity.  The native parts are built in-tree by ``qoraphasecu_amd.build``.
+"""
+
+from .api import (
+    Ensemble,
+    Reconstruction,
+    aligned_error,
+    autocorrelation_support,
+    cuda_available,
+    hip_available,
+    hip_device_info,
+    reconstruct,
+    simulate_intensity,
+)
+from .recipe import ER, HIO, RAAR, Recipe, Shrinkwrap, default_recipe, detwin, repeat
+
+__all__ = [
+    "ER",
+    "HIO",
+    "RAAR",
+    "Ensemble",
+    "Recipe",
+    "Reconstruction",
+    "Shrinkwrap",
+    "aligned_error",
+    "autocorrelation_support",
+    "cuda_available",
+    "default_recipe",
+    "detwin",
+    "hip_available",
+    "hip_device_info",
+    "reconstruct",
+    "repeat",
+    "simulate_intensity",
+]

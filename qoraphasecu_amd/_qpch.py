@@ -84,6 +84,8 @@ def lib():
         L.qpch_solver_energy.argtypes = [vp]
         L.qpch_solver_energy.restype = d
         L.qpch_solver_synchronize.argtypes = [vp]
+        L.qpch_solver_time_passes.argtypes = [vp, i, i, ctypes.POINTER(f), i, ctypes.POINTER(i)]
+        L.qpch_solver_pass_info.argtypes = [vp, i, ctypes.POINTER(i), ctypes.POINTER(i), i, ctypes.POINTER(i)]
         L.qpch_fft.argtypes = [vp, i, szp, i]
         L.qpch_modulus_project.argtypes = [vp, vp, vp, i, szp, ctypes.POINTER(d)]
         L.qpch_support_project.argtypes = [vp, vp, sz]
@@ -239,6 +241,17 @@ class Solver:
         out = np.empty((self.starts, *self.shape), np.complex64)
         check(lib().qpch_solver_read_state(self._h, _ptr(out)))
         return out
+
+    def time_passes(self, low=False, reps=20):
+        """[(axis, mode, ms)] for each pass of one iteration."""
+        ms = (ctypes.c_float * 8)()
+        axes = (ctypes.c_int * 8)()
+        modes = (ctypes.c_int * 8)()
+        cnt = ctypes.c_int(0)
+        check(lib().qpch_solver_pass_info(self._h, int(low), axes, modes, 8, ctypes.byref(cnt)))
+        check(lib().qpch_solver_time_passes(self._h, int(low), reps, ms, 8, ctypes.byref(cnt)))
+        names = {0: "fwd", 1: "inv", 2: "modulus", 3: "update"}
+        return [(axes[k], names[modes[k]], float(ms[k])) for k in range(cnt.value)]
 
     def synchronize(self):
         check(lib().qpch_solver_synchronize(self._h))

@@ -1,0 +1,106 @@
+"""In-tree build of the native parts of qoraphasecu_amd.
+
+  libqpch.so           HIP engine + C ABI (hipcc, gfx950)
+  libqoraphasecu.so    the qpcu:: C++ API over the C ABI (g++ -std=c++20)
+  _core.<ext>.so       the Python module (pybind11), over libqoraphasecu.so
+
+Every call compiles from source; nothing is fetched.  `python qoraphasecu_amd/build.py`
+builds everything.
+"""
+
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import sysconfig
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+INCLUDE = os.path.join(ROOT, "include")
+CSRC = os.path.join(HERE, "csrc")
+ARCH = os.environ.get("QPCH_OFFLOAD_ARCH", "gfx950")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++20",
+    "-fPIC",
+    # packed fp32 instructions stay off: the compiler must not emit v_pk_{add,mul,fma}_f32
+    "-Xclang",
+    "-target-feature",
+    "-Xclang",
+    "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+HOST_SOURCES = ["problem.cpp", "recipe.cpp", "solver.cpp", "hip_solver.cpp", "operators.cpp", "metrics.cpp"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{out}")
+    return out
+
+
+def engine_lib() -> str:
+    return os.path.join(HERE, "libqpch.so")
+
+
+def host_lib() -> str:
+    return os.path.join(HERE, "libqoraphasecu.so")
+
+
+def core_ext() -> str:
+    return os.path.join(HERE, "_core" + sysconfig.get_config_var("EXT_SUFFIX"))
+
+
+def build_engine() -> str:
+    out = engine_lib()
+    tmp = out + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", os.path.join(CSRC, "hip", "engine.hip"), "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_host() -> str:
+    out = host_lib()
+    tmp = out + ".tmp"
+    srcs = [os.path.join(CSRC, "host", s) for s in HOST_SOURCES]
+    _run([CXX, "-O2", "-std=c++20", "-fPIC", "-shared", "-Wall", f"-I{INCLUDE}", *srcs, "-o", tmp,
+          f"-L{HERE}", "-lqpch", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, out)
+    return out
+
+
+def build_core() -> str:
+    import pybind11
+
+    out = core_ext()
+    tmp = out + ".tmp"
+    _run([CXX, "-O2", "-std=c++20", "-fPIC", "-shared", f"-I{INCLUDE}", f"-I{pybind11.get_include()}",
+          f"-I{sysconfig.get_paths()['include']}", os.path.join(CSRC, "python", "bindings.cpp"), "-o", tmp,
+          f"-L{HERE}", "-lqoraphasecu", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, out)
+    return out
+
+
+def build() -> list[str]:
+    """Compile the engine and the host library in parallel, then the Python module."""
+    with ThreadPoolExecutor(max_workers=1) as pool:
+        eng = pool.submit(build_engine)
+        eng.result()
+    built = [engine_lib(), build_host(), build_core()]
+    return built
+
+
+if __name__ == "__main__":
+    for p in build():
+        print(p)
+    sys.exit(0)

@@ -18,7 +18,7 @@
 #include <tuple>
 #include <vector>
 
-#include "../../../include/qpch.h"
+#include "qpch.h"
 #include "generic.hpp"
 #include "misc.hpp"
 #include "passes.hpp"
@@ -719,6 +719,43 @@ struct SolverImpl {
     QPCH_HIP(hipStreamSynchronize(stream));
   }
 
+  // Average duration of each pass of one iteration (group 0), each replayed `reps` times
+  // between two events on the solver's stream.  Leaves the iterate changed.
+  int time_passes(bool lowp, int reps, float* ms, int cap) {
+    if (!fused) throw InvalidArgument("pass timing needs the fused schedule");
+    if (lowp && !low()) throw InvalidArgument("this solver has no reduced-precision storage");
+    if (err_cap <= 0) begin_run(1);
+    set_precision(lowp);
+    if (!wave_valid) {
+      if (!lowp) fused_prologue<float2>(0);
+      else if (storage == QPCH_F16) fused_prologue<__half2>(0);
+      else fused_prologue<__hip_bfloat162>(0);
+    }
+    const auto passes = iteration_passes(lowp);
+    hipEvent_t e0{}, e1{};
+    QPCH_HIP(hipEventCreate(&e0));
+    QPCH_HIP(hipEventCreate(&e1));
+    int k = 0;
+    for (const auto& ps : passes) {
+      if (k >= cap) break;
+      QPCH_HIP(hipEventRecord(e0, stream));
+      for (int r = 0; r < reps; ++r) {
+        if (!lowp) run_pass<float2>(ps, 0, batch, 0.9f, -0.8f);
+        else if (storage == QPCH_F16) run_pass<__half2>(ps, 0, batch, 0.9f, -0.8f);
+        else run_pass<__hip_bfloat162>(ps, 0, batch, 0.9f, -0.8f);
+      }
+      QPCH_HIP(hipEventRecord(e1, stream));
+      QPCH_HIP(hipEventSynchronize(e1));
+      float t = 0.0f;
+      QPCH_HIP(hipEventElapsedTime(&t, e0, e1));
+      ms[k++] = t / static_cast<float>(reps);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    wave_valid = false;
+    return k;
+  }
+
   void read_state(float* host) {
     if (low_active) convert_x(false);
     QPCH_HIP(hipStreamSynchronize(stream));
@@ -764,6 +801,11 @@ int qpch_device_info(qpch_device_info_t* info) {
     QPCH_HIP(hipGetDeviceProperties(&prop, dev));
     std::memset(info, 0, sizeof(*info));
     std::snprintf(info->name, sizeof(info->name), "%s", prop.name);
+    if (info->name[0] == '\0') (void)hipDeviceGetName(info->name, sizeof(info->name), dev);
+    if (info->name[0] == '\0') {
+      std::string arch(prop.gcnArchName);
+      std::snprintf(info->name, sizeof(info->name), "AMD GPU %s", arch.substr(0, arch.find(':')).c_str());
+    }
     std::snprintf(info->arch, sizeof(info->arch), "%s", prop.gcnArchName);
     info->compute_units = prop.multiProcessorCount;
     info->memory_bytes = prop.totalGlobalMem;
@@ -849,6 +891,27 @@ int qpch_solver_read_state(qpch_solver* s, float* host) {
 }
 
 double qpch_solver_energy(const qpch_solver* s) { return s->impl.energy; }
+
+int qpch_solver_time_passes(qpch_solver* s, int low, int reps, float* ms, int cap, int* count) {
+  return guarded([&] {
+    if (reps < 1) throw qpch::InvalidArgument("reps must be >= 1");
+    *count = s->impl.time_passes(low != 0, reps, ms, cap);
+  });
+}
+
+int qpch_solver_pass_info(qpch_solver* s, int low, int* axes, int* modes, int cap, int* count) {
+  return guarded([&] {
+    const auto passes = s->impl.iteration_passes(low != 0);
+    int k = 0;
+    for (const auto& ps : passes) {
+      if (k >= cap) break;
+      axes[k] = ps.axis;
+      modes[k] = static_cast<int>(ps.mode);
+      ++k;
+    }
+    *count = k;
+  });
+}
 
 int qpch_solver_synchronize(qpch_solver* s) {
   return guarded([&] { QPCH_HIP(hipStreamSynchronize(s->impl.stream)); });

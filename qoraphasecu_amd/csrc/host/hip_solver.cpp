@@ -1,0 +1,195 @@
+// Backend::hip: the recipe walk, progress and result layout on the host; every array
+// operation behind the qpch C ABI.  Reference counterpart: the CudaSolver of
+// src/cuda/cuda_solver.cu:58-500 (run 145-163, results 165-191, apply 236-276,
+// iterate 300-338, collect_errors/report 423-453).
+#include <cmath>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <variant>
+
+#include "abi.hpp"
+#include "backends.hpp"
+#include "layout.hpp"
+
+namespace qpcu::detail {
+
+namespace {
+
+constexpr double kEps = 1e-12;  // src/core/pointwise.hpp:16
+
+struct Plan {
+  int schedule;
+  int storage;
+  int mixed;
+  bool batched;
+};
+
+Plan plan_for(Variant v) {
+  switch (v) {
+    case Variant::naive: return {QPCH_UNFUSED, QPCH_F32, 0, false};
+    case Variant::fused: return {QPCH_FUSED, QPCH_F32, 0, false};
+    case Variant::graph: return {QPCH_GRAPH, QPCH_F32, 0, false};
+    case Variant::callback: return {QPCH_GRAPH, QPCH_F32, 0, false};
+    case Variant::batched: return {QPCH_GRAPH, QPCH_F32, 0, true};
+    case Variant::half: return {QPCH_GRAPH, QPCH_F16, 0, true};
+    case Variant::mixed: return {QPCH_GRAPH, QPCH_F16, 1, true};
+    case Variant::bf16: return {QPCH_GRAPH, QPCH_BF16, 0, true};
+    case Variant::mixed_bf16: return {QPCH_GRAPH, QPCH_BF16, 1, true};
+  }
+  throw std::invalid_argument("unknown variant");
+}
+
+// x <- S ? P_M x : a x + b P_M x; src/core/pointwise.hpp:41-45
+struct Coefficients {
+  float a, b;
+};
+
+class HipSolver final : public Solver {
+ public:
+  HipSolver(const Problem& problem, const Options& options)
+      : shape_(problem.shape), n_(numel(problem.shape)), options_(options), starts_(options.starts),
+        errors_(static_cast<std::size_t>(options.starts)) {
+    const Plan plan = plan_for(options.variant);
+    const auto amplitude = to_fft_order<float>(problem.amplitude, shape_);
+    const auto mask = to_fft_order<std::uint8_t>(problem.mask, shape_);
+    const auto support = to_fft_order<std::uint8_t>(problem.support, shape_);
+    std::vector<Complex> start;
+    if (!options.start.empty()) {
+      start.reserve(options.start.size());
+      for (int s = 0; s < starts_; ++s) {
+        const auto part = to_fft_order<Complex>(
+            std::span(options.start).subspan(static_cast<std::size_t>(s) * n_, n_), shape_);
+        start.insert(start.end(), part.begin(), part.end());
+      }
+    }
+    qpch_problem p{};
+    p.ndim = static_cast<int>(shape_.size());
+    for (std::size_t a = 0; a < shape_.size(); ++a) p.shape[a] = shape_[a];
+    p.amplitude = amplitude.data();
+    p.mask = mask.data();
+    p.support = support.data();
+    p.starts = starts_;
+    p.batch = plan.batched ? options.batch : 1;
+    p.seed = options.seed;
+    p.start = start.empty() ? nullptr : reinterpret_cast<const float*>(start.data());
+    p.storage = plan.storage;
+    p.mixed = plan.mixed;
+    p.schedule = plan.schedule;
+    qpch_check(qpch_solver_create(&p, &h_));
+    energy_ = qpch_solver_energy(h_);
+  }
+
+  ~HipSolver() override { qpch_solver_destroy(h_); }
+
+  HipSolver(const HipSolver&) = delete;
+  HipSolver& operator=(const HipSolver&) = delete;
+
+  void run(const Recipe& recipe) override {
+    validate(recipe);
+    const int capacity = recipe.iterations();
+    run_iterations_ = 0;
+    if (capacity > 0) qpch_check(qpch_solver_begin_run(h_, capacity));
+    for (const Op& op : recipe.ops) {
+      std::visit([this](const auto& o) { apply(o); }, op);
+    }
+    if (run_iterations_ > 0) collect_errors();
+    qpch_check(qpch_solver_synchronize(h_));
+  }
+
+  std::vector<Reconstruction> results() const override {
+    std::vector<Complex> x(n_ * static_cast<std::size_t>(starts_));
+    std::vector<std::uint8_t> s(n_ * static_cast<std::size_t>(starts_));
+    qpch_check(qpch_solver_read_state(h_, reinterpret_cast<float*>(x.data())));
+    qpch_check(qpch_solver_get_support(h_, s.data()));
+    std::vector<Reconstruction> out;
+    out.reserve(static_cast<std::size_t>(starts_));
+    for (int k = 0; k < starts_; ++k) {
+      const std::size_t off = static_cast<std::size_t>(k) * n_;
+      out.push_back({to_centered<Complex>(std::span(x).subspan(off, n_), shape_),
+                     to_centered<std::uint8_t>(std::span(s).subspan(off, n_), shape_),
+                     errors_[static_cast<std::size_t>(k)]});
+    }
+    return out;
+  }
+
+ private:
+  void apply(const ER& o) { iterate(o.n, {0.0f, 0.0f}, true); }
+  void apply(const HIO& o) { iterate(o.n, {1.0f, -o.beta}, false); }
+  void apply(const RAAR& o) { iterate(o.n, {o.beta, 1.0f - 2.0f * o.beta}, false); }
+
+  void apply(const Shrinkwrap& o) {
+    qpch_check(qpch_solver_shrinkwrap(h_, o.threshold, o.sigma, o.method == Threshold::max ? 0 : 1));
+  }
+
+  // Once per recipe: a host round trip is cheaper than a kernel for it.
+  void apply(const DetwinBegin& o) {
+    qpch_check(qpch_solver_save_support(h_));
+    std::vector<std::uint8_t> host(n_ * static_cast<std::size_t>(starts_));
+    qpch_check(qpch_solver_get_support(h_, host.data()));
+    for (int k = 0; k < starts_; ++k) {
+      const auto part = std::span(host).subspan(static_cast<std::size_t>(k) * n_, n_);
+      const auto centred = to_centered<std::uint8_t>(part, shape_);
+      const auto half = to_fft_order<std::uint8_t>(half_support(centred, shape_, o.axis), shape_);
+      std::copy(half.begin(), half.end(), part.begin());
+    }
+    qpch_check(qpch_solver_set_support(h_, host.data()));
+  }
+
+  void apply(const DetwinEnd&) { qpch_check(qpch_solver_restore_support(h_)); }
+
+  // Chunks end on multiples of sync_every of the solver's own iteration count, so the
+  // progress schedule does not depend on how the recipe splits into stages.
+  void iterate(int n, Coefficients c, bool refine) {
+    for (int done = 0; done < n;) {
+      const int len = std::min(n - done, options_.sync_every - iteration_ % options_.sync_every);
+      qpch_check(qpch_solver_iterate(h_, len, c.a, c.b, refine ? 1 : 0));
+      done += len;
+      iteration_ += len;
+      run_iterations_ += len;
+      if (options_.progress && iteration_ % options_.sync_every == 0) report();
+    }
+  }
+
+  float to_error(double numerator) const {
+    return static_cast<float>(std::sqrt(numerator / (energy_ + kEps)));
+  }
+
+  void collect_errors() {
+    std::vector<double> host(static_cast<std::size_t>(run_iterations_) * starts_);
+    qpch_check(qpch_solver_read_errors(h_, 0, run_iterations_, host.data()));
+    for (int t = 0; t < run_iterations_; ++t) {
+      for (int k = 0; k < starts_; ++k) {
+        errors_[static_cast<std::size_t>(k)].push_back(
+            to_error(host[static_cast<std::size_t>(t) * starts_ + k]));
+      }
+    }
+  }
+
+  void report() {
+    std::vector<double> host(static_cast<std::size_t>(starts_));
+    qpch_check(qpch_solver_read_errors(h_, run_iterations_ - 1, 1, host.data()));
+    std::vector<float> latest(static_cast<std::size_t>(starts_));
+    for (int k = 0; k < starts_; ++k) latest[static_cast<std::size_t>(k)] = to_error(host[static_cast<std::size_t>(k)]);
+    options_.progress(iteration_, latest);
+  }
+
+  Shape shape_;
+  std::size_t n_;
+  Options options_;
+  int starts_;
+  qpch_solver* h_ = nullptr;
+  double energy_ = 0.0;
+  std::vector<std::vector<float>> errors_;
+  int iteration_ = 0;
+  int run_iterations_ = 0;
+};
+
+}  // namespace
+
+std::unique_ptr<Solver> make_hip_solver(const Problem& problem, const Options& options) {
+  if (!hip_available()) throw std::runtime_error("no HIP device available");
+  return std::make_unique<HipSolver>(problem, options);
+}
+
+}  // namespace qpcu::detail

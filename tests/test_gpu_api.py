@@ -1,0 +1,167 @@
+"""The product API (qpc.reconstruct -> C++ qpcu::reconstruct -> Backend::hip) against the
+oracle and the reference's own recovery and precision bars (tests/test_cuda.cpp,
+tests/test_recover.cpp, python/tests/test_api.py)."""
+
+import numpy as np
+import pytest
+
+import oracle
+import synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def qpc():
+    import qoraphasecu_amd as qpc
+
+    assert qpc.hip_available(), "no HIP device visible"
+    return qpc
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("variant", ["naive", "fused", "graph", "batched", "callback"])
+def test_variant_matches_oracle(qpc, variant):
+    # python/tests/test_api.py:62-73 (rtol 1e-3), with the supports bit-exact
+    obj, sup, inten = synthetic.triangle((64, 64), 24)
+    recipe = qpc.Recipe(qpc.HIO(20), qpc.Shrinkwrap(), qpc.ER(10))
+    want = oracle.reconstruct(inten, recipe.ops(), support=sup, starts=2)
+    got = qpc.reconstruct(inten, support=sup.astype(bool), recipe=recipe, starts=2, backend="hip", variant=variant)
+    for g, (o_obj, o_sup, o_err) in zip(got, want):
+        np.testing.assert_allclose(g.errors, o_err, rtol=1e-3)
+        assert np.linalg.norm(g.object - o_obj) < 1e-3 * np.linalg.norm(o_obj)
+        assert np.array_equal(g.support, o_sup)
+
+
+def test_odd_length_agrees_with_oracle(qpc):
+    # tests/test_cuda.cpp:90-102
+    obj, sup, inten = synthetic.triangle((21, 15), 6)
+    recipe = qpc.Recipe(qpc.HIO(10), qpc.Shrinkwrap(), qpc.ER(10))
+    want = oracle.reconstruct(inten, recipe.ops(), support=sup, starts=2)
+    got = qpc.reconstruct(inten, support=sup.astype(bool), recipe=recipe, starts=2, backend="hip")
+    for g, (o_obj, _, o_err) in zip(got, want):
+        assert abs(g.error - o_err[-1]) <= 1e-3 * o_err[-1]
+        assert rel(g.object, o_obj) < 1e-3
+
+
+def test_given_start_matches_oracle(qpc):
+    # tests/test_cuda.cpp:104-119
+    obj, sup, inten = synthetic.triangle((40, 36), 10)
+    start = np.stack([synthetic.random_field(obj.size, 11).reshape(obj.shape),
+                      synthetic.random_field(obj.size, 12).reshape(obj.shape)])
+    recipe = qpc.Recipe(qpc.HIO(5), qpc.RAAR(5), qpc.ER(5))
+    want = oracle.reconstruct(inten, recipe.ops(), support=sup, starts=2, start=start)
+    for variant in ("graph", "batched"):
+        got = qpc.reconstruct(inten, support=sup.astype(bool), recipe=recipe, starts=2, init=start, backend="hip",
+                              variant=variant)
+        for g, (o_obj, _, o_err) in zip(got, want):
+            assert rel(g.object, o_obj) < 1e-4
+            assert abs(g.error - o_err[-1]) <= 1e-4 * o_err[-1] + 1e-7
+
+
+def test_default_recipe_recovers_without_support(qpc):
+    # tests/test_recover.cpp:69-76 and python/tests/test_api.py:18-24
+    obj, _, inten = synthetic.triangle((64, 64), 24)
+    ens = qpc.reconstruct(inten, starts=8, backend="hip")
+    assert isinstance(ens, qpc.Ensemble) and len(ens) == 8
+    assert ens.best.error < 1e-3
+    assert qpc.aligned_error(ens.best.object, obj) < 1e-2
+    assert ens.ranking[0] == min(range(8), key=lambda i: ens[i].error)
+
+
+def test_recovers_3d(qpc):
+    # tests/test_cuda.cpp:139-148
+    obj, sup, inten = synthetic.triangle((32, 32, 32), 10)
+    runs = qpc.reconstruct(inten, support=sup.astype(bool), recipe=qpc.Recipe(qpc.RAAR(300), qpc.ER(100)),
+                           starts=2, backend="hip")
+    assert min(qpc.aligned_error(r.object, obj) for r in runs) < 1e-2
+
+
+def test_batched_equals_looped(qpc):
+    # tests/test_cuda.cpp:160-180
+    obj, sup, inten = synthetic.triangle((64, 64), 20)
+    recipe = qpc.Recipe(qpc.RAAR(40), qpc.repeat(3, qpc.HIO(20), qpc.Shrinkwrap(method="rms")), qpc.ER(20))
+    a = qpc.reconstruct(inten, recipe=recipe, starts=4, seed=9, backend="hip", variant="graph")
+    b = qpc.reconstruct(inten, recipe=recipe, starts=4, seed=9, backend="hip", variant="batched")
+    c = qpc.reconstruct(inten, recipe=recipe, starts=4, seed=9, batch=2, backend="hip", variant="batched")
+    for k in range(4):
+        assert rel(c[k].object, b[k].object) < 1e-4
+        assert rel(b[k].object, a[k].object) < 1e-4
+        assert abs(b[k].error - a[k].error) <= 1e-4 * a[k].error + 1e-7
+        assert np.array_equal(b[k].support, a[k].support)
+
+
+def test_half_rejects_non_power_of_two(qpc):
+    # tests/test_cuda.cpp:182-190
+    obj, sup, inten = synthetic.triangle((48, 64), 12)
+    for v in ("half", "mixed", "bf16", "mixed_bf16"):
+        with pytest.raises(ValueError):
+            qpc.reconstruct(inten, support=sup.astype(bool), recipe=qpc.Recipe(qpc.ER(1)), backend="hip", variant=v)
+
+
+def test_given_start_is_in_the_data_scale(qpc):
+    # tests/test_cuda.cpp:194-204
+    obj, sup, inten = synthetic.triangle((64, 64), 24)
+    for v in ("half", "mixed", "bf16", "mixed_bf16"):
+        run = qpc.reconstruct(inten, support=sup.astype(bool), recipe=qpc.Recipe(qpc.ER(3)), init=obj,
+                              backend="hip", variant=v)
+        floor = 3e-3 if "bf16" not in v else 2e-2
+        assert run.error < floor, (v, run.error)
+        assert rel(run.object, obj) < (5e-3 if "bf16" not in v else 3e-2), v
+
+
+def test_half_recovers_to_its_floor(qpc):
+    # tests/test_cuda.cpp:208-217
+    obj, sup, inten = synthetic.triangle((64, 64), 24)
+    runs = qpc.reconstruct(inten, support=sup.astype(bool), recipe=qpc.Recipe(qpc.HIO(400), qpc.ER(100)), starts=4,
+                           backend="hip", variant="half")
+    best = runs.best
+    assert best.error < 3e-3
+    assert qpc.aligned_error(best.object, obj) < 5e-3
+
+
+@pytest.mark.parametrize("variant", ["mixed", "mixed_bf16"])
+def test_mixed_converges_past_the_half_floor(qpc, variant):
+    # tests/test_cuda.cpp:221-241
+    obj, sup, inten = synthetic.triangle((64, 64), 24)
+    runs = qpc.reconstruct(inten, support=sup.astype(bool), recipe=qpc.Recipe(qpc.HIO(400), qpc.ER(100)), starts=4,
+                           backend="hip", variant=variant)
+    assert runs.best.error < 1e-5
+    assert qpc.aligned_error(runs.best.object, obj) < 1e-4
+
+
+def test_mixed_converges_3d(qpc):
+    obj, sup, inten = synthetic.corner(64, 24)
+    runs = qpc.reconstruct(inten, support=sup.astype(bool), recipe=qpc.Recipe(qpc.HIO(400), qpc.ER(100)), starts=4,
+                           backend="hip", variant="mixed")
+    assert runs.best.error < 1e-5
+    assert qpc.aligned_error(runs.best.object, obj) < 1e-5
+
+
+def test_mixed_er_matches_graph(qpc):
+    # tests/test_cuda.cpp:244-257
+    obj, sup, inten = synthetic.triangle((64, 32), 12)
+    recipe = qpc.Recipe(qpc.ER(30), qpc.Shrinkwrap(), qpc.ER(10))
+    a = qpc.reconstruct(inten, support=sup.astype(bool), recipe=recipe, starts=2, seed=4, backend="hip", variant="graph")
+    b = qpc.reconstruct(inten, support=sup.astype(bool), recipe=recipe, starts=2, seed=4, backend="hip", variant="mixed")
+    for k in range(2):
+        assert rel(b[k].object, a[k].object) < 1e-4
+        assert abs(b[k].error - a[k].error) <= 1e-4 * a[k].error
+
+
+def test_engine_errors_surface_as_value_errors(qpc):
+    obj, sup, inten = synthetic.triangle((64, 64), 24)
+    with pytest.raises(ValueError):
+        qpc.reconstruct(inten, support=sup.astype(bool), recipe=qpc.Recipe(qpc.ER(0)), backend="hip")
+    with pytest.raises(ValueError):
+        qpc.reconstruct(inten, support=sup.astype(bool), starts=4, batch=3, backend="hip")
+
+
+def test_device_info(qpc):
+    info = qpc.hip_device_info()
+    assert info["name"]
+    assert info["major"] == 9 and info["minor"] == 50
+    assert info["peak_bandwidth_gbs"] > 10.0

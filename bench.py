@@ -154,10 +154,24 @@ def main():
     ap.add_argument("--configs", default="c2,c4")
     ap.add_argument("--variants", default="graph,mixed,mixed_bf16")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-passes", action="store_true", help="skip the per-pass timing")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB 10 times (the reference point of the "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        a = torch.ones(args.calibrate_copy * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": a.numel() * 4, "launches": 10}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (starts shard across GPUs "
-              "with no per-iteration exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+              "with no per-iteration exchange). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import qoraphasecu_amd as qpc
@@ -190,8 +204,8 @@ def main():
             gbs = model_bytes / (res[v]["ms_per_run"] * 1e-3) / 1e9
             res[v]["model_gbs"] = gbs
             res[v]["model_hbm_frac"] = gbs / PEAK_HBM_GBS
-        passes16 = pass_roofline(w, "f16")
-        passes32 = pass_roofline(w, "f32")
+        passes16 = [] if args.no_passes else pass_roofline(w, "f16")
+        passes32 = [] if args.no_passes else pass_roofline(w, "f32")
         res["passes_f16"] = passes16
         res["passes_f32"] = passes32
         for p in passes16:
@@ -208,6 +222,7 @@ def main():
         details[name] = res
         if name == "c2":
             headline = res
+        if name == "c2" and passes16:
             dom = max(passes16, key=lambda p: p["ms"])
             roof = {
                 "bound": "hbm",

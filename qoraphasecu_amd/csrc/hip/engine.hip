@@ -152,19 +152,26 @@ int grid_for(long long work, int threads = 256, int cap = 2048) {
 // ---------------------------------------------------------------------------------------
 // fused line-pass dispatch
 
-template <int N, bool COL, Mode M, class St>
-void launch_pass_t(const PassArgs& a, int starts, hipStream_t s) {
-  using G = dev::PassGeom<N, COL>;
-  const int gx = COL ? a.outer * ((a.inner + G::kB - 1) / G::kB) : (a.outer + G::kB - 1) / G::kB;
-  hipLaunchKernelGGL((dev::line_pass<N, COL, M, St>), dim3(gx, starts), dim3(G::kThreads), 0, s,
-                     a);
-  QPCH_HIP(hipGetLastError());
+// Blocks per start of a pass.
+template <int N, bool COL, int EB>
+int pass_tiles_t(int outer, int inner) {
+  using G = dev::PassGeom<N, COL, EB>;
+  return COL ? outer * ((inner + G::kB - 1) / G::kB) : (outer + G::kB - 1) / G::kB;
 }
 
-template <int N, bool COL>
-int pass_blocks_t(int outer, int inner) {
-  using G = dev::PassGeom<N, COL>;
-  return COL ? outer * ((inner + G::kB - 1) / G::kB) : (outer + G::kB - 1) / G::kB;
+// Starts of a launch go XCD-affine (a start's blocks all on one XCD) when they split
+// evenly over the eight XCDs; a single start keeps its blocks spread over the chip.
+int xcd_slots(int starts) { return (starts >= 8 && starts % 8 == 0) ? starts : 0; }
+
+template <int N, bool COL, Mode M, class St>
+void launch_pass_t(PassArgs a, int starts, hipStream_t s) {
+  using G = dev::PassGeom<N, COL, sizeof(St)>;
+  const int tiles = pass_tiles_t<N, COL, sizeof(St)>(a.outer, a.inner);
+  a.starts = starts;
+  a.slots = xcd_slots(starts);
+  const int gx = tiles * (a.slots > 0 ? a.slots : starts);
+  hipLaunchKernelGGL((dev::line_pass<N, COL, M, St>), dim3(gx), dim3(G::kThreads), 0, s, a);
+  QPCH_HIP(hipGetLastError());
 }
 
 template <int N, class St>
@@ -202,11 +209,12 @@ void launch_pass(int N, bool col, Mode m, const PassArgs& a, int starts, hipStre
   }
 }
 
-int pass_blocks(int N, bool col, int outer, int inner) {
+template <int EB>
+int pass_tiles(int N, bool col, int outer, int inner) {
   switch (N) {
 #define QPCH_CASE(V) \
   case V:            \
-    return col ? pass_blocks_t<V, true>(outer, inner) : pass_blocks_t<V, false>(outer, inner);
+    return col ? pass_tiles_t<V, true, EB>(outer, inner) : pass_tiles_t<V, false, EB>(outer, inner);
     QPCH_FOR_EACH_N(QPCH_CASE)
 #undef QPCH_CASE
     default:
@@ -357,7 +365,6 @@ struct SolverImpl {
   DevBuf<double> partials, errs, sw_part;
   DevBuf<int> counters;
   DevBuf<float> sw_a, sw_b;
-  int partial_stride = 1;
   int err_cap = 0;
   bool low_active = false;
   bool wave_valid = false;
@@ -431,7 +438,8 @@ struct SolverImpl {
     p.support = supp.get() + first * n;
     p.amp = amp_signed.get();
     p.partials = partials.get();
-    p.partial_stride = partial_stride;
+    p.partial_stride = pass_tiles<sizeof(St)>(dims.d[0], dims.inner(0) > 1, static_cast<int>(dims.outer(0)),
+                                              static_cast<int>(dims.inner(0)));
     p.errs = errs.get();
     p.err_stride = starts;
     p.err_offset = first;
@@ -689,12 +697,12 @@ struct SolverImpl {
       w16.resize(static_cast<size_t>(n) * starts);
     }
     counters.resize(groups);
-    // partial sums of the error: one per work-group of the modulus pass
+    // partial sums of the error: one per block of the modulus pass of each start
     int pb = grid_for(n, 256, 1024);
-    if (fused) pb = std::max(pb, pass_blocks(dims.d[0], dims.inner(0) > 1, static_cast<int>(dims.outer(0)), static_cast<int>(dims.inner(0))));
-    partial_stride = fused ? pass_blocks(dims.d[0], dims.inner(0) > 1, static_cast<int>(dims.outer(0)),
-                                         static_cast<int>(dims.inner(0)))
-                           : grid_for(n, 256, 1024);
+    if (fused) {
+      const int o = static_cast<int>(dims.outer(0)), in = static_cast<int>(dims.inner(0));
+      pb = std::max({pb, pass_tiles<8>(dims.d[0], in > 1, o, in), pass_tiles<4>(dims.d[0], in > 1, o, in)});
+    }
     partials.resize(static_cast<size_t>(pb) * batch);
 
     // starting objects

@@ -48,25 +48,30 @@ struct PassArgs {
   int err_offset;               // global index of this launch's first start
   int err_cap;                  // slots available
   int* counter;
+  int starts;                   // starts in this launch
+  int slots;                    // > 0: XCD-affine grid, block = tile * slots + start
   float a, b;                   // update coefficients
   float in_scale;               // applied after the first transform of the pass
   float out_scale;              // applied before the store (kModulus: after projection)
 };
 
-// Work-group geometry of a pass, shared by the kernel and its launcher.
-template <int N, bool COL>
+// Work-group geometry of a pass, shared by the kernel and its launcher.  EB is the
+// storage size of one complex element: column passes take B adjacent columns so that one
+// load instruction of a row covers 128 contiguous bytes.
+template <int N, bool COL, int EB>
 struct PassGeom {
   using S = Shape1<N, 16>;
   static constexpr int kT = S::kT;
   static constexpr int kR = S::kR;
-  static constexpr int kLdsLimit = 64 * 1024 / 8 - N;  // float2 slots for the line tiles
+  static constexpr int kLdsLimit = 96 * 1024 / 8 - N;  // float2 slots for the line tiles
   static constexpr int pow2_floor(int v) { return v <= 1 ? 1 : 2 * pow2_floor(v / 2); }
   static constexpr int clampB(int b) {
     int lim = kLdsLimit / S::kLine;
     if (1024 / kT < lim) lim = 1024 / kT;
     return pow2_floor(b < lim ? b : lim);
   }
-  static constexpr int kB = clampB(COL ? (256 / kT > 16 ? 256 / kT : 16) : (256 / kT > 1 ? 256 / kT : 1));
+  static constexpr int kColB = 128 / EB;
+  static constexpr int kB = clampB(COL ? (256 / kT > kColB ? 256 / kT : kColB) : (256 / kT > 1 ? 256 / kT : 1));
   static constexpr int kThreads = kB * kT;
   static constexpr int kTile = kB * S::kLine;  // float2
 };
@@ -112,10 +117,14 @@ __device__ __forceinline__ float2 project(float2 w, float a, double& acc) {
   return {w.x * f, w.y * f};
 }
 
+// Block numbering: with p.slots > 0 (a multiple of 8) block L handles start L % slots and
+// tile L / slots, so every block of a start lands on the same XCD (blocks are dealt to the
+// eight XCDs round-robin) and the start's arrays stay in that XCD's L2 from one pass to the
+// next; otherwise block L handles tile L % tiles of start L / tiles.
 template <int N, bool COL, Mode M, class St>
-__global__ void __launch_bounds__((PassGeom<N, COL>::kThreads))
+__global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St)>::kThreads))
     line_pass(PassArgs p) {
-  using G = PassGeom<N, COL>;
+  using G = PassGeom<N, COL, sizeof(St)>;
   using Io = Storage<St>;
   constexpr int kT = G::kT;
   constexpr int kR = G::kR;
@@ -123,7 +132,17 @@ __global__ void __launch_bounds__((PassGeom<N, COL>::kThreads))
   __shared__ float2 lds[G::kTile + N];
   __shared__ double red[16];
   float2* tw = lds + G::kTile;
-  load_twiddles<N>(tw);
+
+  int tile, start;
+  if (p.slots > 0) {
+    start = blockIdx.x % p.slots;
+    tile = blockIdx.x / p.slots;
+    if (start >= p.starts) return;  // padding slot: the whole block leaves together
+  } else {
+    const int tiles = gridDim.x / p.starts;
+    start = blockIdx.x / tiles;
+    tile = blockIdx.x % tiles;
+  }
 
   const int tid = threadIdx.x;
   int b, i;
@@ -134,17 +153,16 @@ __global__ void __launch_bounds__((PassGeom<N, COL>::kThreads))
     i = tid % kT;
     b = tid / kT;
   }
-  const int start = blockIdx.y;
   long long base;  // element 0 of this thread's line, relative to its start
   bool valid;
   if constexpr (COL) {
     const int groups = (p.inner + kB - 1) / kB;
-    const int o = blockIdx.x / groups;
-    const int c = (blockIdx.x % groups) * kB + b;
+    const int o = tile / groups;
+    const int c = (tile % groups) * kB + b;
     valid = c < p.inner;
     base = static_cast<long long>(o) * N * p.inner + c;
   } else {
-    const int o = blockIdx.x * kB + b;
+    const int o = tile * kB + b;
     valid = o < p.outer;
     base = static_cast<long long>(o) * N;
   }
@@ -155,13 +173,14 @@ __global__ void __launch_bounds__((PassGeom<N, COL>::kThreads))
   St* w = static_cast<St*>(p.w);
   const St* src = p.src ? static_cast<const St*>(p.src) : w;
   float2 v[kR];
-  __syncthreads();  // twiddles
 
   if constexpr (M == Mode::kFwd || M == Mode::kInv) {
 #pragma unroll
     for (int m = 0; m < kR; ++m) {
       v[m] = valid ? Io::ld(src, soff + base + (i + m * kT) * es) : float2{0.0f, 0.0f};
     }
+    load_twiddles<N>(tw);  // while the loads are in flight
+    __syncthreads();
     fft_line<N, 16, M == Mode::kFwd>(v, i, line, tw);
     if (valid) {
 #pragma unroll
@@ -175,12 +194,17 @@ __global__ void __launch_bounds__((PassGeom<N, COL>::kThreads))
     for (int m = 0; m < kR; ++m) {
       v[m] = valid ? Io::ld(w, soff + base + (i + m * kT) * es) : float2{0.0f, 0.0f};
     }
+    float amp[kR];
+#pragma unroll
+    for (int m = 0; m < kR; ++m) amp[m] = valid ? p.amp[base + (i + m * kT) * es] : -1.0f;
+    load_twiddles<N>(tw);
+    __syncthreads();
     fft_line<N, 16, true>(v, i, line, tw);
     double acc = 0.0;
     if (valid) {
 #pragma unroll
       for (int m = 0; m < kR; ++m) {
-        const float a = p.amp[base + (i + m * kT) * es];
+        const float a = amp[m];
         const float2 u = {v[m].x * p.in_scale, v[m].y * p.in_scale};
         const float2 q = project(u, a, acc);
         v[m] = {q.x * p.out_scale, q.y * p.out_scale};
@@ -193,8 +217,8 @@ __global__ void __launch_bounds__((PassGeom<N, COL>::kThreads))
     }
     const double total = block_sum(acc, red);
     if (tid == 0) {
-      p.partials[static_cast<long long>(start) * p.partial_stride + blockIdx.x] = total;
-      if (blockIdx.x == 0 && start == 0) atomicAdd(p.counter, 1);
+      p.partials[static_cast<long long>(start) * p.partial_stride + tile] = total;
+      if (tile == 0 && start == 0) atomicAdd(p.counter, 1);
     }
   } else {  // kUpdate
     St* x = static_cast<St*>(p.x);
@@ -202,6 +226,17 @@ __global__ void __launch_bounds__((PassGeom<N, COL>::kThreads))
     for (int m = 0; m < kR; ++m) {
       v[m] = valid ? Io::ld(w, soff + base + (i + m * kT) * es) : float2{0.0f, 0.0f};
     }
+    // the old iterate and the support do not depend on the transform: fetch them now
+    float2 xo[kR];
+    bool inside[kR];
+#pragma unroll
+    for (int m = 0; m < kR; ++m) {
+      const long long at = soff + base + (i + m * kT) * es;
+      inside[m] = valid ? p.support[at] != 0 : true;
+      xo[m] = valid ? Io::ld(x, at) : float2{0.0f, 0.0f};
+    }
+    load_twiddles<N>(tw);
+    __syncthreads();
     fft_line<N, 16, false>(v, i, line, tw);
     if (valid) {
 #pragma unroll
@@ -209,10 +244,7 @@ __global__ void __launch_bounds__((PassGeom<N, COL>::kThreads))
         const long long at = soff + base + (i + m * kT) * es;
         const float2 pm = {v[m].x * p.in_scale, v[m].y * p.in_scale};
         float2 nx = pm;
-        if (!p.support[at]) {
-          const float2 xo = Io::ld(x, at);
-          nx = {p.a * xo.x + p.b * pm.x, p.a * xo.y + p.b * pm.y};
-        }
+        if (!inside[m]) nx = {p.a * xo[m].x + p.b * pm.x, p.a * xo[m].y + p.b * pm.y};
         Io::st(x, at, nx);
         v[m] = nx;
       }
@@ -225,7 +257,7 @@ __global__ void __launch_bounds__((PassGeom<N, COL>::kThreads))
                float2{v[m].x * p.out_scale, v[m].y * p.out_scale});
       }
     }
-    if (blockIdx.x == 0 && tid < 64 && p.errs) {
+    if (tile == 0 && tid < 64 && p.errs) {
       // fixed-order sum of the previous modulus pass's partials of this start
       double s = 0.0;
       const double* part = p.partials + static_cast<long long>(start) * p.partial_stride;

@@ -62,7 +62,88 @@ def workload(name):
         return dict(name="c4", shape=(128, 128, 128), starts=8, intensity=inten, mask=mask, recipe=recipe,
                     desc="128^3, 8 starts, repeat(8, RAAR(50, 0.9), Shrinkwrap(rms)) + ER(200), "
                          "noiseless, 1% random-pixel mask, autocorrelation support")
+    if name == "c3":
+        problems = workloads.ensemble_uneven(per_size=16, sizes=(128, 256, 512), seed=0)
+        from qoraphasecu_amd.recipe import default_recipe
+
+        return dict(name="c3", shape=None, starts=8, problems=problems, recipe=default_recipe(),
+                    desc="48 problems (16 each at 128^2, 256^2, 512^2), 8 starts each, default_recipe, "
+                         "Poisson 1e6-1e8 photons, 4-10 px beamstops, autocorrelation supports")
     raise ValueError(name)
+
+
+def time_ensemble(w, variant, steps, warmup):
+    """All problems in one MultiSolver: every shape advances in the same graph launch."""
+    from qoraphasecu_amd import _core
+
+    ops = w["recipe"].ops()
+    iters = w["recipe"].iterations()
+    inten = [p[1] for p in w["problems"]]
+    masks = [p[2] for p in w["problems"]]
+    t0 = time.perf_counter()
+    s = _core.MultiSolver(inten, None, masks, w["starts"], 0, 0, 0.1, "hip", variant, 10)
+    build = time.perf_counter() - t0
+    for _ in range(max(1, warmup)):
+        s.run(ops)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.run(ops)
+    wall = time.perf_counter() - t0
+    runs = w["starts"] * len(inten)
+    rate = runs * iters * steps / wall
+    best = [min(float(r[2][-1]) for r in per) for per in s.results()]
+    log(f"  c3 ensemble {variant:>8}: {wall / steps * 1e3:9.2f} ms/run  {rate:12.4g} start-it/s  "
+        f"(construction + tuning {build:.2f} s; median best error {np.median(best):.3e})")
+    return dict(ms_per_run=wall / steps * 1e3, start_iterations_per_s=rate, construction_s=build,
+                best_final_errors=best)
+
+
+def time_per_problem(w, variant, steps):
+    """The problems one after another, each batched over its own starts."""
+    from qoraphasecu_amd import _core
+
+    ops = w["recipe"].ops()
+    iters = w["recipe"].iterations()
+    solvers = [_core.Solver(p[1], None, p[2], w["starts"], 0, 0, None, 0.1, "hip", variant, 10)
+               for p in w["problems"]]
+    for s in solvers:
+        s.run(ops)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for s in solvers:
+            s.run(ops)
+    wall = time.perf_counter() - t0
+    rate = w["starts"] * len(solvers) * iters * steps / wall
+    log(f"  c3 per-problem {variant:>8}: {wall / steps * 1e3:9.2f} ms/run  {rate:12.4g} start-it/s")
+    return dict(ms_per_run=wall / steps * 1e3, start_iterations_per_s=rate)
+
+
+def time_refinement(steps=500):
+    """Config 5: Adam (lr 1e-2) on the support-projected iterate of 16 objects of 256^2,
+    started from 50 RAAR iterations; the amplitude loss and its adjoint on the GPU."""
+    import torch
+
+    import qoraphasecu_amd as qpc
+    from qoraphasecu_amd import autograd, workloads
+
+    x0, sup, inten, masks = [], [], [], []
+    for seed in range(16):
+        obj, i, m = workloads.problem_2d(256, seed=seed)
+        r = qpc.reconstruct(i, mask=m.astype(bool), recipe=qpc.Recipe(qpc.RAAR(50)), backend="hip", variant="batched")
+        x0.append(r.object)
+        sup.append(r.support)
+        inten.append(i)
+        masks.append(m.astype(bool))
+    x0 = np.stack(x0)
+    autograd.refine(x0, np.stack(inten), np.stack(sup), mask=np.stack(masks), steps=5)  # warm up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out, hist = autograd.refine(x0, np.stack(inten), np.stack(sup), mask=np.stack(masks), steps=steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    h = hist.cpu().numpy()
+    log(f"  c5 adam: {wall / steps * 1e3:.3f} ms/step (16 x 256^2), loss {h[0].sum():.4g} -> {h[-1].sum():.4g}")
+    return dict(ms_per_step=wall / steps * 1e3, steps=steps, loss_first=float(h[0].sum()), loss_last=float(h[-1].sum()))
 
 
 def time_variant(w, variant, batch, steps, warmup):
@@ -151,7 +232,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--configs", default="c2,c4")
+    ap.add_argument("--configs", default="c2,c4,c3,c5")
     ap.add_argument("--variants", default="graph,mixed,mixed_bf16")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-passes", action="store_true", help="skip the per-pass timing")
@@ -182,7 +263,23 @@ def main():
     headline = None
     roof = None
     for name in [c for c in args.configs.split(",") if c]:
+        if name == "c5":
+            details["c5"] = time_refinement()
+            continue
         w = workload(name)
+        if name == "c3":
+            res = {"workload": w["desc"], "iterations": w["recipe"].iterations(), "starts": w["starts"],
+                   "problems": len(w["problems"])}
+            res["ensemble_mixed"] = time_ensemble(w, "mixed", args.steps, args.warmup)
+            res["ensemble_batched"] = time_ensemble(w, "batched", args.steps, args.warmup)
+            res["per_problem_batched"] = time_per_problem(w, "batched", args.steps)
+            res["speedup_ensemble_mixed_vs_per_problem_fp32"] = (
+                res["ensemble_mixed"]["start_iterations_per_s"] / res["per_problem_batched"]["start_iterations_per_s"])
+            from qoraphasecu_amd import _qpch
+
+            res["autotune_log"] = _qpch.autotune_log()
+            details["c3"] = res
+            continue
         log(f"{name}: {w['desc']}")
         res = {"workload": w["desc"], "iterations": w["recipe"].iterations(), "starts": w["starts"]}
         for variant in [v for v in args.variants.split(",") if v]:

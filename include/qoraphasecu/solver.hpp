@@ -80,8 +80,21 @@ std::vector<Reconstruction> reconstruct(const Problem& problem,
                                         const Recipe& recipe = default_recipe(),
                                         const Options& options = {});
 
-// Several independent problems (any mix of shapes) with the same recipe and options:
-// result[p] holds problem p's starts.
+// The state of several independent problems (any mix of shapes), each with
+// options.starts starts; run() continues like Solver::run.  results()[p] holds problem p's
+// starts.  On Backend::hip with a batched or reduced-precision variant and power-of-two
+// extents, problems of one shape share one device solver and every shape advances in the
+// same graph launch; otherwise the problems run one after another.
+class MultiSolver {
+ public:
+  virtual ~MultiSolver() = default;
+  virtual void run(const Recipe& recipe) = 0;
+  virtual std::vector<std::vector<Reconstruction>> results() const = 0;
+};
+
+std::unique_ptr<MultiSolver> make_multi_solver(std::span<const Problem> problems, const Options& options = {});
+
+// make_multi_solver + run + results.
 std::vector<std::vector<Reconstruction>> reconstruct(std::span<const Problem> problems,
                                                      const Recipe& recipe = default_recipe(),
                                                      const Options& options = {});

@@ -57,17 +57,29 @@ typedef struct qpch_solver qpch_solver;
 typedef struct {
   int ndim;          /* 2 or 3 */
   size_t shape[3];   /* slowest first */
-  const float* amplitude;  /* n, FFT order: sqrt of the clipped intensity */
-  const uint8_t* mask;     /* n, FFT order, 1 = unusable pixel */
-  const uint8_t* support;  /* n, FFT order, the initial support of every start */
-  int starts;
-  int batch;           /* starts per launch group, must divide starts (0 = all) */
-  uint64_t seed;       /* random starting phases */
-  const float* start;  /* starts * n complex, FFT order, or NULL for random starts */
+  const float* amplitude;  /* problems * n, FFT order: sqrt of the clipped intensity */
+  const uint8_t* mask;     /* problems * n, FFT order, 1 = unusable pixel */
+  const uint8_t* support;  /* problems * n, FFT order, the initial support of each problem's starts */
+  int starts;          /* per problem */
+  int batch;           /* starts per launch group, must divide problems * starts (0 = all) */
+  uint64_t seed;       /* random starting phases: start k of a problem draws (seed, k) */
+  const float* start;  /* problems * starts * n complex, FFT order, or NULL for random starts */
   int storage;         /* QPCH_F32 / QPCH_F16 / QPCH_BF16 */
   int mixed;           /* with F16/BF16: run refine (ER) iterations in F32 */
   int schedule;        /* QPCH_UNFUSED / QPCH_FUSED / QPCH_GRAPH */
+  int problems;        /* independent problems of this shape stacked in one solver (0 = 1) */
+  int plane;           /* 3D: 0 = automatic, 1 = xy-plane update pass, 2 = line passes only */
+  int xcd;             /* 0 = XCD-affine block numbering when starts per launch are a multiple of 8,
+                          2 = plain numbering */
 } qpch_problem;
+
+/* Launch configuration chosen by qpch_autotune. */
+typedef struct {
+  int batch;  /* starts per launch */
+  int plane;  /* as qpch_problem.plane */
+  int xcd;    /* as qpch_problem.xcd */
+  double us_per_iteration;
+} qpch_tuning;
 
 typedef struct {
   char name[256];
@@ -109,8 +121,10 @@ int qpch_solver_restore_support(qpch_solver* solver);
 int qpch_solver_read_errors(qpch_solver* solver, int first, int count, double* host);
 /* objects in FFT order, rescaled to the data (starts * n complex) */
 int qpch_solver_read_state(qpch_solver* solver, float* host);
-/* usable energy sum A^2 in the solver's scale: the error denominator */
+/* usable energy sum A^2 in the solver's scale: the error denominator (problem 0) */
 double qpch_solver_energy(const qpch_solver* solver);
+/* the same for every problem of the solver (problems doubles) */
+int qpch_solver_energies(const qpch_solver* solver, double* out);
 int qpch_solver_synchronize(qpch_solver* solver);
 
 /* Measurement: the average duration (ms) of each pass of one iteration of the first
@@ -121,6 +135,29 @@ int qpch_solver_synchronize(qpch_solver* solver);
 int qpch_solver_time_passes(qpch_solver* solver, int low, int reps, float* ms, int cap, int* count);
 /* the passes of one iteration: axis and mode (0 fwd, 1 inv, 2 modulus, 3 update) */
 int qpch_solver_pass_info(qpch_solver* solver, int low, int* axes, int* modes, int cap, int* count);
+
+/* --- autotuner --------------------------------------------------------------------------
+ * Times candidate launch configurations of `problem` (starts per launch, 3D plane or line
+ * passes, XCD-affine or plain block numbering) on trial solvers with random starts, a few
+ * graph-replayed iterations each, and returns the fastest.  The configurations give the
+ * same results; only the time differs.  Cached per process by shape, total starts and
+ * storage; qpch_autotune_log() returns one line per tuned key (also printed to stderr
+ * when QPCH_TUNE_LOG=1).  No reference counterpart: the reference leaves Options::batch
+ * to the user (include/qoraphasecu/solver.hpp:44-46).  A problem->batch > 0 is kept and
+ * only the other choices are tuned. */
+int qpch_autotune(const qpch_problem* problem, qpch_tuning* out);
+const char* qpch_autotune_log(void);
+
+/* --- groups: several solvers (e.g. one per problem shape) advanced by one graph launch --
+ * qpch_group_iterate captures, per chunk length and coefficients, one hipGraph in which
+ * every solver's passes form an independent branch, and replays it: the solvers'
+ * iterations run concurrently from a single launch.  Each solver must have had
+ * qpch_solver_begin_run; Shrinkwrap, supports and read-back stay per solver. */
+typedef struct qpch_group qpch_group;
+int qpch_group_create(qpch_solver* const* solvers, int count, qpch_group** out);
+int qpch_group_iterate(qpch_group* group, int iterations, float a, float b, int refine);
+int qpch_group_synchronize(qpch_group* group);
+void qpch_group_destroy(qpch_group* group);
 
 /* --- operators on host arrays (synchronous) --------------------------------------------- */
 
@@ -137,13 +174,18 @@ int qpch_gaussian_blur(const float* host_in, float* host_out, int ndim, const si
 
 /* --- device-pointer entry points (caller's stream) --------------------------------------- */
 
-/* Amplitude loss and its gradient for `batch` objects x (device, FFT order complex):
+/* Amplitude loss and its gradient for `batch` objects x (device pointers, complex):
  *   loss[b] = sum over usable pixels (|F x_b| - A)^2   (F unitary, FP64 accumulation)
- *   grad_b  = 2 (x_b - P_M x_b)                         (the Wirtinger gradient, d/dRe + i d/dIm)
- * precision 0: complex64 x/grad, float amplitude; 1: complex128 x/grad, double amplitude.
- * amplitude is signed: a negative value marks a masked pixel.  stream: hipStream_t or NULL. */
-int qpch_amplitude_loss(const void* x, const void* amplitude, void* grad, double* loss,
-                        int ndim, const size_t* shape, int batch, int precision, void* stream);
+ *   grad_b  = 2 (x_b - P_M x_b) = dL/dRe x + i dL/dIm x  (the gradient torch expects)
+ * The amplitude A is in FFT order (frequency origin at index 0); the real-space origin of
+ * x does not matter (a circular shift changes only spectral phases).  A negative amplitude
+ * marks a masked pixel.  amplitude_stride: 0 = one amplitude for every object, n = one
+ * amplitude array per object (back to back).  precision 0: complex64 x/grad, float A (fused passes for
+ * power-of-two extents); 1: complex128 x/grad, double A (axes up to 1024).  loss is a
+ * device array of `batch` doubles.  Work is queued on `stream` (a hipStream_t, NULL for the
+ * null stream) with stream-ordered scratch memory. */
+int qpch_amplitude_loss(const void* x, const void* amplitude, long long amplitude_stride, void* grad,
+                        double* loss, int ndim, const size_t* shape, int batch, int precision, void* stream);
 
 #ifdef __cplusplus
 }

@@ -17,6 +17,7 @@ from .api import (
     hip_available,
     hip_device_info,
     reconstruct,
+    reconstruct_many,
     simulate_intensity,
 )
 from .recipe import ER, HIO, RAAR, Recipe, Shrinkwrap, default_recipe, detwin, repeat
@@ -37,6 +38,7 @@ __all__ = [
     "hip_available",
     "hip_device_info",
     "reconstruct",
+    "reconstruct_many",
     "repeat",
     "simulate_intensity",
 ]

@@ -37,6 +37,18 @@ class QpchProblem(ctypes.Structure):
         ("storage", ctypes.c_int),
         ("mixed", ctypes.c_int),
         ("schedule", ctypes.c_int),
+        ("problems", ctypes.c_int),
+        ("plane", ctypes.c_int),
+        ("xcd", ctypes.c_int),
+    ]
+
+
+class QpchTuning(ctypes.Structure):
+    _fields_ = [
+        ("batch", ctypes.c_int),
+        ("plane", ctypes.c_int),
+        ("xcd", ctypes.c_int),
+        ("us_per_iteration", ctypes.c_double),
     ]
 
 
@@ -84,13 +96,21 @@ def lib():
         L.qpch_solver_energy.argtypes = [vp]
         L.qpch_solver_energy.restype = d
         L.qpch_solver_synchronize.argtypes = [vp]
+        L.qpch_solver_energies.argtypes = [vp, vp]
+        L.qpch_autotune.argtypes = [ctypes.POINTER(QpchProblem), ctypes.POINTER(QpchTuning)]
+        L.qpch_autotune_log.restype = ctypes.c_char_p
+        L.qpch_group_create.argtypes = [ctypes.POINTER(vp), i, ctypes.POINTER(vp)]
+        L.qpch_group_iterate.argtypes = [vp, i, f, f, i]
+        L.qpch_group_synchronize.argtypes = [vp]
+        L.qpch_group_destroy.argtypes = [vp]
+        L.qpch_group_destroy.restype = None
         L.qpch_solver_time_passes.argtypes = [vp, i, i, ctypes.POINTER(f), i, ctypes.POINTER(i)]
         L.qpch_solver_pass_info.argtypes = [vp, i, ctypes.POINTER(i), ctypes.POINTER(i), i, ctypes.POINTER(i)]
         L.qpch_fft.argtypes = [vp, i, szp, i]
         L.qpch_modulus_project.argtypes = [vp, vp, vp, i, szp, ctypes.POINTER(d)]
         L.qpch_support_project.argtypes = [vp, vp, sz]
         L.qpch_gaussian_blur.argtypes = [vp, vp, i, szp, f]
-        L.qpch_amplitude_loss.argtypes = [vp, vp, vp, vp, i, szp, i, i, vp]
+        L.qpch_amplitude_loss.argtypes = [vp, vp, ctypes.c_longlong, vp, vp, i, szp, i, i, vp]
         _lib = L
     return _lib
 
@@ -172,6 +192,14 @@ def gaussian_blur(image, sigma):
     return out
 
 
+def amplitude_loss(x_ptr, amp_ptr, amp_stride, grad_ptr, loss_ptr, shape, batch, precision, stream):
+    """Device-pointer entry: see qpch_amplitude_loss in include/qpch.h."""
+    nd, sh = _shape(shape)
+    check(lib().qpch_amplitude_loss(ctypes.c_void_p(x_ptr), ctypes.c_void_p(amp_ptr), amp_stride,
+                                    ctypes.c_void_p(grad_ptr), ctypes.c_void_p(loss_ptr), nd, sh, batch,
+                                    precision, ctypes.c_void_p(stream)))
+
+
 # --- solver ------------------------------------------------------------------------------
 
 
@@ -179,10 +207,14 @@ class Solver:
     """A device solver over FFT-order inputs (what the C++ HipSolver drives)."""
 
     def __init__(self, amplitude, mask, support, starts=1, batch=0, seed=0, start=None,
-                 storage="f32", mixed=False, schedule="graph"):
-        self.shape = tuple(amplitude.shape)
+                 storage="f32", mixed=False, schedule="graph", problems=1, plane=0, xcd=0):
+        """amplitude / mask / support: one problem's FFT-order arrays, or `problems` of them
+        stacked along a leading axis; `starts` per problem."""
+        shape = tuple(amplitude.shape)
+        self.problems = problems
+        self.shape = shape[1:] if problems > 1 else shape
         self.n = int(np.prod(self.shape))
-        self.starts = starts
+        self.starts = starts * problems
         self._keep = [
             np.ascontiguousarray(amplitude, dtype=np.float32),
             np.ascontiguousarray(mask, dtype=np.uint8),
@@ -193,6 +225,9 @@ class Solver:
         p.ndim = len(self.shape)
         for k, s in enumerate(self.shape):
             p.shape[k] = s
+        p.problems = problems
+        p.plane = plane
+        p.xcd = xcd
         p.amplitude = _ptr(self._keep[0])
         p.mask = _ptr(self._keep[1])
         p.support = _ptr(self._keep[2])
@@ -207,6 +242,9 @@ class Solver:
         check(lib().qpch_solver_create(ctypes.byref(p), ctypes.byref(h)))
         self._h = h
         self.energy = lib().qpch_solver_energy(self._h)
+        e = np.empty(problems, np.float64)
+        check(lib().qpch_solver_energies(self._h, _ptr(e)))
+        self.energies = e
 
     def begin_run(self, capacity):
         check(lib().qpch_solver_begin_run(self._h, capacity))
@@ -263,3 +301,32 @@ class Solver:
 
     def __del__(self):
         self.close()
+
+
+class Group:
+    """Solvers advanced together, one graph launch per chunk (qpch_group_*)."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        arr = (ctypes.c_void_p * len(self.solvers))(*[s._h.value for s in self.solvers])
+        h = ctypes.c_void_p()
+        check(lib().qpch_group_create(arr, len(self.solvers), ctypes.byref(h)))
+        self._h = h
+
+    def iterate(self, n, a, b, refine=False):
+        check(lib().qpch_group_iterate(self._h, n, a, b, int(refine)))
+
+    def synchronize(self):
+        check(lib().qpch_group_synchronize(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().qpch_group_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def autotune_log() -> str:
+    return lib().qpch_autotune_log().decode()

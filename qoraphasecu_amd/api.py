@@ -180,3 +180,50 @@ def reconstruct(
     )
     results = tuple(Reconstruction(obj, sup.astype(bool), err) for obj, sup, err in runs)
     return results[0] if starts == 1 else Ensemble(results)
+
+
+def reconstruct_many(
+    intensities,
+    *,
+    supports=None,
+    masks=None,
+    recipe=None,
+    starts: int = 1,
+    batch: int | None = None,
+    seed: int | None = None,
+    support_threshold: float = 0.1,
+    backend: Literal["auto", "hip"] = "auto",
+    variant: str = "batched",
+    sync_every: int = 10,
+) -> list:
+    """Several independent problems (any mix of 2D/3D shapes) with one recipe.
+
+    On the batched and reduced-precision variants, problems of one shape share one
+    stacked device solver and all shapes advance together, one graph launch per chunk of
+    iterations.  Returns one :class:`Reconstruction` or :class:`Ensemble` per problem."""
+    intensities = [np.ascontiguousarray(i, dtype=np.float32) for i in intensities]
+    if not intensities:
+        raise ValueError("no problems")
+    for i in intensities:
+        if i.ndim not in (2, 3):
+            raise ValueError(f"intensity must be 2D or 3D, got ndim={i.ndim}")
+    if backend == "auto":
+        backend = "hip" if hip_available() else "cpu"
+    if recipe is None:
+        recipe = default_recipe()
+
+    def flags(name, arrays):
+        if arrays is None:
+            return None
+        if len(arrays) != len(intensities):
+            raise ValueError(f"{name} needs one entry per intensity")
+        return [None if a is None else _flag(name, a, i.shape) for a, i in zip(arrays, intensities)]
+
+    runs = _core.reconstruct_many(intensities, flags("support", supports), flags("mask", masks), recipe.ops(),
+                                  starts, 0 if batch is None else batch, 0 if seed is None else seed,
+                                  support_threshold, backend, variant, sync_every)
+    out = []
+    for per in runs:
+        results = tuple(Reconstruction(obj, sup.astype(bool), err) for obj, sup, err in per)
+        out.append(results[0] if starts == 1 else Ensemble(results))
+    return out

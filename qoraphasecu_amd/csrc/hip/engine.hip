@@ -10,8 +10,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -22,6 +24,7 @@
 #include "generic.hpp"
 #include "misc.hpp"
 #include "passes.hpp"
+#include "plane.hpp"
 
 namespace qpch {
 
@@ -137,6 +140,11 @@ Dims make_dims(int ndim, const size_t* shape) {
 
 constexpr int kMaxFused = 4096;
 
+bool eager_launches() {
+  const char* v = std::getenv("QPCH_EAGER");
+  return v && v[0] == '1';
+}
+
 bool fusable(const Dims& d) {
   for (int a = 0; a < d.nd; ++a) {
     if (!is_pow2(d.d[a]) || d.d[a] < 2 || d.d[a] > kMaxFused) return false;
@@ -168,7 +176,7 @@ void launch_pass_t(PassArgs a, int starts, hipStream_t s) {
   using G = dev::PassGeom<N, COL, sizeof(St)>;
   const int tiles = pass_tiles_t<N, COL, sizeof(St)>(a.outer, a.inner);
   a.starts = starts;
-  a.slots = xcd_slots(starts);
+  a.slots = a.no_xcd ? 0 : xcd_slots(starts);
   const int gx = tiles * (a.slots > 0 ? a.slots : starts);
   hipLaunchKernelGGL((dev::line_pass<N, COL, M, St>), dim3(gx), dim3(G::kThreads), 0, s, a);
   QPCH_HIP(hipGetLastError());
@@ -182,6 +190,7 @@ void launch_pass_n(bool col, Mode m, const PassArgs& a, int starts, hipStream_t 
       case Mode::kInv: return launch_pass_t<N, true, Mode::kInv, St>(a, starts, s);
       case Mode::kModulus: return launch_pass_t<N, true, Mode::kModulus, St>(a, starts, s);
       case Mode::kUpdate: return launch_pass_t<N, true, Mode::kUpdate, St>(a, starts, s);
+      case Mode::kGrad: return launch_pass_t<N, true, Mode::kGrad, St>(a, starts, s);
     }
   } else {
     switch (m) {
@@ -189,6 +198,7 @@ void launch_pass_n(bool col, Mode m, const PassArgs& a, int starts, hipStream_t 
       case Mode::kInv: return launch_pass_t<N, false, Mode::kInv, St>(a, starts, s);
       case Mode::kModulus: return launch_pass_t<N, false, Mode::kModulus, St>(a, starts, s);
       case Mode::kUpdate: return launch_pass_t<N, false, Mode::kUpdate, St>(a, starts, s);
+      case Mode::kGrad: return launch_pass_t<N, false, Mode::kGrad, St>(a, starts, s);
     }
   }
 }
@@ -220,6 +230,39 @@ int pass_tiles(int N, bool col, int outer, int inner) {
     default:
       throw InvalidArgument("no fused pass for axis length " + std::to_string(N));
   }
+}
+
+// --- 3D plane pass: (ny, nx) planes of 1024 to 16384 elements, both axes >= 16 ---------
+
+#define QPCH_FOR_EACH_PLANE(X) \
+  X(16, 64) X(16, 128) X(32, 32) X(32, 64) X(32, 128) X(64, 16) X(64, 32) X(64, 64) X(64, 128) \
+  X(128, 16) X(128, 32) X(128, 64) X(128, 128)
+
+bool plane_supported(int ny, int nx) {
+#define QPCH_CASE(A, B) \
+  if (ny == A && nx == B) return true;
+  QPCH_FOR_EACH_PLANE(QPCH_CASE)
+#undef QPCH_CASE
+  return false;
+}
+
+template <int NY, int NX, class St>
+void launch_plane_t(PassArgs a, int starts, hipStream_t s) {
+  using G = dev::PlaneGeom<NY, NX>;
+  a.starts = starts;
+  a.slots = a.no_xcd ? 0 : xcd_slots(starts);
+  const int gx = a.outer * (a.slots > 0 ? a.slots : starts);
+  hipLaunchKernelGGL((dev::plane_update<NY, NX, St>), dim3(gx), dim3(G::kThreads), 0, s, a);
+  QPCH_HIP(hipGetLastError());
+}
+
+template <class St>
+void launch_plane(int ny, int nx, const PassArgs& a, int starts, hipStream_t s) {
+#define QPCH_CASE(A, B)                                  \
+  if (ny == A && nx == B) return launch_plane_t<A, B, St>(a, starts, s);
+  QPCH_FOR_EACH_PLANE(QPCH_CASE)
+#undef QPCH_CASE
+  throw InvalidArgument("no plane pass for this plane shape");
 }
 
 // ---------------------------------------------------------------------------------------
@@ -257,10 +300,26 @@ void launch_generic(float2* data, long long n_per_start, int batch, int N, long 
   const dev::GenericPlan plan = generic_plan(N);
   const long long lines = outer * inner;
   const size_t lds = sizeof(float2) * 3 * static_cast<size_t>(N);
-  hipLaunchKernelGGL(dev::generic_fft, dim3(static_cast<unsigned>(lines), batch),
+  hipLaunchKernelGGL(dev::generic_fft<float>, dim3(static_cast<unsigned>(lines), batch),
                      dim3(dev::kGenericThreads), lds, s, data, n_per_start,
                      static_cast<int>(inner), plan, forward ? 1 : 0, scale);
   QPCH_HIP(hipGetLastError());
+}
+
+// Unnormalised FP64 transform along every axis (generic kernel, lines up to 1024).
+void device_fft_f64(double2* data, const Dims& d, int batch, bool forward, hipStream_t s) {
+  for (int a = d.nd - 1; a >= 0; --a) {
+    const int N = d.d[a];
+    if (N == 1) continue;
+    if (N > 1024) throw InvalidArgument("the FP64 transform supports axes up to 1024");
+    const dev::GenericPlan plan = generic_plan(N);
+    const long long lines = d.outer(a) * d.inner(a);
+    const size_t lds = sizeof(double2) * 3 * static_cast<size_t>(N);
+    hipLaunchKernelGGL(dev::generic_fft<double>, dim3(static_cast<unsigned>(lines), batch),
+                       dim3(dev::kGenericThreads), lds, s, data, d.n, static_cast<int>(d.inner(a)), plan,
+                       forward ? 1 : 0, 1.0);
+    QPCH_HIP(hipGetLastError());
+  }
 }
 
 void scale_inplace(float2* data, long long total, float scale, hipStream_t s) {
@@ -355,8 +414,11 @@ struct SolverImpl {
   int schedule = QPCH_GRAPH;
   bool fused = true;
   hipStream_t stream = nullptr;
-  float amp_scale = 1.0f;
-  double energy = 0.0;
+  float amp_scale = 1.0f;  // problem 0's
+  double energy = 0.0;     // problem 0's
+  int problems = 1, per_problem = 1;
+  std::vector<float> amp_scales;
+  std::vector<double> energies;
 
   DevBuf<float> amp_signed, amp_true;
   DevBuf<std::uint8_t> supp, saved;
@@ -368,6 +430,7 @@ struct SolverImpl {
   int err_cap = 0;
   bool low_active = false;
   bool wave_valid = false;
+  int generation = 0;  // bumped when buffers that graphs capture are reallocated
   std::map<std::tuple<int, float, float>, std::pair<int, DevBuf<float>>> taps_cache;  // (N, sigma) -> (J, taps)
   std::map<std::tuple<int, int, float, float, bool>, hipGraphExec_t> graphs;
 
@@ -400,7 +463,10 @@ struct SolverImpl {
     Mode mode;
     bool from_x;  // prologue: read x instead of the wave
     float in_scale, out_scale;
+    bool plane = false;  // 3D: the update on whole xy-planes (axes 1 and 2 at once)
   };
+  bool plane3d = false;  // 3D iterations use the plane pass
+  bool xcd_off = false;  // keep the plain block numbering even for multiples of 8 starts
 
   std::vector<PassSpec> iteration_passes(bool lowp) const {
     const int nd = dims.nd;
@@ -409,6 +475,11 @@ struct SolverImpl {
     auto ax = [&](int a) { return lowp ? static_cast<float>(1.0 / std::sqrt(static_cast<double>(dims.d[a]))) : 1.0f; };
     const float ms = lowp ? ax(0) : s;
     v.push_back({0, Mode::kModulus, false, ms, ms});
+    if (nd == 3 && plane3d) {
+      const float ps = lowp ? ax(1) * ax(2) : 1.0f;
+      v.push_back({2, Mode::kUpdate, false, ps, ps, true});
+      return v;
+    }
     if (nd == 3) v.push_back({1, Mode::kInv, false, 1.0f, ax(1)});
     v.push_back({nd - 1, Mode::kUpdate, false, ax(nd - 1), ax(nd - 1)});
     if (nd == 3) v.push_back({1, Mode::kFwd, false, 1.0f, ax(1)});
@@ -437,6 +508,8 @@ struct SolverImpl {
     p.x = xp<St>(first);
     p.support = supp.get() + first * n;
     p.amp = amp_signed.get();
+    p.amp_stride = problems > 1 ? n : 0;
+    p.amp_group = per_problem;
     p.partials = partials.get();
     p.partial_stride = pass_tiles<sizeof(St)>(dims.d[0], dims.inner(0) > 1, static_cast<int>(dims.outer(0)),
                                               static_cast<int>(dims.inner(0)));
@@ -447,8 +520,14 @@ struct SolverImpl {
     p.counter = counters.get() + g;
     p.a = a;
     p.b = b;
+    p.no_xcd = xcd_off ? 1 : 0;
     p.in_scale = ps.in_scale;
     p.out_scale = ps.out_scale;
+    if (ps.plane) {
+      p.outer = dims.d[0];  // one tile per plane
+      launch_plane<St>(dims.d[1], dims.d[2], p, count, stream);
+      return;
+    }
     launch_pass<St>(dims.d[ps.axis], inner > 1, ps.mode, p, count, stream);
   }
 
@@ -552,6 +631,32 @@ struct SolverImpl {
     QPCH_HIP(hipGraphLaunch(it->second, stream));
   }
 
+  // --- group support: the precision switch and prologue run eagerly; the chunk's passes
+  // for every launch group are then enqueued on `stream` (inside a capture by the group).
+  bool prepare(bool refine) {
+    if (!fused) throw InvalidArgument("grouped solvers need the fused schedule");
+    if (err_cap <= 0) throw InvalidArgument("qpch_solver_begin_run must precede iterate");
+    const bool lowp = low() && !(mixed && refine);
+    set_precision(lowp);
+    if (!wave_valid) {
+      for (int g = 0; g < groups; ++g) {
+        if (!lowp) fused_prologue<float2>(g);
+        else if (storage == QPCH_F16) fused_prologue<__half2>(g);
+        else fused_prologue<__hip_bfloat162>(g);
+      }
+    }
+    return lowp;
+  }
+
+  void enqueue(int len, float a, float b, bool lowp) {
+    for (int g = 0; g < groups; ++g) {
+      if (!lowp) fused_iterations<float2>(g, len, a, b);
+      else if (storage == QPCH_F16) fused_iterations<__half2>(g, len, a, b);
+      else fused_iterations<__hip_bfloat162>(g, len, a, b);
+    }
+    wave_valid = true;
+  }
+
   void iterate(int len, float a, float b, bool refine) {
     if (len < 1) throw InvalidArgument("iterations must be >= 1");
     if (err_cap <= 0) throw InvalidArgument("qpch_solver_begin_run must precede iterate");
@@ -580,6 +685,7 @@ struct SolverImpl {
     if (errs.size() < need) {
       clear_graphs();  // captured graphs hold the history's address
       errs.resize(need);
+      ++generation;
     }
     err_cap = capacity;
     hipLaunchKernelGGL(dev::fill_double, dim3(grid_for(static_cast<long long>(errs.size()))), dim3(256), 0,
@@ -646,8 +752,10 @@ struct SolverImpl {
     dims = make_dims(p.ndim, p.shape);
     n = dims.n;
     if (p.starts < 1) throw InvalidArgument("starts must be >= 1");
-    starts = p.starts;
-    batch = p.batch ? p.batch : p.starts;
+    problems = p.problems > 1 ? p.problems : 1;
+    per_problem = p.starts;
+    starts = per_problem * problems;  // every problem's starts, problem-major
+    batch = p.batch ? p.batch : starts;
     if (batch < 1 || starts % batch != 0) throw InvalidArgument("batch must divide starts");
     groups = starts / batch;
     storage = p.storage;
@@ -655,11 +763,17 @@ struct SolverImpl {
     mixed = p.mixed != 0 && storage != QPCH_F32;
     schedule = p.schedule;
     if (schedule < QPCH_UNFUSED || schedule > QPCH_GRAPH) throw InvalidArgument("unknown schedule");
+    // QPCH_EAGER=1 launches the graph schedule's kernels one by one (for kernel tracers
+    // that do not follow graph launches); the kernels and results are unchanged.
+    if (schedule == QPCH_GRAPH && eager_launches()) schedule = QPCH_FUSED;
     fused = fusable(dims) && schedule != QPCH_UNFUSED;
+    plane3d = fused && dims.nd == 3 && plane_supported(dims.d[1], dims.d[2]) && p.plane != 2;
+    xcd_off = p.xcd == 2;
     if (storage != QPCH_F32 && !fusable(dims)) {
       throw InvalidArgument("fp16/bf16 storage needs power-of-two extents between 2 and 4096");
     }
     if (storage != QPCH_F32 && schedule == QPCH_UNFUSED) throw InvalidArgument("fp16/bf16 storage needs a fused schedule");
+    if (problems > 1 && !fused) throw InvalidArgument("several problems in one solver need power-of-two extents");
     for (int a = 0; a < dims.nd; ++a) {
       if (!(is_pow2(dims.d[a]) && dims.d[a] <= kMaxFused) && dims.d[a] > dev::kGenericMaxN) {
         throw InvalidArgument("axis length " + std::to_string(dims.d[a]) + " is not supported");
@@ -668,25 +782,35 @@ struct SolverImpl {
     if (!p.amplitude || !p.mask || !p.support) throw InvalidArgument("amplitude, mask and support are required");
     QPCH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
 
-    std::vector<float> amp(p.amplitude, p.amplitude + n);
-    if (storage != QPCH_F32) {
-      // keep the unnormalised spectra inside fp16 range: peak amplitude 1
-      const float peak = *std::max_element(amp.begin(), amp.end());
-      amp_scale = peak > 0.0f ? peak : 1.0f;
-      for (float& v : amp) v /= amp_scale;
+    const long long np = n * problems;
+    std::vector<float> amp(p.amplitude, p.amplitude + np);
+    amp_scales.assign(problems, 1.0f);
+    energies.assign(problems, 0.0);
+    std::vector<float> signed_amp(np);
+    for (int q = 0; q < problems; ++q) {
+      float* a = amp.data() + q * n;
+      if (storage != QPCH_F32) {
+        // keep the unnormalised spectra inside fp16 range: peak amplitude 1 per problem
+        const float peak = *std::max_element(a, a + n);
+        amp_scales[q] = peak > 0.0f ? peak : 1.0f;
+        for (long long i = 0; i < n; ++i) a[i] /= amp_scales[q];
+      }
+      double e = 0.0;
+      for (long long i = 0; i < n; ++i) {
+        const bool masked = p.mask[q * n + i] != 0;
+        signed_amp[q * n + i] = masked ? -1.0f : a[i];
+        if (!masked) e += static_cast<double>(a[i]) * a[i];
+      }
+      energies[q] = e;
     }
-    std::vector<float> signed_amp(n);
-    double e = 0.0;
-    for (long long i = 0; i < n; ++i) {
-      signed_amp[i] = p.mask[i] ? -1.0f : amp[i];
-      if (!p.mask[i]) e += static_cast<double>(amp[i]) * amp[i];
-    }
-    energy = e;
-    amp_true = upload(amp.data(), n);
-    amp_signed = upload(signed_amp.data(), n);
+    energy = energies[0];
+    amp_scale = amp_scales[0];
+    amp_true = upload(amp.data(), np);
+    amp_signed = upload(signed_amp.data(), np);
     std::vector<std::uint8_t> sup(static_cast<size_t>(n) * starts);
     for (int s = 0; s < starts; ++s) {
-      for (long long i = 0; i < n; ++i) sup[s * n + i] = p.support[i] != 0;
+      const std::uint8_t* src = p.support + (s / per_problem) * n;
+      for (long long i = 0; i < n; ++i) sup[s * n + i] = src[i] != 0;
     }
     supp = upload(sup.data(), sup.size());
     saved.resize(sup.size());
@@ -708,13 +832,16 @@ struct SolverImpl {
     // starting objects
     if (p.start) {
       std::vector<float> host(p.start, p.start + 2 * n * starts);
-      if (amp_scale != 1.0f) for (float& v : host) v /= amp_scale;
+      for (int s = 0; s < starts; ++s) {
+        const float sc = amp_scales[s / per_problem];
+        if (sc != 1.0f) for (long long i = 0; i < 2 * n; ++i) host[2 * n * s + i] /= sc;
+      }
       QPCH_HIP(hipMemcpy(w32.get(), host.data(), sizeof(float2) * n * starts, hipMemcpyHostToDevice));
     } else {
       for (int g = 0; g < groups; ++g) {
         float2* w = w32.get() + static_cast<long long>(g) * batch * n;
         hipLaunchKernelGGL(dev::random_wave, dim3(grid_for(n, 256, 2048), batch), dim3(256), 0, stream, w,
-                           amp_true.get(), n, p.seed, static_cast<std::uint64_t>(g) * batch);
+                           amp_true.get(), n, p.seed, static_cast<std::uint64_t>(g) * batch, per_problem);
         QPCH_HIP(hipGetLastError());
         device_fft(w, dims, batch, false, static_cast<float>(1.0 / std::sqrt(static_cast<double>(n))), stream);
       }
@@ -773,8 +900,9 @@ struct SolverImpl {
     }
     std::vector<float> tmp;
     QPCH_HIP(hipMemcpy(host, x32.get(), sizeof(float2) * n * starts, hipMemcpyDeviceToHost));
-    if (amp_scale != 1.0f) {
-      for (long long i = 0; i < 2 * n * starts; ++i) host[i] *= amp_scale;
+    for (int s = 0; s < starts; ++s) {
+      const float sc = amp_scales[s / per_problem];
+      if (sc != 1.0f) for (long long i = 0; i < 2 * n; ++i) host[2 * n * s + i] *= sc;
     }
   }
 };
@@ -783,6 +911,91 @@ struct SolverImpl {
 
 struct qpch_solver {
   qpch::SolverImpl impl;
+};
+
+// Several solvers advanced by one graph launch: each solver's passes are one branch of the
+// graph, forked from and joined back to the group's own stream.
+struct qpch_group {
+  std::vector<qpch_solver*> solvers;
+  hipStream_t main = nullptr;
+  std::vector<hipEvent_t> done;  // one per solver
+  hipEvent_t fork = nullptr, joined = nullptr;
+  struct Entry {
+    hipGraphExec_t exec;
+    std::vector<int> generations;
+  };
+  std::map<std::tuple<int, float, float, int>, Entry> graphs;
+
+  ~qpch_group() {
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second.exec);
+    for (hipEvent_t e : done) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+    if (joined) (void)hipEventDestroy(joined);
+    if (main) (void)hipStreamDestroy(main);
+  }
+
+  // an event recorded on the group stream now (it orders nothing it did not already)
+  hipEvent_t fork_ready() {
+    QPCH_HIP(hipEventRecord(fork, main));
+    return fork;
+  }
+
+  void iterate(int len, float a, float b, bool refine) {
+    if (len < 1) throw qpch::InvalidArgument("iterations must be >= 1");
+    int lowmask = 0;
+    for (size_t k = 0; k < solvers.size(); ++k) {
+      if (solvers[k]->impl.prepare(refine)) lowmask |= 1 << (k % 30);
+      QPCH_HIP(hipEventRecord(done[k], solvers[k]->impl.stream));
+      QPCH_HIP(hipStreamWaitEvent(main, done[k], 0));
+    }
+    if (qpch::eager_launches()) {
+      for (size_t k = 0; k < solvers.size(); ++k) {
+        auto& m = solvers[k]->impl;
+        QPCH_HIP(hipStreamWaitEvent(m.stream, fork_ready(), 0));
+        m.enqueue(len, a, b, (lowmask >> (k % 30)) & 1);
+      }
+      return;
+    }
+    std::vector<int> gens;
+    for (auto* s : solvers) gens.push_back(s->impl.generation);
+    const auto key = std::make_tuple(len, a, b, lowmask);
+    auto it = graphs.find(key);
+    if (it != graphs.end() && it->second.generations != gens) {
+      (void)hipGraphExecDestroy(it->second.exec);
+      graphs.erase(it);
+      it = graphs.end();
+    }
+    if (it == graphs.end()) {
+      QPCH_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
+      hipGraph_t graph{};
+      try {
+        QPCH_HIP(hipEventRecord(fork, main));
+        for (size_t k = 0; k < solvers.size(); ++k) {
+          auto& m = solvers[k]->impl;
+          QPCH_HIP(hipStreamWaitEvent(m.stream, fork, 0));
+          m.enqueue(len, a, b, (lowmask >> (k % 30)) & 1);
+          QPCH_HIP(hipEventRecord(done[k], m.stream));
+          QPCH_HIP(hipStreamWaitEvent(main, done[k], 0));
+        }
+      } catch (...) {
+        (void)hipStreamEndCapture(main, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+      }
+      QPCH_HIP(hipStreamEndCapture(main, &graph));
+      hipGraphExec_t exec{};
+      const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      QPCH_HIP(e);
+      it = graphs.emplace(key, Entry{exec, gens}).first;
+    }
+    QPCH_HIP(hipGraphLaunch(it->second.exec, main));
+    QPCH_HIP(hipEventRecord(joined, main));
+    for (auto* s : solvers) {
+      QPCH_HIP(hipStreamWaitEvent(s->impl.stream, joined, 0));
+      s->impl.wave_valid = true;
+    }
+  }
 };
 
 using qpch::guarded;
@@ -900,6 +1113,49 @@ int qpch_solver_read_state(qpch_solver* s, float* host) {
 
 double qpch_solver_energy(const qpch_solver* s) { return s->impl.energy; }
 
+int qpch_solver_energies(const qpch_solver* s, double* out) {
+  return guarded([&] {
+    for (int q = 0; q < s->impl.problems; ++q) out[q] = s->impl.energies[q];
+  });
+}
+
+int qpch_group_create(qpch_solver* const* solvers, int count, qpch_group** out) {
+  *out = nullptr;
+  return guarded([&] {
+    if (count < 1) throw qpch::InvalidArgument("a group needs at least one solver");
+    auto g = std::make_unique<qpch_group>();
+    for (int k = 0; k < count; ++k) {
+      if (!solvers[k]) throw qpch::InvalidArgument("null solver");
+      if (!solvers[k]->impl.fused) throw qpch::InvalidArgument("grouped solvers need the fused schedule");
+      g->solvers.push_back(solvers[k]);
+      hipEvent_t e{};
+      QPCH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      g->done.push_back(e);
+    }
+    QPCH_HIP(hipStreamCreateWithFlags(&g->main, hipStreamNonBlocking));
+    QPCH_HIP(hipEventCreateWithFlags(&g->fork, hipEventDisableTiming));
+    QPCH_HIP(hipEventCreateWithFlags(&g->joined, hipEventDisableTiming));
+    *out = g.release();
+  });
+}
+
+int qpch_group_iterate(qpch_group* g, int iterations, float a, float b, int refine) {
+  return guarded([&] { g->iterate(iterations, a, b, refine != 0); });
+}
+
+int qpch_group_synchronize(qpch_group* g) {
+  return guarded([&] {
+    QPCH_HIP(hipStreamSynchronize(g->main));
+    for (auto* s : g->solvers) QPCH_HIP(hipStreamSynchronize(s->impl.stream));
+  });
+}
+
+void qpch_group_destroy(qpch_group* g) {
+  if (!g) return;
+  (void)hipStreamSynchronize(g->main);
+  delete g;
+}
+
 int qpch_solver_time_passes(qpch_solver* s, int low, int reps, float* ms, int cap, int* count) {
   return guarded([&] {
     if (reps < 1) throw qpch::InvalidArgument("reps must be >= 1");
@@ -1008,8 +1264,183 @@ int qpch_gaussian_blur(const float* host_in, float* host_out, int ndim, const si
   });
 }
 
-int qpch_amplitude_loss(const void*, const void*, void*, double*, int, const size_t*, int, int, void*) {
-  return guarded([&] { throw qpch::InvalidArgument("qpch_amplitude_loss is not available yet"); });
+// --- autotuner -----------------------------------------------------------------------
+
+namespace {
+std::mutex g_tune_mu;
+std::map<std::string, qpch_tuning> g_tune_cache;
+std::string g_tune_log;
+
+std::string tune_key(const qpch_problem& p) {
+  std::string k = std::to_string(p.ndim);
+  for (int a = 0; a < p.ndim; ++a) k += "x" + std::to_string(p.shape[a]);
+  k += "/s" + std::to_string(p.starts * (p.problems > 1 ? p.problems : 1)) + "/st" + std::to_string(p.storage) +
+       "/m" + std::to_string(p.mixed) + "/b" + std::to_string(p.batch);
+  return k;
+}
+}  // namespace
+
+const char* qpch_autotune_log(void) { return g_tune_log.c_str(); }
+
+int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
+  return guarded([&] {
+    if (!problem) throw qpch::InvalidArgument("null problem");
+    const std::string key = tune_key(*problem);
+    {
+      std::lock_guard<std::mutex> lock(g_tune_mu);
+      auto it = g_tune_cache.find(key);
+      if (it != g_tune_cache.end()) {
+        *out = it->second;
+        return;
+      }
+    }
+    const qpch::Dims d = qpch::make_dims(problem->ndim, problem->shape);
+    const int total = problem->starts * (problem->problems > 1 ? problem->problems : 1);
+    std::vector<int> batches;
+    if (problem->batch > 0) {
+      batches.push_back(problem->batch);  // fixed by the caller: tune the rest
+    } else {
+      for (int b = total; b >= 1 && batches.size() < 4; b /= 2) {
+        if (total % b == 0) batches.push_back(b);
+        if (b < 8) break;
+      }
+    }
+    std::vector<int> planes{0};
+    if (d.nd == 3 && qpch::plane_supported(d.d[1], d.d[2])) planes = {1, 2};
+    std::string line = "qpch autotune " + key + ":";
+    qpch_tuning best{};
+    best.us_per_iteration = 1e300;
+    const bool lowp = problem->storage != QPCH_F32;
+    for (int b : batches) {
+      for (int pl : planes) {
+        for (int xcd : {0, 2}) {
+          if (xcd == 2 && !(b >= 8 && b % 8 == 0)) continue;  // numbering only differs then
+          qpch_problem q = *problem;
+          q.batch = b;
+          q.plane = pl;
+          q.xcd = xcd;
+          q.start = nullptr;
+          q.schedule = QPCH_GRAPH;
+          auto s = std::make_unique<qpch_solver>();
+          s->impl.init(q);
+          auto& m = s->impl;
+          m.begin_run(32);
+          m.iterate(4, 0.9f, -0.8f, !lowp);  // capture and warm up
+          hipEvent_t e0{}, e1{};
+          QPCH_HIP(hipEventCreate(&e0));
+          QPCH_HIP(hipEventCreate(&e1));
+          QPCH_HIP(hipEventRecord(e0, m.stream));
+          for (int r = 0; r < 4; ++r) m.iterate(4, 0.9f, -0.8f, !lowp);
+          QPCH_HIP(hipEventRecord(e1, m.stream));
+          QPCH_HIP(hipEventSynchronize(e1));
+          float ms = 0.0f;
+          QPCH_HIP(hipEventElapsedTime(&ms, e0, e1));
+          (void)hipEventDestroy(e0);
+          (void)hipEventDestroy(e1);
+          const double us = ms * 1e3 / 16.0;
+          line += " [batch " + std::to_string(b) + (pl ? (pl == 1 ? " plane" : " lines") : "") +
+                  (xcd == 2 ? " xcd-off" : "") + ": " + std::to_string(us) + " us/it]";
+          if (us < best.us_per_iteration) best = qpch_tuning{b, pl, xcd, us};
+        }
+      }
+    }
+    line += " -> batch " + std::to_string(best.batch) + " plane " + std::to_string(best.plane) + " xcd " +
+            std::to_string(best.xcd) + "\n";
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune_cache[key] = best;
+    g_tune_log += line;
+    if (const char* v = std::getenv("QPCH_TUNE_LOG"); v && v[0] == '1') std::fputs(line.c_str(), stderr);
+    *out = best;
+  });
+}
+
+int qpch_amplitude_loss(const void* x, const void* amplitude, long long amplitude_stride, void* grad,
+                        double* loss, int ndim, const size_t* shape, int batch, int precision, void* stream) {
+  return guarded([&] {
+    const qpch::Dims d = qpch::make_dims(ndim, shape);
+    if (batch < 1) throw qpch::InvalidArgument("batch must be >= 1");
+    if (precision != 0 && precision != 1) throw qpch::InvalidArgument("precision must be 0 (fp32) or 1 (fp64)");
+    if (!x || !amplitude || !grad || !loss) throw qpch::InvalidArgument("null pointer");
+    if (amplitude_stride != 0 && amplitude_stride != d.n) {
+      throw qpch::InvalidArgument("amplitude_stride must be 0 (shared) or the element count (one per object)");
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const long long total = d.n * batch;
+    const double s = 1.0 / std::sqrt(static_cast<double>(d.n));
+    // stream-ordered scratch: safe for callers that interleave several streams
+    struct Scratch {
+      hipStream_t st;
+      std::vector<void*> ptrs;
+      void* get(size_t bytes) {
+        void* p = nullptr;
+        QPCH_HIP(hipMallocAsync(&p, bytes, st));
+        ptrs.push_back(p);
+        return p;
+      }
+      ~Scratch() {
+        for (void* p : ptrs) (void)hipFreeAsync(p, st);
+      }
+    } scratch{st, {}};
+    if (precision == 0 && qpch::fusable(d)) {
+      auto* w = static_cast<float2*>(scratch.get(sizeof(float2) * total));
+      const int o0 = static_cast<int>(d.outer(0)), in0 = static_cast<int>(d.inner(0));
+      const int tiles = qpch::pass_tiles<8>(d.d[0], in0 > 1, o0, in0);
+      auto* partials = static_cast<double*>(scratch.get(sizeof(double) * tiles * batch));
+      auto* counter = static_cast<int*>(scratch.get(sizeof(int)));
+      auto pass = [&](int axis, qpch::dev::Mode mode, const void* src, float in_scale, float out_scale) {
+        qpch::dev::PassArgs p{};
+        p.outer = static_cast<int>(d.outer(axis));
+        p.inner = static_cast<int>(d.inner(axis));
+        p.n = d.n;
+        p.w = w;
+        p.src = src;
+        p.x = const_cast<void*>(x);
+        p.out = grad;
+        p.amp = static_cast<const float*>(amplitude);
+        p.amp_stride = amplitude_stride;
+        p.partials = partials;
+        p.partial_stride = tiles;
+        p.counter = counter;
+        p.in_scale = in_scale;
+        p.out_scale = out_scale;
+        qpch::launch_pass<float2>(d.d[axis], p.inner > 1, mode, p, batch, st);
+      };
+      const int last = d.nd - 1;
+      pass(last, qpch::dev::Mode::kFwd, x, 1.0f, 1.0f);
+      if (d.nd == 3) pass(1, qpch::dev::Mode::kFwd, nullptr, 1.0f, 1.0f);
+      pass(0, qpch::dev::Mode::kModulus, nullptr, static_cast<float>(s), static_cast<float>(s));
+      if (d.nd == 3) pass(1, qpch::dev::Mode::kInv, nullptr, 1.0f, 1.0f);
+      pass(last, qpch::dev::Mode::kGrad, nullptr, 1.0f, 1.0f);
+      hipLaunchKernelGGL(qpch::dev::sum_partials, dim3((batch + 63) / 64), dim3(64), 0, st, partials, tiles, loss,
+                         batch);
+      QPCH_HIP(hipGetLastError());
+      return;
+    }
+    const int g = qpch::grid_for(d.n, 256, 512);
+    auto* partials = static_cast<double*>(scratch.get(sizeof(double) * g * batch));
+    if (precision == 0) {
+      auto* w = static_cast<float2*>(scratch.get(sizeof(float2) * total));
+      QPCH_HIP(hipMemcpyAsync(w, x, sizeof(float2) * total, hipMemcpyDeviceToDevice, st));
+      qpch::device_fft(w, d, batch, true, 1.0f, st);
+      hipLaunchKernelGGL(qpch::dev::loss_project<float>, dim3(g, batch), dim3(256), 0, st, w,
+                         static_cast<const float*>(amplitude), amplitude_stride, d.n, static_cast<float>(s), partials);
+      qpch::device_fft(w, d, batch, false, 1.0f, st);
+      hipLaunchKernelGGL(qpch::dev::grad_from<float>, dim3(qpch::grid_for(total)), dim3(256), 0, st,
+                         static_cast<const float2*>(x), w, static_cast<float2*>(grad), total, 1.0f);
+    } else {
+      auto* w = static_cast<double2*>(scratch.get(sizeof(double2) * total));
+      QPCH_HIP(hipMemcpyAsync(w, x, sizeof(double2) * total, hipMemcpyDeviceToDevice, st));
+      qpch::device_fft_f64(w, d, batch, true, st);
+      hipLaunchKernelGGL(qpch::dev::loss_project<double>, dim3(g, batch), dim3(256), 0, st, w,
+                         static_cast<const double*>(amplitude), amplitude_stride, d.n, s, partials);
+      qpch::device_fft_f64(w, d, batch, false, st);
+      hipLaunchKernelGGL(qpch::dev::grad_from<double>, dim3(qpch::grid_for(total)), dim3(256), 0, st,
+                         static_cast<const double2*>(x), w, static_cast<double2*>(grad), total, 1.0);
+    }
+    QPCH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(qpch::dev::sum_partials, dim3((batch + 63) / 64), dim3(64), 0, st, partials, g, loss, batch);
+    QPCH_HIP(hipGetLastError());
+  });
 }
 
 }  // extern "C"

@@ -21,22 +21,46 @@ struct GenericPlan {
   int fac[32];
 };
 
+template <class Real>
+struct Cplx;
+template <>
+struct Cplx<float> {
+  using T = float2;
+};
+template <>
+struct Cplx<double> {
+  using T = double2;
+};
+
+template <class C>
+__device__ __forceinline__ C gadd(C a, C b) {
+  return {a.x + b.x, a.y + b.y};
+}
+template <class C>
+__device__ __forceinline__ C gmul(C a, C b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
 // One line per work-group, autosort mixed-radix stages ping-ponging between two LDS
-// buffers; radix-r butterflies are evaluated as O(r^2) sums with the line's own W_N table.
-// Lines are addressed like the fused passes: element k of line (o, c) of start s at
-// s*n + o*N*inner + k*inner + c, line = o*inner + c = blockIdx.x.
+// buffers; radix-r butterflies are evaluated as O(r^2) sums with the line's own W_N table
+// (evaluated in double, rounded once to Real).  Lines are addressed like the fused passes:
+// element k of line (o, c) of start s at s*n + o*N*inner + k*inner + c, line = o*inner + c
+// = blockIdx.x.  Real = float (the unfused schedule) or double (the FP64 adjoint).
+template <class Real>
 __global__ void __launch_bounds__(kGenericThreads)
-    generic_fft(float2* data, long long n_per_start, int inner, GenericPlan plan, int forward,
-                float scale) {
-  extern __shared__ float2 sm[];
+    generic_fft(typename Cplx<Real>::T* data, long long n_per_start, int inner, GenericPlan plan,
+                int forward, Real scale) {
+  using C = typename Cplx<Real>::T;
+  extern __shared__ unsigned char sm_raw[];
+  C* sm = reinterpret_cast<C*>(sm_raw);
   const int N = plan.n;
-  float2* bufa = sm;
-  float2* bufb = sm + N;
-  float2* tw = sm + 2 * N;
+  C* bufa = sm;
+  C* bufb = sm + N;
+  C* tw = sm + 2 * N;
   for (int j = threadIdx.x; j < N; j += blockDim.x) {
     double s, c;
     sincospi(2.0 * static_cast<double>(j) / static_cast<double>(N), &s, &c);
-    tw[j] = {static_cast<float>(c), static_cast<float>(forward ? -s : s)};
+    tw[j] = {static_cast<Real>(c), static_cast<Real>(forward ? -s : s)};
   }
   const int o = blockIdx.x / inner;
   const int c = blockIdx.x % inner;
@@ -44,19 +68,19 @@ __global__ void __launch_bounds__(kGenericThreads)
                          static_cast<long long>(o) * N * inner + c;
   for (int k = threadIdx.x; k < N; k += blockDim.x) bufa[k] = data[base + static_cast<long long>(k) * inner];
   __syncthreads();
-  float2* src = bufa;
-  float2* dst = bufb;
+  C* src = bufa;
+  C* dst = bufb;
   if (plan.nfac == 0) {
     for (int j = threadIdx.x; j < N; j += blockDim.x) {
-      float2 acc = {0.0f, 0.0f};
+      C acc = {0, 0};
       for (int q = 0; q < N; ++q) {
-        const float2 w = tw[static_cast<int>((static_cast<long long>(j) * q) % N)];
-        acc = cadd(acc, cmul(src[q], w));
+        const C w = tw[static_cast<int>((static_cast<long long>(j) * q) % N)];
+        acc = gadd(acc, gmul(src[q], w));
       }
       dst[j] = acc;
     }
     __syncthreads();
-    float2* t = src;
+    C* t = src;
     src = dst;
     dst = t;
   } else {
@@ -69,26 +93,26 @@ __global__ void __launch_bounds__(kGenericThreads)
         const int k = t % L;
         const int outb = (t - k) * r + k;
         for (int j = 0; j < r; ++j) {
-          float2 acc = {0.0f, 0.0f};
+          C acc = {0, 0};
           for (int q = 0; q < r; ++q) {
-            float2 v = src[t + q * nr];
+            C v = src[t + q * nr];
             const int tq = q * k * m;  // stage twiddle W_{L r}^{q k}
-            if (tq) v = cmul(v, tw[tq]);
+            if (tq) v = gmul(v, tw[tq]);
             const int e = ((j * q) % r) * nr;  // W_r^{j q} = W_N^{(j q mod r) N / r}
-            acc = cadd(acc, e ? cmul(v, tw[e]) : v);
+            acc = gadd(acc, e ? gmul(v, tw[e]) : v);
           }
           dst[outb + j * L] = acc;
         }
       }
       __syncthreads();
-      float2* tt = src;
+      C* tt = src;
       src = dst;
       dst = tt;
       L *= r;
     }
   }
   for (int k = threadIdx.x; k < N; k += blockDim.x) {
-    const float2 v = src[k];
+    const C v = src[k];
     data[base + static_cast<long long>(k) * inner] = {v.x * scale, v.y * scale};
   }
 }
@@ -144,6 +168,57 @@ __global__ void __launch_bounds__(256)
       }
     }
   }
+}
+
+// --- amplitude loss (FP64 and non-power-of-two path) ---------------------------------
+// w holds the unnormalised spectrum of x; afterwards it holds the input of an unnormalised
+// inverse transform that lands on P_M x, and partials[start][block] the FP64 sums of
+// (|F x| - A)^2 over usable pixels (amp < 0 marks a masked pixel).
+template <class Real>
+__global__ void __launch_bounds__(256)
+    loss_project(typename Cplx<Real>::T* w, const Real* amp, long long amp_stride, long long n, Real scale,
+                 double* partials) {
+  __shared__ double red[16];
+  const long long soff = static_cast<long long>(blockIdx.y) * n;
+  double acc = 0.0;
+  for (long long k = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; k < n;
+       k += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const auto v = w[soff + k];
+    const Real re = v.x * scale, im = v.y * scale;
+    const Real mag = sqrt(re * re + im * im);
+    const Real a = amp[blockIdx.y * amp_stride + k];
+    Real target = mag;
+    if (a >= Real(0)) {
+      const Real d = mag - a;
+      acc += static_cast<double>(d) * static_cast<double>(d);
+      target = a;
+    }
+    const Real f = target / (mag + Real(1e-12)) * scale;
+    w[soff + k] = {re * f, im * f};
+  }
+  const double t = block_sum(acc, red);
+  if (threadIdx.x == 0) partials[static_cast<long long>(blockIdx.y) * gridDim.x + blockIdx.x] = t;
+}
+
+// grad = 2 (x - P_M x), P_M x = in_scale * p
+template <class Real>
+__global__ void grad_from(const typename Cplx<Real>::T* x, const typename Cplx<Real>::T* p,
+                          typename Cplx<Real>::T* grad, long long total, Real in_scale) {
+  for (long long k = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; k < total;
+       k += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const auto xv = x[k];
+    const auto pv = p[k];
+    grad[k] = {Real(2) * (xv.x - in_scale * pv.x), Real(2) * (xv.y - in_scale * pv.y)};
+  }
+}
+
+// loss[s] = sum of the `count` partials of start s, in order.
+__global__ void sum_partials(const double* partials, int count, double* loss, int starts) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= starts) return;
+  double t = 0.0;
+  for (int k = 0; k < count; ++k) t += partials[static_cast<long long>(s) * count + k];
+  loss[s] = t;
 }
 
 }  // namespace qpch::dev

@@ -27,9 +27,13 @@ __device__ __forceinline__ float uniform01(std::uint64_t seed, std::uint64_t sta
 
 // w = A e^{i 2 pi u}: the spectrum of a random start before its unitary inverse
 // transform (src/cpu/kernels.cpp:104-111).
+// Start g of the launch belongs to problem (first + g) / per_problem and is that
+// problem's start (first + g) % per_problem; each problem has its own amplitude row.
 __global__ void random_wave(float2* w, const float* amp, long long n, std::uint64_t seed,
-                            std::uint64_t first_start) {
-  const std::uint64_t start = first_start + blockIdx.y;
+                            std::uint64_t first_start, int per_problem) {
+  const std::uint64_t global = first_start + blockIdx.y;
+  const std::uint64_t start = global % static_cast<std::uint64_t>(per_problem);
+  amp += static_cast<long long>(global / static_cast<std::uint64_t>(per_problem)) * n;
   const long long soff = static_cast<long long>(blockIdx.y) * n;
   for (long long k = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; k < n;
        k += static_cast<long long>(gridDim.x) * blockDim.x) {

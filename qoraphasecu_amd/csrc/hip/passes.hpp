@@ -15,6 +15,8 @@
 //   kUpdate      inverse -> P_M x, x = S ? P_M x : a x + b P_M x, forward of the new x:
 //                the real-axis pass; reference src/cpu/kernels.cpp:44-50,
 //                src/core/pointwise.hpp:34-45
+//   kGrad        inverse -> P_M x, out = 2 (x - P_M x): the last pass of the amplitude
+//                loss gradient (no reference counterpart)
 // The error numerator of a start is reduced in a fixed order: the modulus pass writes one
 // FP64 partial per work-group, the next update pass (work-group 0 of each start) sums
 // them and stores errs[slot][start]; slot = *counter - 1, the modulus pass having
@@ -30,7 +32,7 @@
 
 namespace qpch::dev {
 
-enum class Mode : int { kFwd = 0, kInv = 1, kModulus = 2, kUpdate = 3 };
+enum class Mode : int { kFwd = 0, kInv = 1, kModulus = 2, kUpdate = 3, kGrad = 4 };
 
 struct PassArgs {
   int outer;
@@ -38,9 +40,12 @@ struct PassArgs {
   long long n;  // elements per start
   void* w;      // wave, St*
   const void* src;  // kFwd/kInv input when not null (St_in = St)
-  void* x;          // kUpdate: iterate, St*
+  void* x;          // kUpdate / kGrad: iterate, St*
+  void* out;        // kGrad: gradient, St*
   const std::uint8_t* support;  // kUpdate: [start][n]
   const float* amp;             // kModulus: signed amplitude [n], -1 = masked
+  long long amp_stride;         // 0: one amplitude for every start; n: one per problem
+  int amp_group;                // consecutive starts per problem (0 or 1: one per start)
   double* partials;             // [start][partial_stride]
   int partial_stride;
   double* errs;                 // [slot][err_stride]
@@ -50,6 +55,7 @@ struct PassArgs {
   int* counter;
   int starts;                   // starts in this launch
   int slots;                    // > 0: XCD-affine grid, block = tile * slots + start
+  int no_xcd;                   // launcher input: 1 keeps the plain block numbering
   float a, b;                   // update coefficients
   float in_scale;               // applied after the first transform of the pass
   float out_scale;              // applied before the store (kModulus: after projection)
@@ -66,14 +72,17 @@ struct PassGeom {
   static constexpr int kLdsLimit = 96 * 1024 / 8 - N;  // float2 slots for the line tiles
   static constexpr int pow2_floor(int v) { return v <= 1 ? 1 : 2 * pow2_floor(v / 2); }
   static constexpr int clampB(int b) {
-    int lim = kLdsLimit / S::kLine;
+    int lim = kLdsLimit / (S::kLine + 1);
     if (1024 / kT < lim) lim = 1024 / kT;
     return pow2_floor(b < lim ? b : lim);
   }
   static constexpr int kColB = 128 / EB;
   static constexpr int kB = clampB(COL ? (256 / kT > kColB ? 256 / kT : kColB) : (256 / kT > 1 ? 256 / kT : 1));
   static constexpr int kThreads = kB * kT;
-  static constexpr int kTile = kB * S::kLine;  // float2
+  // Column passes put consecutive lanes on different lines: an odd line stride (in float2)
+  // spreads them over the LDS banks; a row pass's lanes walk along one line.
+  static constexpr int kStride = COL ? (S::kLine | 1) : S::kLine;
+  static constexpr int kTile = kB * kStride;  // float2
 };
 
 // Sum over the work-group in a fixed order; the result is valid in thread 0.
@@ -168,7 +177,7 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St)>::kThreads))
   }
   const long long es = COL ? p.inner : 1;
   const long long soff = static_cast<long long>(start) * p.n;
-  float2* line = lds + b * Shape1<N, 16>::kLine;
+  float2* line = lds + b * G::kStride;
 
   St* w = static_cast<St*>(p.w);
   const St* src = p.src ? static_cast<const St*>(p.src) : w;
@@ -195,8 +204,9 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St)>::kThreads))
       v[m] = valid ? Io::ld(w, soff + base + (i + m * kT) * es) : float2{0.0f, 0.0f};
     }
     float amp[kR];
+    const long long arow = p.amp_stride * ((p.err_offset + start) / (p.amp_group > 1 ? p.amp_group : 1));
 #pragma unroll
-    for (int m = 0; m < kR; ++m) amp[m] = valid ? p.amp[base + (i + m * kT) * es] : -1.0f;
+    for (int m = 0; m < kR; ++m) amp[m] = valid ? p.amp[arow + base + (i + m * kT) * es] : -1.0f;
     load_twiddles<N>(tw);
     __syncthreads();
     fft_line<N, 16, true>(v, i, line, tw);
@@ -219,6 +229,28 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St)>::kThreads))
     if (tid == 0) {
       p.partials[static_cast<long long>(start) * p.partial_stride + tile] = total;
       if (tile == 0 && start == 0) atomicAdd(p.counter, 1);
+    }
+  } else if constexpr (M == Mode::kGrad) {
+    const St* x = static_cast<const St*>(p.x);
+#pragma unroll
+    for (int m = 0; m < kR; ++m) {
+      v[m] = valid ? Io::ld(w, soff + base + (i + m * kT) * es) : float2{0.0f, 0.0f};
+    }
+    float2 xo[kR];
+#pragma unroll
+    for (int m = 0; m < kR; ++m) {
+      xo[m] = valid ? Io::ld(x, soff + base + (i + m * kT) * es) : float2{0.0f, 0.0f};
+    }
+    load_twiddles<N>(tw);
+    __syncthreads();
+    fft_line<N, 16, false>(v, i, line, tw);
+    if (valid) {
+      St* out = static_cast<St*>(p.out);
+#pragma unroll
+      for (int m = 0; m < kR; ++m) {
+        const float2 pm = {v[m].x * p.in_scale, v[m].y * p.in_scale};
+        Io::st(out, soff + base + (i + m * kT) * es, float2{2.0f * (xo[m].x - pm.x), 2.0f * (xo[m].y - pm.y)});
+      }
     }
   } else {  // kUpdate
     St* x = static_cast<St*>(p.x);

@@ -1,0 +1,149 @@
+// The real-space pass of a 3D iteration on whole xy-planes: one work-group holds a
+// NY x NX plane (up to 128 x 128, 141 KB of LDS), so the y and x inverse transforms, the
+// update and the y and x forward transforms all happen between one read and one write of
+// the plane.  With the z-line modulus pass this makes a 3D iteration two HBM passes
+// (53 B per element in fp32, 29 B in fp16), like 2D.
+//
+// Thread layouts (radix 16, each thread holds 16 elements of one line):
+//   row layout:    thread t -> row y = t / (NX/16), butterfly i = t % (NX/16), x = i + m NX/16
+//   column layout: thread t -> column x = t % NX, butterfly j = t / NX, y = j + m NY/16
+// The plane moves between the two through an LDS transpose.  The wave is read and written
+// in the row layout (runs of NX/16 consecutive elements per row); the iterate and the
+// support in the column layout (each wave of 64 lanes covers 64 consecutive x).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fft_engine.hpp"
+#include "passes.hpp"
+#include "storage.hpp"
+
+namespace qpch::dev {
+
+template <int NY, int NX>
+struct PlaneGeom {
+  using SX = Shape1<NX, 16>;
+  using SY = Shape1<NY, 16>;
+  static_assert(SX::kR == 16 && SY::kR == 16, "plane passes need both axes >= 16");
+  static constexpr int kThreads = NY * NX / 16;
+  static constexpr int kTX = NX / 16;
+  static constexpr int kTY = NY / 16;
+  // y-lines are handled by consecutive lanes: an odd stride spreads them over the banks
+  static constexpr int kColStride = SY::kLine | 1;
+  // transpose rows: 8 lanes write a row, 8 rows a wave; a row stride of 8 float2 more than
+  // a multiple of 32 puts four rows side by side on the 64 banks
+  static constexpr int kTStride = NX >= 32 ? NX + 8 : NX + 1;
+  static constexpr int kRowLines = NY * SX::kLine;  // x-line scratch
+  static constexpr int kColLines = NX * kColStride;  // y-line scratch
+  static constexpr int kTranspose = NY * kTStride;  // padded plane
+  static constexpr int kMax1 = kRowLines > kColLines ? kRowLines : kColLines;
+  static constexpr int kScratch = kMax1 > kTranspose ? kMax1 : kTranspose;
+  static constexpr int kLds = kScratch + NX + NY;  // + both twiddle tables (float2)
+};
+
+// x-layout registers -> column layout through the padded LDS plane
+template <int NY, int NX>
+__device__ __forceinline__ void rows_to_cols(float2* v, int y, int i, int x, int j, float2* t) {
+  using G = PlaneGeom<NY, NX>;
+#pragma unroll
+  for (int m = 0; m < 16; ++m) t[y * G::kTStride + i + m * G::kTX] = v[m];
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 16; ++m) v[m] = t[(j + m * G::kTY) * G::kTStride + x];
+  __syncthreads();
+}
+
+template <int NY, int NX>
+__device__ __forceinline__ void cols_to_rows(float2* v, int y, int i, int x, int j, float2* t) {
+  using G = PlaneGeom<NY, NX>;
+#pragma unroll
+  for (int m = 0; m < 16; ++m) t[(j + m * G::kTY) * G::kTStride + x] = v[m];
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 16; ++m) v[m] = t[y * G::kTStride + i + m * G::kTX];
+  __syncthreads();
+}
+
+// The plane update pass: the same argument block as the line passes (outer = planes per
+// start, n = elements per start); block numbering as in line_pass, one tile per plane.
+template <int NY, int NX, class St>
+__global__ void __launch_bounds__((PlaneGeom<NY, NX>::kThreads))
+    plane_update(PassArgs p) {
+  using G = PlaneGeom<NY, NX>;
+  using Io = Storage<St>;
+  __shared__ float2 lds[G::kLds];
+  __shared__ double red[16];
+  float2* twx = lds + G::kScratch;
+  float2* twy = twx + NX;
+
+  int tile, start;
+  if (p.slots > 0) {
+    start = blockIdx.x % p.slots;
+    tile = blockIdx.x / p.slots;
+    if (start >= p.starts) return;
+  } else {
+    const int tiles = gridDim.x / p.starts;
+    start = blockIdx.x / tiles;
+    tile = blockIdx.x % tiles;
+  }
+  const int t = threadIdx.x;
+  const int y = t / G::kTX, i = t % G::kTX;  // row layout
+  const int x = t % NX, j = t / NX;          // column layout
+  const long long plane = static_cast<long long>(start) * p.n + static_cast<long long>(tile) * NY * NX;
+  St* w = static_cast<St*>(p.w);
+  St* xs = static_cast<St*>(p.x);
+  const std::uint8_t* sup = p.support;
+
+  float2 v[16];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) v[m] = Io::ld(w, plane + static_cast<long long>(y) * NX + i + m * G::kTX);
+  load_twiddles<NX>(twx);
+  load_twiddles<NY>(twy);
+  __syncthreads();
+
+  // P_M x: x inverse, y inverse
+  fft_line<NX, 16, false>(v, i, lds + y * G::SX::kLine, twx);
+  rows_to_cols<NY, NX>(v, y, i, x, j, lds);
+  fft_line<NY, 16, false>(v, j, lds + x * G::kColStride, twy);
+  // (the old iterate and the support are read here rather than prefetched: at 128 x 128
+  // the 1024-thread block has 128 registers per thread, and holding them across the
+  // transforms would spill)
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const long long at = plane + static_cast<long long>(j + m * G::kTY) * NX + x;
+    const float2 pm = {v[m].x * p.in_scale, v[m].y * p.in_scale};
+    float2 nx = pm;
+    if (!sup[at]) {
+      const float2 xo = Io::ld(xs, at);
+      nx = {p.a * xo.x + p.b * pm.x, p.a * xo.y + p.b * pm.y};
+    }
+    Io::st(xs, at, nx);
+    v[m] = nx;
+  }
+  // the next iteration's partial spectrum: y forward, x forward
+  fft_line<NY, 16, true>(v, j, lds + x * G::kColStride, twy);
+  cols_to_rows<NY, NX>(v, y, i, x, j, lds);
+  fft_line<NX, 16, true>(v, i, lds + y * G::SX::kLine, twx);
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    Io::st(w, plane + static_cast<long long>(y) * NX + i + m * G::kTX,
+           float2{v[m].x * p.out_scale, v[m].y * p.out_scale});
+  }
+
+  if (tile == 0 && t < 64 && p.errs) {
+    double s = 0.0;
+    const double* part = p.partials + static_cast<long long>(start) * p.partial_stride;
+    for (int k = t; k < p.partial_stride; k += 64) s += part[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if (t == 0) {
+      const int slot = *p.counter - 1;
+      if (slot >= 0 && slot < p.err_cap) {
+        p.errs[static_cast<long long>(slot) * p.err_stride + p.err_offset + start] = s;
+      }
+    }
+  }
+  (void)red;
+}
+
+}  // namespace qpch::dev

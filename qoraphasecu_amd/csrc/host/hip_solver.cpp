@@ -2,7 +2,10 @@
 // operation behind the qpch C ABI.  Reference counterpart: the CudaSolver of
 // src/cuda/cuda_solver.cu:58-500 (run 145-163, results 165-191, apply 236-276,
 // iterate 300-338, collect_errors/report 423-453).
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -18,12 +21,34 @@ namespace {
 
 constexpr double kEps = 1e-12;  // src/core/pointwise.hpp:16
 
+bool is_fusable(const Shape& shape) {
+  for (std::size_t a : shape) {
+    if (a < 2 || a > 4096 || (a & (a - 1)) != 0) return false;
+  }
+  return true;
+}
+
 struct Plan {
   int schedule;
   int storage;
   int mixed;
   bool batched;
 };
+
+// Launch configuration from the engine's autotuner (starts per launch, 3D plane or line
+// passes, XCD-affine numbering); QPCH_TUNE=0 keeps the defaults.  The configuration does
+// not change results.
+void tune(qpch_problem& q) {
+  if (const char* v = std::getenv("QPCH_TUNE"); v && v[0] == '0') {
+    if (q.batch == 0) q.batch = q.starts * (q.problems > 1 ? q.problems : 1);
+    return;
+  }
+  qpch_tuning t{};
+  qpch_check(qpch_autotune(&q, &t));
+  q.batch = t.batch;
+  q.plane = t.plane;
+  q.xcd = t.xcd;
+}
 
 Plan plan_for(Variant v) {
   switch (v) {
@@ -76,6 +101,7 @@ class HipSolver final : public Solver {
     p.storage = plan.storage;
     p.mixed = plan.mixed;
     p.schedule = plan.schedule;
+    if (plan.schedule == QPCH_GRAPH && is_fusable(shape_)) tune(p);
     qpch_check(qpch_solver_create(&p, &h_));
     energy_ = qpch_solver_energy(h_);
   }
@@ -185,7 +211,222 @@ class HipSolver final : public Solver {
   int run_iterations_ = 0;
 };
 
+// Several problems (any mix of shapes) in one engine: problems of one shape are stacked in
+// one device solver, and the solvers of the different shapes advance together, every
+// chunk of iterations one graph launch with a branch per shape.  The recipe walk, progress
+// and result layout follow HipSolver.
+class HipEnsemble final : public MultiSolver {
+ public:
+  HipEnsemble(std::span<const Problem> problems, const Options& options)
+      : options_(options), problems_(problems.size()), errors_(problems.size()) {
+    const Plan plan = plan_for(options.variant);
+    std::map<Shape, std::vector<std::size_t>> by_shape;
+    for (std::size_t k = 0; k < problems.size(); ++k) by_shape[problems[k].shape].push_back(k);
+    std::size_t start_offset = 0;
+    std::vector<std::size_t> offsets(problems.size());
+    for (std::size_t k = 0; k < problems.size(); ++k) {
+      offsets[k] = start_offset;
+      start_offset += numel(problems[k].shape) * static_cast<std::size_t>(options.starts);
+    }
+    if (!options.start.empty() && options.start.size() != start_offset) {
+      throw std::invalid_argument("start must hold one array per start of every problem");
+    }
+    for (auto& [shape, members] : by_shape) {
+      Class c;
+      c.shape = shape;
+      c.n = numel(shape);
+      c.members = members;
+      std::vector<float> amp;
+      std::vector<std::uint8_t> mask, sup;
+      std::vector<Complex> start;
+      for (std::size_t k : members) {
+        const Problem& p = problems[k];
+        const auto a = to_fft_order<float>(p.amplitude, shape);
+        const auto m = to_fft_order<std::uint8_t>(p.mask, shape);
+        const auto su = to_fft_order<std::uint8_t>(p.support, shape);
+        amp.insert(amp.end(), a.begin(), a.end());
+        mask.insert(mask.end(), m.begin(), m.end());
+        sup.insert(sup.end(), su.begin(), su.end());
+        if (!options.start.empty()) {
+          for (int s = 0; s < options.starts; ++s) {
+            const auto part = to_fft_order<Complex>(
+                std::span(options.start).subspan(offsets[k] + static_cast<std::size_t>(s) * c.n, c.n), shape);
+            start.insert(start.end(), part.begin(), part.end());
+          }
+        }
+      }
+      qpch_problem q{};
+      q.ndim = static_cast<int>(shape.size());
+      for (std::size_t a = 0; a < shape.size(); ++a) q.shape[a] = shape[a];
+      q.amplitude = amp.data();
+      q.mask = mask.data();
+      q.support = sup.data();
+      q.starts = options.starts;
+      q.batch = options.batch;
+      q.seed = options.seed;
+      q.start = start.empty() ? nullptr : reinterpret_cast<const float*>(start.data());
+      q.storage = plan.storage;
+      q.mixed = plan.mixed;
+      q.schedule = QPCH_GRAPH;
+      q.problems = static_cast<int>(members.size());
+      tune(q);
+      qpch_check(qpch_solver_create(&q, &c.h));
+      c.energies.resize(members.size());
+      qpch_check(qpch_solver_energies(c.h, c.energies.data()));
+      classes_.push_back(std::move(c));
+    }
+    std::vector<qpch_solver*> hs;
+    for (auto& c : classes_) hs.push_back(c.h);
+    qpch_check(qpch_group_create(hs.data(), static_cast<int>(hs.size()), &group_));
+  }
+
+  ~HipEnsemble() override {
+    if (group_) qpch_group_destroy(group_);
+    for (auto& c : classes_) qpch_solver_destroy(c.h);
+  }
+
+  HipEnsemble(const HipEnsemble&) = delete;
+  HipEnsemble& operator=(const HipEnsemble&) = delete;
+
+  void run(const Recipe& recipe) override {
+    validate(recipe);
+    const int capacity = recipe.iterations();
+    run_iterations_ = 0;
+    if (capacity > 0) {
+      for (auto& c : classes_) qpch_check(qpch_solver_begin_run(c.h, capacity));
+    }
+    for (const Op& op : recipe.ops) {
+      std::visit([this](const auto& o) { apply(o); }, op);
+    }
+    if (run_iterations_ > 0) collect_errors();
+    qpch_check(qpch_group_synchronize(group_));
+  }
+
+  std::vector<std::vector<Reconstruction>> results() const override {
+    std::vector<std::vector<Reconstruction>> out(problems_);
+    const auto starts = static_cast<std::size_t>(options_.starts);
+    for (const auto& c : classes_) {
+      const std::size_t total = c.members.size() * starts;
+      std::vector<Complex> x(c.n * total);
+      std::vector<std::uint8_t> s(c.n * total);
+      qpch_check(qpch_solver_read_state(c.h, reinterpret_cast<float*>(x.data())));
+      qpch_check(qpch_solver_get_support(c.h, s.data()));
+      for (std::size_t q = 0; q < c.members.size(); ++q) {
+        const std::size_t k = c.members[q];
+        for (std::size_t t = 0; t < starts; ++t) {
+          const std::size_t off = (q * starts + t) * c.n;
+          out[k].push_back({to_centered<Complex>(std::span(x).subspan(off, c.n), c.shape),
+                            to_centered<std::uint8_t>(std::span(s).subspan(off, c.n), c.shape),
+                            errors_[k][t]});
+        }
+      }
+    }
+    return out;
+  }
+
+ private:
+  struct Class {
+    Shape shape;
+    std::size_t n = 0;
+    std::vector<std::size_t> members;
+    qpch_solver* h = nullptr;
+    std::vector<double> energies;
+  };
+
+  void apply(const ER& o) { iterate(o.n, {0.0f, 0.0f}, true); }
+  void apply(const HIO& o) { iterate(o.n, {1.0f, -o.beta}, false); }
+  void apply(const RAAR& o) { iterate(o.n, {o.beta, 1.0f - 2.0f * o.beta}, false); }
+
+  void apply(const Shrinkwrap& o) {
+    for (auto& c : classes_) {
+      qpch_check(qpch_solver_shrinkwrap(c.h, o.threshold, o.sigma, o.method == Threshold::max ? 0 : 1));
+    }
+  }
+
+  void apply(const DetwinBegin& o) {
+    for (auto& c : classes_) {
+      qpch_check(qpch_solver_save_support(c.h));
+      const std::size_t total = c.members.size() * static_cast<std::size_t>(options_.starts);
+      std::vector<std::uint8_t> host(c.n * total);
+      qpch_check(qpch_solver_get_support(c.h, host.data()));
+      for (std::size_t k = 0; k < total; ++k) {
+        const auto part = std::span(host).subspan(k * c.n, c.n);
+        const auto centred = to_centered<std::uint8_t>(part, c.shape);
+        const auto half = to_fft_order<std::uint8_t>(half_support(centred, c.shape, o.axis), c.shape);
+        std::copy(half.begin(), half.end(), part.begin());
+      }
+      qpch_check(qpch_solver_set_support(c.h, host.data()));
+    }
+  }
+
+  void apply(const DetwinEnd&) {
+    for (auto& c : classes_) qpch_check(qpch_solver_restore_support(c.h));
+  }
+
+  void iterate(int n, Coefficients cf, bool refine) {
+    for (int done = 0; done < n;) {
+      const int len = std::min(n - done, options_.sync_every - iteration_ % options_.sync_every);
+      qpch_check(qpch_group_iterate(group_, len, cf.a, cf.b, refine ? 1 : 0));
+      done += len;
+      iteration_ += len;
+      run_iterations_ += len;
+      if (options_.progress && iteration_ % options_.sync_every == 0) report();
+    }
+  }
+
+  // per problem p, start t: errors of class start (q * starts + t)
+  void collect_errors() {
+    const int starts = options_.starts;
+    for (auto& c : classes_) {
+      const int total = static_cast<int>(c.members.size()) * starts;
+      std::vector<double> host(static_cast<std::size_t>(run_iterations_) * total);
+      qpch_check(qpch_solver_read_errors(c.h, 0, run_iterations_, host.data()));
+      for (std::size_t q = 0; q < c.members.size(); ++q) {
+        auto& errs = errors_[c.members[q]];
+        errs.resize(static_cast<std::size_t>(starts));
+        for (int t = 0; t < run_iterations_; ++t) {
+          for (int s = 0; s < starts; ++s) {
+            const double num = host[static_cast<std::size_t>(t) * total + q * starts + s];
+            errs[static_cast<std::size_t>(s)].push_back(
+                static_cast<float>(std::sqrt(num / (c.energies[q] + kEps))));
+          }
+        }
+      }
+    }
+  }
+
+  void report() {
+    const int starts = options_.starts;
+    std::vector<float> latest(problems_ * static_cast<std::size_t>(starts));
+    for (auto& c : classes_) {
+      const int total = static_cast<int>(c.members.size()) * starts;
+      std::vector<double> host(static_cast<std::size_t>(total));
+      qpch_check(qpch_solver_read_errors(c.h, run_iterations_ - 1, 1, host.data()));
+      for (std::size_t q = 0; q < c.members.size(); ++q) {
+        for (int s = 0; s < starts; ++s) {
+          latest[c.members[q] * starts + s] =
+              static_cast<float>(std::sqrt(host[q * starts + s] / (c.energies[q] + kEps)));
+        }
+      }
+    }
+    options_.progress(iteration_, latest);
+  }
+
+  Options options_;
+  std::size_t problems_;
+  std::vector<Class> classes_;
+  qpch_group* group_ = nullptr;
+  std::vector<std::vector<std::vector<float>>> errors_;  // [problem][start][iteration]
+  int iteration_ = 0;
+  int run_iterations_ = 0;
+};
+
 }  // namespace
+
+std::unique_ptr<MultiSolver> make_hip_ensemble(std::span<const Problem> problems, const Options& options) {
+  if (!hip_available()) throw std::runtime_error("no HIP device available");
+  return std::make_unique<HipEnsemble>(problems, options);
+}
 
 std::unique_ptr<Solver> make_hip_solver(const Problem& problem, const Options& options) {
   if (!hip_available()) throw std::runtime_error("no HIP device available");

@@ -99,14 +99,67 @@ std::vector<Reconstruction> reconstruct(const Problem& problem, const Recipe& re
   return solver->results();
 }
 
-std::vector<std::vector<Reconstruction>> reconstruct(std::span<const Problem> problems,
-                                                     const Recipe& recipe, const Options& options) {
+namespace {
+
+// One solver per problem, run one after another.
+class SequentialMultiSolver final : public MultiSolver {
+ public:
+  SequentialMultiSolver(std::span<const Problem> problems, const Options& options) {
+    for (const Problem& p : problems) solvers_.push_back(make_solver(p, options));
+  }
+  void run(const Recipe& recipe) override {
+    validate(recipe);
+    for (auto& s : solvers_) s->run(recipe);
+  }
+  std::vector<std::vector<Reconstruction>> results() const override {
+    std::vector<std::vector<Reconstruction>> out;
+    for (const auto& s : solvers_) out.push_back(s->results());
+    return out;
+  }
+
+ private:
+  std::vector<std::unique_ptr<Solver>> solvers_;
+};
+
+}  // namespace
+
+std::unique_ptr<MultiSolver> make_multi_solver(std::span<const Problem> problems, const Options& options) {
   if (problems.empty()) throw std::invalid_argument("no problems");
+  if (options.starts < 1) throw std::invalid_argument("starts must be >= 1");
+  if (options.sync_every < 1) throw std::invalid_argument("sync_every must be >= 1");
+  bool ensemble = options.backend == Backend::hip;
+  switch (options.variant) {
+    case Variant::naive:
+    case Variant::fused:
+    case Variant::graph:
+    case Variant::callback:
+      ensemble = false;  // one start per launch: the problems run one after another
+      break;
+    default:
+      break;
+  }
+  for (const Problem& p : problems) {
+    const std::size_t n = numel(p.shape);
+    if (p.shape.size() != 2 && p.shape.size() != 3) throw std::invalid_argument("problem shape must be 2D or 3D");
+    if (p.amplitude.size() != n || p.mask.size() != n || p.support.size() != n) {
+      throw std::invalid_argument("problem arrays do not match its shape; build it with make_problem");
+    }
+    for (std::size_t a : p.shape) ensemble = ensemble && a >= 2 && (a & (a - 1)) == 0 && a <= 4096;
+  }
+  if (!ensemble) {
+    if (!options.start.empty()) throw std::invalid_argument("start is not supported for several problems here");
+    return std::make_unique<SequentialMultiSolver>(problems, options);
+  }
+  if (options.batch < 0) throw std::invalid_argument("batch must be >= 0");
+  return detail::make_hip_ensemble(problems, options);
+}
+
+std::vector<std::vector<Reconstruction>> reconstruct(std::span<const Problem> problems, const Recipe& recipe,
+                                                     const Options& options) {
   validate(recipe);
-  std::vector<std::vector<Reconstruction>> out;
-  out.reserve(problems.size());
-  for (const Problem& p : problems) out.push_back(reconstruct(p, recipe, options));
-  return out;
+  auto solver = make_multi_solver(problems, options);
+  solver->run(recipe);
+  return solver->results();
 }
 
 std::size_t best_index(std::span<const Reconstruction> runs) {

@@ -96,6 +96,105 @@ py::list reconstruct(const Array<float>& intensity, std::optional<Array<std::uin
   return out;
 }
 
+// Several problems in one engine run (qpcu::reconstruct over a span of problems).
+py::list reconstruct_many(const std::vector<Array<float>>& intensities,
+                          std::optional<std::vector<std::optional<Array<std::uint8_t>>>> supports,
+                          std::optional<std::vector<std::optional<Array<std::uint8_t>>>> masks,
+                          const std::vector<OpTuple>& ops, int starts, int batch, std::uint64_t seed,
+                          float support_threshold, const std::string& backend, const std::string& variant,
+                          int sync_every) {
+  const qpcu::Recipe recipe = to_recipe(ops);
+  qpcu::Options options;
+  options.backend = qpcu::parse_backend(backend);
+  options.variant = qpcu::parse_variant(variant);
+  options.starts = starts;
+  options.batch = batch;
+  options.seed = seed;
+  options.sync_every = sync_every;
+  std::vector<qpcu::Shape> shapes;
+  for (const auto& a : intensities) shapes.push_back(shape_of(a));
+  std::vector<std::vector<qpcu::Reconstruction>> runs;
+  {
+    py::gil_scoped_release release;
+    std::vector<qpcu::Problem> problems;
+    for (std::size_t k = 0; k < intensities.size(); ++k) {
+      std::span<const std::uint8_t> sup, msk;
+      if (supports && k < supports->size() && (*supports)[k]) sup = view(*(*supports)[k]);
+      if (masks && k < masks->size() && (*masks)[k]) msk = view(*(*masks)[k]);
+      problems.push_back(qpcu::make_problem(view(intensities[k]), shapes[k], sup, msk, support_threshold));
+    }
+    runs = qpcu::reconstruct(std::span<const qpcu::Problem>(problems), recipe, options);
+  }
+  py::list out;
+  for (std::size_t k = 0; k < runs.size(); ++k) {
+    py::list per;
+    for (auto& run : runs[k]) {
+      const qpcu::Shape history{run.errors.size()};
+      per.append(py::make_tuple(to_numpy(std::move(run.object), shapes[k]), to_numpy(std::move(run.support), shapes[k]),
+                                to_numpy(std::move(run.errors), history)));
+    }
+    out.append(per);
+  }
+  return out;
+}
+
+// The multi-problem state that outlives one run.
+class PyMultiSolver {
+ public:
+  PyMultiSolver(const std::vector<Array<float>>& intensities,
+                std::optional<std::vector<std::optional<Array<std::uint8_t>>>> supports,
+                std::optional<std::vector<std::optional<Array<std::uint8_t>>>> masks, int starts, int batch,
+                std::uint64_t seed, float support_threshold, const std::string& backend, const std::string& variant,
+                int sync_every) {
+    qpcu::Options options;
+    options.backend = qpcu::parse_backend(backend);
+    options.variant = qpcu::parse_variant(variant);
+    options.starts = starts;
+    options.batch = batch;
+    options.seed = seed;
+    options.sync_every = sync_every;
+    for (const auto& a : intensities) shapes_.push_back(shape_of(a));
+    py::gil_scoped_release release;
+    std::vector<qpcu::Problem> problems;
+    for (std::size_t k = 0; k < intensities.size(); ++k) {
+      std::span<const std::uint8_t> sup, msk;
+      if (supports && k < supports->size() && (*supports)[k]) sup = view(*(*supports)[k]);
+      if (masks && k < masks->size() && (*masks)[k]) msk = view(*(*masks)[k]);
+      problems.push_back(qpcu::make_problem(view(intensities[k]), shapes_[k], sup, msk, support_threshold));
+    }
+    solver_ = qpcu::make_multi_solver(problems, options);
+  }
+
+  void run(const std::vector<OpTuple>& ops) {
+    const qpcu::Recipe recipe = to_recipe(ops);
+    py::gil_scoped_release release;
+    solver_->run(recipe);
+  }
+
+  py::list results() const {
+    std::vector<std::vector<qpcu::Reconstruction>> runs;
+    {
+      py::gil_scoped_release release;
+      runs = solver_->results();
+    }
+    py::list out;
+    for (std::size_t k = 0; k < runs.size(); ++k) {
+      py::list per;
+      for (auto& run : runs[k]) {
+        const qpcu::Shape history{run.errors.size()};
+        per.append(py::make_tuple(to_numpy(std::move(run.object), shapes_[k]),
+                                  to_numpy(std::move(run.support), shapes_[k]), to_numpy(std::move(run.errors), history)));
+      }
+      out.append(per);
+    }
+    return out;
+  }
+
+ private:
+  std::vector<qpcu::Shape> shapes_;
+  std::unique_ptr<qpcu::MultiSolver> solver_;
+};
+
 // A solver that outlives one run: the benchmark's construct / warm-up / timed-run protocol.
 class PySolver {
  public:
@@ -158,6 +257,18 @@ PYBIND11_MODULE(_core, m) {
            py::arg("backend"), py::arg("variant"), py::arg("sync_every"))
       .def("run", &PySolver::run, py::arg("ops"))
       .def("results", &PySolver::results);
+  py::class_<PyMultiSolver>(m, "MultiSolver")
+      .def(py::init<const std::vector<Array<float>>&, std::optional<std::vector<std::optional<Array<std::uint8_t>>>>,
+                    std::optional<std::vector<std::optional<Array<std::uint8_t>>>>, int, int, std::uint64_t, float,
+                    const std::string&, const std::string&, int>(),
+           py::arg("intensities"), py::arg("supports").none(true), py::arg("masks").none(true), py::arg("starts"),
+           py::arg("batch"), py::arg("seed"), py::arg("support_threshold"), py::arg("backend"), py::arg("variant"),
+           py::arg("sync_every"))
+      .def("run", &PyMultiSolver::run, py::arg("ops"))
+      .def("results", &PyMultiSolver::results);
+  m.def("reconstruct_many", &reconstruct_many, py::arg("intensities"), py::arg("supports").none(true),
+        py::arg("masks").none(true), py::arg("ops"), py::arg("starts"), py::arg("batch"), py::arg("seed"),
+        py::arg("support_threshold"), py::arg("backend"), py::arg("variant"), py::arg("sync_every"));
   m.def(
       "make_problem",
       [](const Array<float>& intensity, std::optional<Array<std::uint8_t>> support,

@@ -83,7 +83,8 @@ def _problem(shape, m, masked=False):
     return obj, amp, mask, sup
 
 
-@pytest.mark.parametrize("shape", [(64, 64), (48, 40), (32, 32, 32), (256, 256), (16, 8, 32)])
+@pytest.mark.parametrize("shape", [(64, 64), (48, 40), (32, 32, 32), (256, 256), (16, 8, 32), (8, 128, 128),
+                                   (16, 64, 32)])
 @pytest.mark.parametrize("schedule", ["graph", "fused", "unfused"])
 def test_teacher_forced_iteration(qpch, shape, schedule):
     """One iteration from the same start: error within 1e-5 relative, object within 1e-5."""
@@ -130,3 +131,22 @@ def test_agrees_with_oracle_short_recipe(qpch, shape, m, schedule):
         assert np.array_equal(oracle.to_centered(sups[k]).astype(bool), ores[k][1])
     print(f"{shape} {schedule}: worst per-iteration relative error difference {worst:.2e}")
     s.close()
+
+
+@pytest.mark.parametrize("shape", [(64, 64), (256, 256), (32, 32, 32), (8, 128, 128)])
+@pytest.mark.parametrize("storage,tol,floor", [("f16", 3e-3, 5e-4), ("bf16", 2e-2, 4e-3)])
+def test_low_precision_iteration(qpch, shape, storage, tol, floor):
+    """One RAAR and one ER iteration in fp16 / bf16 storage against the FP64 oracle from
+    the same start: errors within the storage precision (fp32 arithmetic throughout); the
+    floor is the storage's unit roundoff, below which an error value is rounding noise."""
+    obj, amp, mask, sup = _problem(shape, max(4, shape[-1] // 4), masked=True)
+    start = np.stack([obj * np.complex64(0.9) + 0.05 * synthetic.random_field(obj.size, 11).reshape(shape)])
+    ops = [R.raar(1), R.er(1)]
+    o = oracle.Solver(amp, mask, sup, starts=1, start=start, prec="d")
+    o.run(ops)
+    want = o.results()[0][2]
+    s = qpch.Solver(oracle.to_fft_order(amp), oracle.to_fft_order(mask), oracle.to_fft_order(sup), starts=1,
+                    start=np.stack([oracle.to_fft_order(start[0])]), storage=storage, mixed=False)
+    got = R.run_abi(s, ops)[:, 0]
+    s.close()
+    assert np.all(np.abs(got - want) <= tol * want + floor), (got, want)

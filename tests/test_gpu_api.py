@@ -165,3 +165,51 @@ def test_device_info(qpc):
     assert info["name"]
     assert info["major"] == 9 and info["minor"] == 50
     assert info["peak_bandwidth_gbs"] > 10.0
+
+
+@pytest.mark.parametrize("variant", ["batched", "mixed"])
+def test_ensemble_equals_separate_runs(qpc, variant):
+    """Problems of two 2D shapes and one 3D shape advanced by one graph launch per chunk
+    give the same runs as each problem alone."""
+    cases = [synthetic.triangle((64, 64), 16), synthetic.triangle((64, 64), 20), synthetic.triangle((32, 32), 8),
+             synthetic.triangle((16, 32, 32), 8)]
+    recipe = qpc.Recipe(qpc.HIO(10), qpc.Shrinkwrap(), qpc.RAAR(10), qpc.detwin(qpc.HIO(5)), qpc.ER(10))
+    many = qpc.reconstruct_many([c[2] for c in cases], supports=[c[1].astype(bool) for c in cases], recipe=recipe,
+                                starts=2, seed=5, backend="hip", variant=variant)
+    assert len(many) == len(cases)
+    for (obj, sup, inten), ens in zip(cases, many):
+        single = qpc.reconstruct(inten, support=sup.astype(bool), recipe=recipe, starts=2, seed=5, backend="hip",
+                                 variant=variant)
+        for a, b in zip(ens, single):
+            np.testing.assert_allclose(a.errors, b.errors, rtol=1e-6)
+            assert rel(a.object, b.object) < 1e-6
+            assert np.array_equal(a.support, b.support)
+
+
+def test_launch_configuration_does_not_change_results(qpc, qpch):
+    """Plane vs line passes, XCD-affine vs plain numbering, and starts per launch give the
+    same iterates and errors."""
+    import recipes as R
+
+    obj, sup, inten = synthetic.triangle((32, 32, 32), 8)
+    amp, mask, sup = oracle.make_problem(inten, sup)
+    f = oracle.to_fft_order
+    ops = R.flat(R.raar(6), R.shrinkwrap(method="rms"), R.er(4))
+    runs = []
+    for plane, xcd, batch in [(1, 0, 0), (2, 0, 0), (1, 2, 0), (1, 0, 4)]:
+        s = qpch.Solver(f(amp), f(mask), f(sup), starts=8, seed=2, plane=plane, xcd=xcd, batch=batch)
+        errs = R.run_abi(s, ops)
+        runs.append((errs, s.read_state(), s.get_support()))
+        s.close()
+    for errs, x, su in runs[1:]:
+        np.testing.assert_allclose(errs, runs[0][0], rtol=1e-6)
+        assert rel(x, runs[0][1]) < 1e-6
+        assert np.array_equal(su, runs[0][2])
+
+
+def test_autotuner_reports_its_choice(qpc, qpch):
+    obj, sup, inten = synthetic.triangle((64, 64), 16)
+    qpc.reconstruct(inten, support=sup.astype(bool), recipe=qpc.Recipe(qpc.ER(2)), starts=16, backend="hip",
+                    variant="mixed")
+    log = qpch.autotune_log()
+    assert "2x64x64/s16" in log and "->" in log

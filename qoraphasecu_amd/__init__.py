@@ -8,6 +8,15 @@ Arrays are centre-origin, ``mask=True`` marks an unusable pixel, ``intensity`` i
 measured intensity.  The native parts are built in-tree by ``qoraphasecu_amd.build``.
 """
 
+import importlib.util as _ilu
+
+# One HIP runtime per process: torch's ROCm wheel carries its own libamdhip64, which its
+# libraries load by path.  If this package's engine brought up the system runtime first,
+# torch would start a second one and find no GPU.  Loading torch first (when it is
+# installed) lets the engine bind to the same runtime, so both can share the device.
+if _ilu.find_spec("torch") is not None:
+    import torch  # noqa: F401
+
 from .api import (
     Ensemble,
     Reconstruction,

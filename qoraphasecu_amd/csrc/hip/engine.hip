@@ -25,6 +25,7 @@
 #include "misc.hpp"
 #include "passes.hpp"
 #include "plane.hpp"
+#include "blur.hpp"
 
 namespace qpch {
 
@@ -265,6 +266,46 @@ void launch_plane(int ny, int nx, const PassArgs& a, int starts, hipStream_t s) 
   throw InvalidArgument("no plane pass for this plane shape");
 }
 
+// --- Shrinkwrap blur passes ---------------------------------------------------------------
+
+template <int N, bool COL, dev::Blur M, class St>
+void launch_blur_t(dev::BlurArgs a, int starts, hipStream_t s) {
+  using G = dev::PassGeom<N, COL, 8>;
+  const int tiles = pass_tiles_t<N, COL, 8>(a.outer, a.inner);
+  a.starts = starts;
+  a.slots = xcd_slots(starts);
+  hipLaunchKernelGGL((dev::blur_pass<N, COL, M, St>), dim3(tiles * (a.slots > 0 ? a.slots : starts)),
+                     dim3(G::kThreads), 0, s, a);
+  QPCH_HIP(hipGetLastError());
+}
+
+template <int N, class St>
+void launch_blur_n(bool col, dev::Blur m, const dev::BlurArgs& a, int starts, hipStream_t s) {
+  if (m == dev::Blur::kMag) {
+    return col ? launch_blur_t<N, true, dev::Blur::kMag, St>(a, starts, s)
+               : launch_blur_t<N, false, dev::Blur::kMag, St>(a, starts, s);
+  }
+  if (m == dev::Blur::kGauss) {
+    return col ? launch_blur_t<N, true, dev::Blur::kGauss, float2>(a, starts, s)
+               : launch_blur_t<N, false, dev::Blur::kGauss, float2>(a, starts, s);
+  }
+  return col ? launch_blur_t<N, true, dev::Blur::kReal, float2>(a, starts, s)
+             : launch_blur_t<N, false, dev::Blur::kReal, float2>(a, starts, s);
+}
+
+template <class St>
+void launch_blur(int N, bool col, dev::Blur m, const dev::BlurArgs& a, int starts, hipStream_t s) {
+  switch (N) {
+#define QPCH_CASE(V) \
+  case V:            \
+    return launch_blur_n<V, St>(col, m, a, starts, s);
+    QPCH_FOR_EACH_N(QPCH_CASE)
+#undef QPCH_CASE
+    default:
+      throw InvalidArgument("no blur pass for axis length " + std::to_string(N));
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 // generic (any length) transforms
 
@@ -427,6 +468,7 @@ struct SolverImpl {
   DevBuf<double> partials, errs, sw_part;
   DevBuf<int> counters;
   DevBuf<float> sw_a, sw_b;
+  DevBuf<float2> sw_c;  // the blur's spectrum scratch
   int err_cap = 0;
   bool low_active = false;
   bool wave_valid = false;
@@ -706,8 +748,89 @@ struct SolverImpl {
     return it->second;
   }
 
+  // exp(-2 pi^2 sigma^2 f^2) for every FFT-order index of an axis of length N
+  const float* transfer_for(int N, float sigma) {
+    const auto key = std::make_tuple(N, sigma, 1.0f);
+    auto it = taps_cache.find(key);
+    if (it == taps_cache.end()) {
+      const double pi = 3.14159265358979323846;
+      const double k = -2.0 * pi * pi * static_cast<double>(sigma) * static_cast<double>(sigma);
+      std::vector<float> g(N);
+      for (int j = 0; j < N; ++j) {
+        const int sj = j < (N + 1) / 2 ? j : j - N;
+        const double f = static_cast<double>(sj) / N;
+        g[j] = static_cast<float>(std::exp(k * f * f));
+      }
+      DevBuf<float> d = upload(g.data(), g.size());
+      it = taps_cache.emplace(key, std::make_pair(0, std::move(d))).first;
+    }
+    return it->second.second.get();
+  }
+
+  template <class St>
+  void shrinkwrap_fft(float threshold, float sigma, int method) {
+    const int nd = dims.nd;
+    const int last = nd - 1;
+    dev::BlurArgs a{};
+    a.n = n;
+    a.x = xp<St>(0);
+    a.w = sw_c.get();
+    a.out = sw_a.get();
+    a.support = supp.get();
+    a.g0 = transfer_for(dims.d[0], sigma);
+    a.g1 = transfer_for(dims.d[1], sigma);
+    a.g2 = nd == 3 ? transfer_for(dims.d[2], sigma) : nullptr;
+    a.d1 = dims.d[1];
+    a.d2 = nd == 3 ? dims.d[2] : 1;
+    a.method = method;
+    a.scale = static_cast<float>(1.0 / static_cast<double>(n));
+    a.partials = sw_part.get();
+    auto geometry = [&](int axis) {
+      a.outer = static_cast<int>(dims.outer(axis));
+      a.inner = static_cast<int>(dims.inner(axis));
+    };
+    auto line = [&](int axis, Mode mode) {
+      PassArgs p{};
+      p.outer = static_cast<int>(dims.outer(axis));
+      p.inner = static_cast<int>(dims.inner(axis));
+      p.n = n;
+      p.w = sw_c.get();
+      p.out_scale = 1.0f;
+      launch_pass<float2>(dims.d[axis], p.inner > 1, mode, p, starts, stream);
+    };
+    geometry(last);
+    launch_blur<St>(dims.d[last], false, dev::Blur::kMag, a, starts, stream);
+    if (nd == 3) line(1, Mode::kFwd);
+    geometry(0);
+    launch_blur<St>(dims.d[0], true, dev::Blur::kGauss, a, starts, stream);
+    if (nd == 3) line(1, Mode::kInv);
+    geometry(last);
+    launch_blur<St>(dims.d[last], false, dev::Blur::kReal, a, starts, stream);
+    const int tiles = pass_tiles<8>(dims.d[last], false, static_cast<int>(dims.outer(last)), 1);
+    hipLaunchKernelGGL(dev::shrinkwrap_threshold, dim3(grid_for(n, 256, 2048), starts), dim3(256), 0, stream,
+                       sw_a.get(), sw_part.get(), tiles, n, method, threshold, supp.get());
+    QPCH_HIP(hipGetLastError());
+  }
+
   void shrinkwrap(float threshold, float sigma, int method) {
     if (!(threshold > 0.0f) || !(sigma >= 0.0f)) throw InvalidArgument("Shrinkwrap needs threshold > 0, sigma >= 0");
+    if (fused) {
+      const long long total = n * starts;
+      if (sw_a.size() < static_cast<size_t>(total)) sw_a.resize(total);
+      if (sw_c.size() < static_cast<size_t>(total)) sw_c.resize(total);
+      const int last = dims.nd - 1;
+      const size_t need = static_cast<size_t>(pass_tiles<8>(dims.d[last], false, static_cast<int>(dims.outer(last)), 1)) *
+                          starts * 2;
+      if (sw_part.size() < need) sw_part.resize(need);
+      if (!low_active) {
+        shrinkwrap_fft<float2>(threshold, sigma, method);
+      } else if (storage == QPCH_F16) {
+        shrinkwrap_fft<__half2>(threshold, sigma, method);
+      } else {
+        shrinkwrap_fft<__hip_bfloat162>(threshold, sigma, method);
+      }
+      return;
+    }
     const long long total = n * starts;
     if (sw_a.size() < static_cast<size_t>(total)) {
       sw_a.resize(total);

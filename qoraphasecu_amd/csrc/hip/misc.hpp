@@ -153,35 +153,48 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// support = blurred > threshold * level, the level reduced from the partials in a fixed
-// order by every block.
+// support = blurred > threshold * level.  Every block reduces its start's partials to the
+// level itself, in a fixed order (lane-strided sums, then a fixed tree), so all blocks of a
+// start agree on it bit for bit.
 __global__ void __launch_bounds__(256)
     shrinkwrap_threshold(const float* blurred, const double* partials, int blocks, long long n,
                          int method, float threshold, std::uint8_t* support) {
+  __shared__ double red[16];
+  __shared__ float redf[16];
   __shared__ float cut;
-  if (threadIdx.x == 0) {
-    const double* part = partials + static_cast<long long>(blockIdx.y) * blocks * 2;
-    float level;
+  const double* part = partials + static_cast<long long>(blockIdx.y) * blocks * 2;
+  double s = 0.0, c = 0.0;
+  float m = 0.0f;
+  for (int k = threadIdx.x; k < blocks; k += blockDim.x) {
     if (method == 0) {
-      float m = 0.0f;
-      for (int k = 0; k < blocks; ++k) m = fmaxf(m, static_cast<float>(part[2 * k]));
-      level = m;
+      m = fmaxf(m, static_cast<float>(part[2 * k]));
     } else {
-      double s = 0.0, c = 0.0;
-      for (int k = 0; k < blocks; ++k) {
-        s += part[2 * k];
-        c += part[2 * k + 1];
-      }
-      level = static_cast<float>(sqrt(s / fmax(c, 1.0)));
+      s += part[2 * k];
+      c += part[2 * k + 1];
     }
-    cut = threshold * level;
+  }
+  if (method == 0) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_down(m, o, 64));
+    if ((threadIdx.x & 63) == 0) redf[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float level = redf[0];
+      for (int k = 1; k < static_cast<int>((blockDim.x + 63) >> 6); ++k) level = fmaxf(level, redf[k]);
+      cut = threshold * level;
+    }
+  } else {
+    const double ts = block_sum(s, red);
+    __syncthreads();
+    const double tc = block_sum(c, red);
+    if (threadIdx.x == 0) cut = threshold * static_cast<float>(sqrt(ts / fmax(tc, 1.0)));
   }
   __syncthreads();
   const long long soff = static_cast<long long>(blockIdx.y) * n;
-  const float c = cut;
+  const float cv = cut;
   for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < n;
        e += static_cast<long long>(gridDim.x) * blockDim.x) {
-    support[soff + e] = blurred[soff + e] > c;
+    support[soff + e] = blurred[soff + e] > cv;
   }
 }
 

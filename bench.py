@@ -177,16 +177,19 @@ def pass_roofline(w, storage, reps=50):
     amp, mask, sup = _core.make_problem(w["intensity"], None, w["mask"], 0.1)
     f = np.fft.ifftshift
     s = _qpch.Solver(f(amp).copy(), f(mask).copy(), f(sup).copy(), starts=w["starts"], batch=0,
-                     storage=storage, mixed=storage != "f32", schedule="graph")
+                     storage=storage, mixed=storage != "f32", schedule="graph", lanes=0, tuned=True)
     low = storage != "f32"
     passes = s.time_passes(low=low, reps=reps)
+    tuning = s.tuning
     s.close()
     n = int(np.prod(w["shape"]))
     key = "f16" if low else "f32"
     out = []
+    per_launch = tuning["batch"] if tuning and tuning["batch"] > 0 else w["starts"]
     for axis, mode, ms in passes:
-        b = PASS_BYTES[(mode, key)] * n * w["starts"]
-        out.append(dict(axis=axis, mode=mode, ms=ms, algorithmic_bytes=b, achieved_gbs=b / (ms * 1e-3) / 1e9))
+        b = PASS_BYTES[(mode, key)] * n * per_launch
+        out.append(dict(axis=axis, mode=mode, ms=ms, algorithmic_bytes=b, achieved_gbs=b / (ms * 1e-3) / 1e9,
+                        launch=tuning))
     return out
 
 

@@ -71,6 +71,10 @@ typedef struct {
   int plane;           /* 3D: 0 = automatic, 1 = xy-plane update pass, 2 = line passes only */
   int xcd;             /* 0 = XCD-affine block numbering when starts per launch are a multiple of 8,
                           2 = plain numbering */
+  int lanes;           /* launch groups in flight at once, each on its own stream (0 or 1 = one
+                          after another; at most 8) */
+  int radix;           /* FFT radix of the line passes: 0 or 16 = radix 16, 8 = radix 8 for axes
+                          of 64 to 2048 (other axes stay radix 16) */
 } qpch_problem;
 
 /* Launch configuration chosen by qpch_autotune. */
@@ -79,6 +83,8 @@ typedef struct {
   int plane;  /* as qpch_problem.plane */
   int xcd;    /* as qpch_problem.xcd */
   double us_per_iteration;
+  int lanes;  /* as qpch_problem.lanes */
+  int radix;  /* as qpch_problem.radix */
 } qpch_tuning;
 
 typedef struct {
@@ -144,7 +150,8 @@ int qpch_solver_pass_info(qpch_solver* solver, int low, int* axes, int* modes, i
  * storage; qpch_autotune_log() returns one line per tuned key (also printed to stderr
  * when QPCH_TUNE_LOG=1).  No reference counterpart: the reference leaves Options::batch
  * to the user (include/qoraphasecu/solver.hpp:44-46).  A problem->batch > 0 is kept and
- * only the other choices are tuned. */
+ * only the other choices are tuned; problem->lanes == 1 keeps launch groups sequential and a
+ * problem->radix != 0 is kept. */
 int qpch_autotune(const qpch_problem* problem, qpch_tuning* out);
 const char* qpch_autotune_log(void);
 

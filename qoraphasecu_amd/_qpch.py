@@ -40,6 +40,8 @@ class QpchProblem(ctypes.Structure):
         ("problems", ctypes.c_int),
         ("plane", ctypes.c_int),
         ("xcd", ctypes.c_int),
+        ("lanes", ctypes.c_int),
+        ("radix", ctypes.c_int),
     ]
 
 
@@ -49,6 +51,8 @@ class QpchTuning(ctypes.Structure):
         ("plane", ctypes.c_int),
         ("xcd", ctypes.c_int),
         ("us_per_iteration", ctypes.c_double),
+        ("lanes", ctypes.c_int),
+        ("radix", ctypes.c_int),
     ]
 
 
@@ -207,14 +211,16 @@ class Solver:
     """A device solver over FFT-order inputs (what the C++ HipSolver drives)."""
 
     def __init__(self, amplitude, mask, support, starts=1, batch=0, seed=0, start=None,
-                 storage="f32", mixed=False, schedule="graph", problems=1, plane=0, xcd=0):
+                 storage="f32", mixed=False, schedule="graph", problems=1, plane=0, xcd=0, lanes=1, radix=0, tuned=False):
         """amplitude / mask / support: one problem's FFT-order arrays, or `problems` of them
-        stacked along a leading axis; `starts` per problem."""
+        stacked along a leading axis; `starts` per problem.  tuned: take the launch choices
+        the autotuner makes for this problem (qpch_autotune) instead of the arguments."""
         shape = tuple(amplitude.shape)
         self.problems = problems
         self.shape = shape[1:] if problems > 1 else shape
         self.n = int(np.prod(self.shape))
         self.starts = starts * problems
+        self.tuning = None
         self._keep = [
             np.ascontiguousarray(amplitude, dtype=np.float32),
             np.ascontiguousarray(mask, dtype=np.uint8),
@@ -228,6 +234,8 @@ class Solver:
         p.problems = problems
         p.plane = plane
         p.xcd = xcd
+        p.lanes = lanes
+        p.radix = radix
         p.amplitude = _ptr(self._keep[0])
         p.mask = _ptr(self._keep[1])
         p.support = _ptr(self._keep[2])
@@ -238,6 +246,12 @@ class Solver:
         p.storage = STORAGE[storage]
         p.mixed = int(mixed)
         p.schedule = SCHEDULE[schedule]
+        if tuned:
+            t = QpchTuning()
+            check(lib().qpch_autotune(ctypes.byref(p), ctypes.byref(t)))
+            p.batch, p.plane, p.xcd, p.lanes, p.radix = t.batch, t.plane, t.xcd, t.lanes, t.radix
+            self.tuning = dict(batch=t.batch, plane=t.plane, xcd=t.xcd, lanes=t.lanes, radix=t.radix,
+                               us_per_iteration=t.us_per_iteration)
         h = ctypes.c_void_p()
         check(lib().qpch_solver_create(ctypes.byref(p), ctypes.byref(h)))
         self._h = h

@@ -87,6 +87,8 @@ __global__ void __launch_bounds__((PassGeom<N, COL, 8>::kThreads))
   }
   const long long es = COL ? p.inner : 1;
   const long long soff = static_cast<long long>(start) * p.n;
+  // unconditional loads (all in flight at once); lanes past a ragged tile read line 0
+  const long long lb = soff + (valid ? base : 0);
   float2* line = lds + b * G::kStride;
   float2 v[kR];
 
@@ -94,7 +96,7 @@ __global__ void __launch_bounds__((PassGeom<N, COL, 8>::kThreads))
     const St* x = static_cast<const St*>(p.x);
 #pragma unroll
     for (int m = 0; m < kR; ++m) {
-      float2 e = valid ? Storage<St>::ld(x, soff + base + (i + m * kT) * es) : float2{0.0f, 0.0f};
+      float2 e = Storage<St>::ld(x, lb + (i + m * kT) * es);
       v[m] = {sqrtf(e.x * e.x + e.y * e.y), 0.0f};
     }
     load_twiddles<N>(tw);
@@ -106,12 +108,13 @@ __global__ void __launch_bounds__((PassGeom<N, COL, 8>::kThreads))
     }
   } else if constexpr (M == Blur::kGauss) {
 #pragma unroll
-    for (int m = 0; m < kR; ++m) v[m] = valid ? p.w[soff + base + (i + m * kT) * es] : float2{0.0f, 0.0f};
+    for (int m = 0; m < kR; ++m) v[m] = p.w[lb + (i + m * kT) * es];
     load_twiddles<N>(tw);
     __syncthreads();
     fft_line<N, 16, true>(v, i, line, tw);
     // column c of axis 0 is (axis-1 index, axis-2 index) = (c / d2, c % d2)
-    const float gc = valid ? p.g1[c / p.d2] * (p.g2 ? p.g2[c % p.d2] : 1.0f) : 0.0f;
+    const int cc = valid ? c : 0;
+    const float gc = p.g1[cc / p.d2] * (p.g2 ? p.g2[cc % p.d2] : 1.0f);
 #pragma unroll
     for (int m = 0; m < kR; ++m) {
       const float g = gc * p.g0[i + m * kT];
@@ -124,7 +127,12 @@ __global__ void __launch_bounds__((PassGeom<N, COL, 8>::kThreads))
     }
   } else {  // kReal
 #pragma unroll
-    for (int m = 0; m < kR; ++m) v[m] = valid ? p.w[soff + base + (i + m * kT) * es] : float2{0.0f, 0.0f};
+    for (int m = 0; m < kR; ++m) v[m] = p.w[lb + (i + m * kT) * es];
+    bool inside[kR];
+    if (p.method != 0) {
+#pragma unroll
+      for (int m = 0; m < kR; ++m) inside[m] = p.support[lb + (i + m * kT) * es] != 0;
+    }
     load_twiddles<N>(tw);
     __syncthreads();
     fft_line<N, 16, false>(v, i, line, tw);
@@ -133,12 +141,12 @@ __global__ void __launch_bounds__((PassGeom<N, COL, 8>::kThreads))
     if (valid) {
 #pragma unroll
       for (int m = 0; m < kR; ++m) {
-        const long long at = soff + base + (i + m * kT) * es;
+        const long long at = lb + (i + m * kT) * es;
         const float r = v[m].x * p.scale;
         p.out[at] = r;
         if (p.method == 0) {
           peak = fmaxf(peak, r);
-        } else if (p.support[at]) {
+        } else if (inside[m]) {
           sum += static_cast<double>(r) * static_cast<double>(r);
           count += 1.0;
         }

@@ -161,10 +161,15 @@ int grid_for(long long work, int threads = 256, int cap = 2048) {
 // ---------------------------------------------------------------------------------------
 // fused line-pass dispatch
 
+// Radix-8 kernels exist for these lengths (shorter lines already fit one or two radix-16
+// stages; longer ones would need a fourth LDS exchange).
+constexpr bool has_radix8(int N) { return N >= 64 && N <= 2048; }
+int radix_for(int N, int radix) { return radix == 8 && has_radix8(N) ? 8 : 16; }
+
 // Blocks per start of a pass.
-template <int N, bool COL, int EB>
+template <int N, bool COL, int EB, int R = 16>
 int pass_tiles_t(int outer, int inner) {
-  using G = dev::PassGeom<N, COL, EB>;
+  using G = dev::PassGeom<N, COL, EB, R>;
   return COL ? outer * ((inner + G::kB - 1) / G::kB) : (outer + G::kB - 1) / G::kB;
 }
 
@@ -172,15 +177,23 @@ int pass_tiles_t(int outer, int inner) {
 // evenly over the eight XCDs; a single start keeps its blocks spread over the chip.
 int xcd_slots(int starts) { return (starts >= 8 && starts % 8 == 0) ? starts : 0; }
 
-template <int N, bool COL, Mode M, class St>
-void launch_pass_t(PassArgs a, int starts, hipStream_t s) {
-  using G = dev::PassGeom<N, COL, sizeof(St)>;
-  const int tiles = pass_tiles_t<N, COL, sizeof(St)>(a.outer, a.inner);
+template <int N, bool COL, Mode M, class St, int R>
+void launch_pass_r(PassArgs a, int starts, hipStream_t s) {
+  using G = dev::PassGeom<N, COL, sizeof(St), R>;
+  const int tiles = pass_tiles_t<N, COL, sizeof(St), R>(a.outer, a.inner);
   a.starts = starts;
   a.slots = a.no_xcd ? 0 : xcd_slots(starts);
   const int gx = tiles * (a.slots > 0 ? a.slots : starts);
-  hipLaunchKernelGGL((dev::line_pass<N, COL, M, St>), dim3(gx), dim3(G::kThreads), 0, s, a);
+  hipLaunchKernelGGL((dev::line_pass<N, COL, M, St, R>), dim3(gx), dim3(G::kThreads), 0, s, a);
   QPCH_HIP(hipGetLastError());
+}
+
+template <int N, bool COL, Mode M, class St>
+void launch_pass_t(PassArgs a, int starts, hipStream_t s) {
+  if constexpr (has_radix8(N) && M != Mode::kGrad) {
+    if (a.radix == 8) return launch_pass_r<N, COL, M, St, 8>(a, starts, s);
+  }
+  launch_pass_r<N, COL, M, St, 16>(a, starts, s);
 }
 
 template <int N, class St>
@@ -221,10 +234,12 @@ void launch_pass(int N, bool col, Mode m, const PassArgs& a, int starts, hipStre
 }
 
 template <int EB>
-int pass_tiles(int N, bool col, int outer, int inner) {
+int pass_tiles(int N, bool col, int outer, int inner, int radix = 16) {
+  const bool r8 = radix_for(N, radix) == 8;
   switch (N) {
-#define QPCH_CASE(V) \
-  case V:            \
+#define QPCH_CASE(V)                                                                              \
+  case V:                                                                                         \
+    if (r8) return col ? pass_tiles_t<V, true, EB, 8>(outer, inner) : pass_tiles_t<V, false, EB, 8>(outer, inner); \
     return col ? pass_tiles_t<V, true, EB>(outer, inner) : pass_tiles_t<V, false, EB>(outer, inner);
     QPCH_FOR_EACH_N(QPCH_CASE)
 #undef QPCH_CASE
@@ -455,6 +470,11 @@ struct SolverImpl {
   int schedule = QPCH_GRAPH;
   bool fused = true;
   hipStream_t stream = nullptr;
+  hipStream_t on = nullptr;   // where run_pass and chunk launch: the solver's stream or a lane
+  int lanes = 1;              // launch groups in flight at once (each on its own stream)
+  std::vector<hipStream_t> lane_streams;
+  std::vector<hipEvent_t> lane_done;
+  hipEvent_t lane_fork = nullptr;
   float amp_scale = 1.0f;  // problem 0's
   double energy = 0.0;     // problem 0's
   int problems = 1, per_problem = 1;
@@ -478,6 +498,9 @@ struct SolverImpl {
 
   ~SolverImpl() {
     clear_graphs();
+    for (hipStream_t l : lane_streams) (void)hipStreamDestroy(l);
+    for (hipEvent_t e : lane_done) (void)hipEventDestroy(e);
+    if (lane_fork) (void)hipEventDestroy(lane_fork);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -509,6 +532,7 @@ struct SolverImpl {
   };
   bool plane3d = false;  // 3D iterations use the plane pass
   bool xcd_off = false;  // keep the plain block numbering even for multiples of 8 starts
+  int radix = 16;        // FFT radix of the line passes (8 or 16)
 
   std::vector<PassSpec> iteration_passes(bool lowp) const {
     const int nd = dims.nd;
@@ -552,9 +576,9 @@ struct SolverImpl {
     p.amp = amp_signed.get();
     p.amp_stride = problems > 1 ? n : 0;
     p.amp_group = per_problem;
-    p.partials = partials.get();
     p.partial_stride = pass_tiles<sizeof(St)>(dims.d[0], dims.inner(0) > 1, static_cast<int>(dims.outer(0)),
-                                              static_cast<int>(dims.inner(0)));
+                                              static_cast<int>(dims.inner(0)), radix);
+    p.partials = partials.get() + static_cast<long long>(first) * p.partial_stride;  // own rows per group
     p.errs = errs.get();
     p.err_stride = starts;
     p.err_offset = first;
@@ -563,14 +587,15 @@ struct SolverImpl {
     p.a = a;
     p.b = b;
     p.no_xcd = xcd_off ? 1 : 0;
+    p.radix = radix;
     p.in_scale = ps.in_scale;
     p.out_scale = ps.out_scale;
     if (ps.plane) {
       p.outer = dims.d[0];  // one tile per plane
-      launch_plane<St>(dims.d[1], dims.d[2], p, count, stream);
+      launch_plane<St>(dims.d[1], dims.d[2], p, count, on ? on : stream);
       return;
     }
-    launch_pass<St>(dims.d[ps.axis], inner > 1, ps.mode, p, count, stream);
+    launch_pass<St>(dims.d[ps.axis], inner > 1, ps.mode, p, count, on ? on : stream);
   }
 
   template <class St>
@@ -653,24 +678,24 @@ struct SolverImpl {
     const auto key = std::make_tuple(g, len, a, b, !std::is_same_v<St, float2>);
     auto it = graphs.find(key);
     if (it == graphs.end()) {
-      QPCH_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      QPCH_HIP(hipStreamBeginCapture(on, hipStreamCaptureModeThreadLocal));
       try {
         fused_iterations<St>(g, len, a, b);
       } catch (...) {
         hipGraph_t dummy{};
-        (void)hipStreamEndCapture(stream, &dummy);
+        (void)hipStreamEndCapture(on, &dummy);
         if (dummy) (void)hipGraphDestroy(dummy);
         throw;
       }
       hipGraph_t graph{};
-      QPCH_HIP(hipStreamEndCapture(stream, &graph));
+      QPCH_HIP(hipStreamEndCapture(on, &graph));
       hipGraphExec_t exec{};
       const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
       QPCH_HIP(e);
       it = graphs.emplace(key, exec).first;
     }
-    QPCH_HIP(hipGraphLaunch(it->second, stream));
+    QPCH_HIP(hipGraphLaunch(it->second, on));
   }
 
   // --- group support: the precision switch and prologue run eagerly; the chunk's passes
@@ -705,17 +730,31 @@ struct SolverImpl {
     const bool lowp = low() && !(mixed && refine);
     set_precision(lowp);
     const bool need_prologue = !wave_valid;
+    const int L = fused ? std::min<int>(static_cast<int>(lane_streams.size()), groups) : 0;
+    if (L > 1) {
+      // launch groups run side by side on the lanes, forked from and joined to the stream
+      QPCH_HIP(hipEventRecord(lane_fork, stream));
+      for (int l = 0; l < L; ++l) QPCH_HIP(hipStreamWaitEvent(lane_streams[l], lane_fork, 0));
+    }
     for (int g = 0; g < groups; ++g) {
       if (!fused) {
         unfused_iterations(g, len, a, b);
         continue;
       }
+      on = L > 1 ? lane_streams[g % L] : stream;
       if (!lowp) {
         chunk<float2>(g, len, a, b, need_prologue);
       } else if (storage == QPCH_F16) {
         chunk<__half2>(g, len, a, b, need_prologue);
       } else {
         chunk<__hip_bfloat162>(g, len, a, b, need_prologue);
+      }
+    }
+    on = stream;
+    if (L > 1) {
+      for (int l = 0; l < L; ++l) {
+        QPCH_HIP(hipEventRecord(lane_done[l], lane_streams[l]));
+        QPCH_HIP(hipStreamWaitEvent(stream, lane_done[l], 0));
       }
     }
     wave_valid = fused;
@@ -892,6 +931,8 @@ struct SolverImpl {
     fused = fusable(dims) && schedule != QPCH_UNFUSED;
     plane3d = fused && dims.nd == 3 && plane_supported(dims.d[1], dims.d[2]) && p.plane != 2;
     xcd_off = p.xcd == 2;
+    if (p.radix != 0 && p.radix != 8 && p.radix != 16) throw InvalidArgument("radix must be 0, 8 or 16");
+    radix = p.radix == 8 ? 8 : 16;
     if (storage != QPCH_F32 && !fusable(dims)) {
       throw InvalidArgument("fp16/bf16 storage needs power-of-two extents between 2 and 4096");
     }
@@ -904,6 +945,19 @@ struct SolverImpl {
     }
     if (!p.amplitude || !p.mask || !p.support) throw InvalidArgument("amplitude, mask and support are required");
     QPCH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    on = stream;
+    lanes = std::max(1, std::min(p.lanes, 8));
+    if (fused && lanes > 1 && groups > 1) {
+      for (int l = 0; l < std::min(lanes, groups); ++l) {
+        hipStream_t st{};
+        QPCH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        lane_streams.push_back(st);
+        hipEvent_t e{};
+        QPCH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        lane_done.push_back(e);
+      }
+      QPCH_HIP(hipEventCreateWithFlags(&lane_fork, hipEventDisableTiming));
+    }
 
     const long long np = n * problems;
     std::vector<float> amp(p.amplitude, p.amplitude + np);
@@ -948,9 +1002,9 @@ struct SolverImpl {
     int pb = grid_for(n, 256, 1024);
     if (fused) {
       const int o = static_cast<int>(dims.outer(0)), in = static_cast<int>(dims.inner(0));
-      pb = std::max({pb, pass_tiles<8>(dims.d[0], in > 1, o, in), pass_tiles<4>(dims.d[0], in > 1, o, in)});
+      pb = std::max({pb, pass_tiles<8>(dims.d[0], in > 1, o, in, radix), pass_tiles<4>(dims.d[0], in > 1, o, in, radix)});
     }
-    partials.resize(static_cast<size_t>(pb) * batch);
+    partials.resize(static_cast<size_t>(pb) * starts);  // groups may run at once: rows of their own
 
     // starting objects
     if (p.start) {
@@ -1389,6 +1443,8 @@ int qpch_gaussian_blur(const float* host_in, float* host_out, int ndim, const si
 
 // --- autotuner -----------------------------------------------------------------------
 
+}  // extern "C"
+
 namespace {
 std::mutex g_tune_mu;
 std::map<std::string, qpch_tuning> g_tune_cache;
@@ -1398,10 +1454,13 @@ std::string tune_key(const qpch_problem& p) {
   std::string k = std::to_string(p.ndim);
   for (int a = 0; a < p.ndim; ++a) k += "x" + std::to_string(p.shape[a]);
   k += "/s" + std::to_string(p.starts * (p.problems > 1 ? p.problems : 1)) + "/st" + std::to_string(p.storage) +
-       "/m" + std::to_string(p.mixed) + "/b" + std::to_string(p.batch);
+       "/m" + std::to_string(p.mixed) + "/b" + std::to_string(p.batch) + "/l" + std::to_string(p.lanes) + "/r" +
+       std::to_string(p.radix);
   return k;
 }
 }  // namespace
+
+extern "C" {
 
 const char* qpch_autotune_log(void) { return g_tune_log.c_str(); }
 
@@ -1430,45 +1489,67 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
     }
     std::vector<int> planes{0};
     if (d.nd == 3 && qpch::plane_supported(d.d[1], d.d[2])) planes = {1, 2};
+    bool r8 = false;
+    for (int a = 0; a < d.nd; ++a) r8 = r8 || qpch::radix_for(d.d[a], 8) == 8;
+    std::vector<int> radices{16};
+    if (r8 && problem->radix == 0) radices.push_back(8);
+    if (problem->radix != 0) radices = {problem->radix};
     std::string line = "qpch autotune " + key + ":";
     qpch_tuning best{};
     best.us_per_iteration = 1e300;
     const bool lowp = problem->storage != QPCH_F32;
+    auto trial = [&](const qpch_tuning& t) {
+      qpch_problem q = *problem;
+      q.batch = t.batch;
+      q.lanes = t.lanes;
+      q.plane = t.plane;
+      q.xcd = t.xcd;
+      q.radix = t.radix;
+      q.start = nullptr;
+      q.schedule = QPCH_GRAPH;
+      auto s = std::make_unique<qpch_solver>();
+      s->impl.init(q);
+      auto& m = s->impl;
+      m.begin_run(32);
+      m.iterate(4, 0.9f, -0.8f, !lowp);  // capture and warm up
+      hipEvent_t e0{}, e1{};
+      QPCH_HIP(hipEventCreate(&e0));
+      QPCH_HIP(hipEventCreate(&e1));
+      QPCH_HIP(hipEventRecord(e0, m.stream));
+      for (int r = 0; r < 4; ++r) m.iterate(4, 0.9f, -0.8f, !lowp);
+      QPCH_HIP(hipEventRecord(e1, m.stream));
+      QPCH_HIP(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      QPCH_HIP(hipEventElapsedTime(&ms, e0, e1));
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      qpch_tuning r = t;
+      r.us_per_iteration = ms * 1e3 / 16.0;
+      line += " [batch " + std::to_string(t.batch) + (t.lanes > 1 ? " x" + std::to_string(t.lanes) + " lanes" : "") +
+              (t.plane ? (t.plane == 1 ? " plane" : " lines") : "") + (t.xcd == 2 ? " xcd-off" : "") +
+              (t.radix == 8 ? " radix-8" : "") + ": " + std::to_string(r.us_per_iteration) + " us/it]";
+      if (r.us_per_iteration < best.us_per_iteration) best = r;
+    };
+    // first the split of the starts into launches (default kernels), then the kernel
+    // choices for the best split
     for (int b : batches) {
-      for (int pl : planes) {
-        for (int xcd : {0, 2}) {
-          if (xcd == 2 && !(b >= 8 && b % 8 == 0)) continue;  // numbering only differs then
-          qpch_problem q = *problem;
-          q.batch = b;
-          q.plane = pl;
-          q.xcd = xcd;
-          q.start = nullptr;
-          q.schedule = QPCH_GRAPH;
-          auto s = std::make_unique<qpch_solver>();
-          s->impl.init(q);
-          auto& m = s->impl;
-          m.begin_run(32);
-          m.iterate(4, 0.9f, -0.8f, !lowp);  // capture and warm up
-          hipEvent_t e0{}, e1{};
-          QPCH_HIP(hipEventCreate(&e0));
-          QPCH_HIP(hipEventCreate(&e1));
-          QPCH_HIP(hipEventRecord(e0, m.stream));
-          for (int r = 0; r < 4; ++r) m.iterate(4, 0.9f, -0.8f, !lowp);
-          QPCH_HIP(hipEventRecord(e1, m.stream));
-          QPCH_HIP(hipEventSynchronize(e1));
-          float ms = 0.0f;
-          QPCH_HIP(hipEventElapsedTime(&ms, e0, e1));
-          (void)hipEventDestroy(e0);
-          (void)hipEventDestroy(e1);
-          const double us = ms * 1e3 / 16.0;
-          line += " [batch " + std::to_string(b) + (pl ? (pl == 1 ? " plane" : " lines") : "") +
-                  (xcd == 2 ? " xcd-off" : "") + ": " + std::to_string(us) + " us/it]";
-          if (us < best.us_per_iteration) best = qpch_tuning{b, pl, xcd, us};
+      std::vector<int> lane_options{1};
+      if (problem->lanes != 1 && total / b > 1) lane_options.push_back(std::min(4, total / b));
+      for (int ln : lane_options) trial(qpch_tuning{b, planes[0], 0, 0.0, ln, radices[0]});
+    }
+    const qpch_tuning split = best;
+    for (int pl : planes) {
+      for (int xcd : {0, 2}) {
+        if (xcd == 2 && !(split.batch >= 8 && split.batch % 8 == 0)) continue;  // numbering only differs then
+        for (int rx : radices) {
+          if (pl == split.plane && xcd == split.xcd && rx == split.radix) continue;
+          trial(qpch_tuning{split.batch, pl, xcd, 0.0, split.lanes, rx});
         }
       }
     }
-    line += " -> batch " + std::to_string(best.batch) + " plane " + std::to_string(best.plane) + " xcd " +
-            std::to_string(best.xcd) + "\n";
+    line += " -> batch " + std::to_string(best.batch) + " lanes " + std::to_string(best.lanes) + " plane " +
+            std::to_string(best.plane) + " xcd " + std::to_string(best.xcd) + " radix " + std::to_string(best.radix) +
+            "\n";
     std::lock_guard<std::mutex> lock(g_tune_mu);
     g_tune_cache[key] = best;
     g_tune_log += line;

@@ -56,6 +56,7 @@ struct PassArgs {
   int starts;                   // starts in this launch
   int slots;                    // > 0: XCD-affine grid, block = tile * slots + start
   int no_xcd;                   // launcher input: 1 keeps the plain block numbering
+  int radix;                    // launcher input: 8 selects the radix-8 kernels where built
   float a, b;                   // update coefficients
   float in_scale;               // applied after the first transform of the pass
   float out_scale;              // applied before the store (kModulus: after projection)
@@ -63,10 +64,11 @@ struct PassArgs {
 
 // Work-group geometry of a pass, shared by the kernel and its launcher.  EB is the
 // storage size of one complex element: column passes take B adjacent columns so that one
-// load instruction of a row covers 128 contiguous bytes.
-template <int N, bool COL, int EB>
+// load instruction of a row covers 128 contiguous bytes.  R is the FFT radix: 16 holds 16
+// elements per thread (fewer LDS exchanges), 8 holds 8 (twice the waves in flight).
+template <int N, bool COL, int EB, int R = 16>
 struct PassGeom {
-  using S = Shape1<N, 16>;
+  using S = Shape1<N, R>;
   static constexpr int kT = S::kT;
   static constexpr int kR = S::kR;
   static constexpr int kLdsLimit = 96 * 1024 / 8 - N;  // float2 slots for the line tiles
@@ -130,10 +132,10 @@ __device__ __forceinline__ float2 project(float2 w, float a, double& acc) {
 // tile L / slots, so every block of a start lands on the same XCD (blocks are dealt to the
 // eight XCDs round-robin) and the start's arrays stay in that XCD's L2 from one pass to the
 // next; otherwise block L handles tile L % tiles of start L / tiles.
-template <int N, bool COL, Mode M, class St>
-__global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St)>::kThreads))
+template <int N, bool COL, Mode M, class St, int R>
+__global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St), R>::kThreads))
     line_pass(PassArgs p) {
-  using G = PassGeom<N, COL, sizeof(St)>;
+  using G = PassGeom<N, COL, sizeof(St), R>;
   using Io = Storage<St>;
   constexpr int kT = G::kT;
   constexpr int kR = G::kR;
@@ -177,6 +179,10 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St)>::kThreads))
   }
   const long long es = COL ? p.inner : 1;
   const long long soff = static_cast<long long>(start) * p.n;
+  // Loads are unconditional so that all of a thread's loads are in flight at once (a load
+  // under `valid ?` becomes a branch with its own wait): a lane past the last line of a
+  // ragged tile reads line 0 of its start instead and its results are never stored.
+  const long long lb = soff + (valid ? base : 0);
   float2* line = lds + b * G::kStride;
 
   St* w = static_cast<St*>(p.w);
@@ -186,11 +192,11 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St)>::kThreads))
   if constexpr (M == Mode::kFwd || M == Mode::kInv) {
 #pragma unroll
     for (int m = 0; m < kR; ++m) {
-      v[m] = valid ? Io::ld(src, soff + base + (i + m * kT) * es) : float2{0.0f, 0.0f};
+      v[m] = Io::ld(src, lb + (i + m * kT) * es);
     }
     load_twiddles<N>(tw);  // while the loads are in flight
     __syncthreads();
-    fft_line<N, 16, M == Mode::kFwd>(v, i, line, tw);
+    fft_line<N, R, M == Mode::kFwd>(v, i, line, tw);
     if (valid) {
 #pragma unroll
       for (int m = 0; m < kR; ++m) {
@@ -200,16 +206,15 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St)>::kThreads))
     }
   } else if constexpr (M == Mode::kModulus) {
 #pragma unroll
-    for (int m = 0; m < kR; ++m) {
-      v[m] = valid ? Io::ld(w, soff + base + (i + m * kT) * es) : float2{0.0f, 0.0f};
-    }
+    for (int m = 0; m < kR; ++m) v[m] = Io::ld(w, lb + (i + m * kT) * es);
     float amp[kR];
     const long long arow = p.amp_stride * ((p.err_offset + start) / (p.amp_group > 1 ? p.amp_group : 1));
+    const long long ab = arow + (valid ? base : 0);
 #pragma unroll
-    for (int m = 0; m < kR; ++m) amp[m] = valid ? p.amp[arow + base + (i + m * kT) * es] : -1.0f;
+    for (int m = 0; m < kR; ++m) amp[m] = p.amp[ab + (i + m * kT) * es];
     load_twiddles<N>(tw);
     __syncthreads();
-    fft_line<N, 16, true>(v, i, line, tw);
+    fft_line<N, R, true>(v, i, line, tw);
     double acc = 0.0;
     if (valid) {
 #pragma unroll
@@ -220,7 +225,7 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St)>::kThreads))
         v[m] = {q.x * p.out_scale, q.y * p.out_scale};
       }
     }
-    fft_line<N, 16, false>(v, i, line, tw);
+    fft_line<N, R, false>(v, i, line, tw);
     if (valid) {
 #pragma unroll
       for (int m = 0; m < kR; ++m) Io::st(w, soff + base + (i + m * kT) * es, v[m]);
@@ -233,17 +238,13 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St)>::kThreads))
   } else if constexpr (M == Mode::kGrad) {
     const St* x = static_cast<const St*>(p.x);
 #pragma unroll
-    for (int m = 0; m < kR; ++m) {
-      v[m] = valid ? Io::ld(w, soff + base + (i + m * kT) * es) : float2{0.0f, 0.0f};
-    }
+    for (int m = 0; m < kR; ++m) v[m] = Io::ld(w, lb + (i + m * kT) * es);
     float2 xo[kR];
 #pragma unroll
-    for (int m = 0; m < kR; ++m) {
-      xo[m] = valid ? Io::ld(x, soff + base + (i + m * kT) * es) : float2{0.0f, 0.0f};
-    }
+    for (int m = 0; m < kR; ++m) xo[m] = Io::ld(x, lb + (i + m * kT) * es);
     load_twiddles<N>(tw);
     __syncthreads();
-    fft_line<N, 16, false>(v, i, line, tw);
+    fft_line<N, R, false>(v, i, line, tw);
     if (valid) {
       St* out = static_cast<St*>(p.out);
 #pragma unroll
@@ -255,21 +256,19 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St)>::kThreads))
   } else {  // kUpdate
     St* x = static_cast<St*>(p.x);
 #pragma unroll
-    for (int m = 0; m < kR; ++m) {
-      v[m] = valid ? Io::ld(w, soff + base + (i + m * kT) * es) : float2{0.0f, 0.0f};
-    }
+    for (int m = 0; m < kR; ++m) v[m] = Io::ld(w, lb + (i + m * kT) * es);
     // the old iterate and the support do not depend on the transform: fetch them now
     float2 xo[kR];
     bool inside[kR];
 #pragma unroll
     for (int m = 0; m < kR; ++m) {
-      const long long at = soff + base + (i + m * kT) * es;
-      inside[m] = valid ? p.support[at] != 0 : true;
-      xo[m] = valid ? Io::ld(x, at) : float2{0.0f, 0.0f};
+      const long long at = lb + (i + m * kT) * es;
+      inside[m] = p.support[at] != 0;
+      xo[m] = Io::ld(x, at);
     }
     load_twiddles<N>(tw);
     __syncthreads();
-    fft_line<N, 16, false>(v, i, line, tw);
+    fft_line<N, R, false>(v, i, line, tw);
     if (valid) {
 #pragma unroll
       for (int m = 0; m < kR; ++m) {
@@ -281,7 +280,7 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St)>::kThreads))
         v[m] = nx;
       }
     }
-    fft_line<N, 16, true>(v, i, line, tw);
+    fft_line<N, R, true>(v, i, line, tw);
     if (valid) {
 #pragma unroll
       for (int m = 0; m < kR; ++m) {

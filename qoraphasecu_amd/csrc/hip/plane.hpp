@@ -107,18 +107,29 @@ __global__ void __launch_bounds__((PlaneGeom<NY, NX>::kThreads))
   fft_line<NY, 16, false>(v, j, lds + x * G::kColStride, twy);
   // (the old iterate and the support are read here rather than prefetched: at 128 x 128
   // the 1024-thread block has 128 registers per thread, and holding them across the
-  // transforms would spill)
+  // transforms would spill.  They are read kG elements at a time, unconditionally, so
+  // that 2 kG loads are in flight together.)
+  constexpr int kG = sizeof(St) == 4 ? 4 : 2;
 #pragma unroll
-  for (int m = 0; m < 16; ++m) {
-    const long long at = plane + static_cast<long long>(j + m * G::kTY) * NX + x;
-    const float2 pm = {v[m].x * p.in_scale, v[m].y * p.in_scale};
-    float2 nx = pm;
-    if (!sup[at]) {
-      const float2 xo = Io::ld(xs, at);
-      nx = {p.a * xo.x + p.b * pm.x, p.a * xo.y + p.b * pm.y};
+  for (int h = 0; h < 16; h += kG) {
+    float2 xo[kG];
+    bool inside[kG];
+#pragma unroll
+    for (int q = 0; q < kG; ++q) {
+      const long long at = plane + static_cast<long long>(j + (h + q) * G::kTY) * NX + x;
+      inside[q] = sup[at] != 0;
+      xo[q] = Io::ld(xs, at);
     }
-    Io::st(xs, at, nx);
-    v[m] = nx;
+#pragma unroll
+    for (int q = 0; q < kG; ++q) {
+      const int m = h + q;
+      const long long at = plane + static_cast<long long>(j + m * G::kTY) * NX + x;
+      const float2 pm = {v[m].x * p.in_scale, v[m].y * p.in_scale};
+      float2 nx = pm;
+      if (!inside[q]) nx = {p.a * xo[q].x + p.b * pm.x, p.a * xo[q].y + p.b * pm.y};
+      Io::st(xs, at, nx);
+      v[m] = nx;
+    }
   }
   // the next iteration's partial spectrum: y forward, x forward
   fft_line<NY, 16, true>(v, j, lds + x * G::kColStride, twy);

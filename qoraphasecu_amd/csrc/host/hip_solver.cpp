@@ -48,6 +48,8 @@ void tune(qpch_problem& q) {
   q.batch = t.batch;
   q.plane = t.plane;
   q.xcd = t.xcd;
+  q.lanes = t.lanes;
+  q.radix = t.radix;
 }
 
 Plan plan_for(Variant v) {
@@ -101,6 +103,7 @@ class HipSolver final : public Solver {
     p.storage = plan.storage;
     p.mixed = plan.mixed;
     p.schedule = plan.schedule;
+    p.lanes = plan.batched ? 0 : 1;  // one start per launch stays one launch at a time
     if (plan.schedule == QPCH_GRAPH && is_fusable(shape_)) tune(p);
     qpch_check(qpch_solver_create(&p, &h_));
     energy_ = qpch_solver_energy(h_);

@@ -85,9 +85,10 @@ def _problem(shape, m, masked=False):
 
 @pytest.mark.parametrize("shape", [(64, 64), (48, 40), (32, 32, 32), (256, 256), (16, 8, 32), (8, 128, 128),
                                    (16, 64, 32)])
-@pytest.mark.parametrize("schedule", ["graph", "fused", "unfused"])
-def test_teacher_forced_iteration(qpch, shape, schedule):
-    """One iteration from the same start: error within 1e-5 relative, object within 1e-5."""
+@pytest.mark.parametrize("schedule,radix", [("graph", 0), ("fused", 0), ("unfused", 0), ("graph", 8)])
+def test_teacher_forced_iteration(qpch, shape, schedule, radix):
+    """One iteration from the same start: error within 1e-5 relative, object within 1e-5
+    (radix 8: the radix-8 line passes on the axes of 64 or more)."""
     obj, amp, mask, sup = _problem(shape, max(4, shape[-1] // 4), masked=True)
     start = np.stack([synthetic.random_field(obj.size, 11).reshape(shape),
                       synthetic.random_field(obj.size, 12).reshape(shape)])
@@ -97,7 +98,7 @@ def test_teacher_forced_iteration(qpch, shape, schedule):
         ores = o.results()
         sf = np.stack([oracle.to_fft_order(s) for s in start])
         s = qpch.Solver(oracle.to_fft_order(amp), oracle.to_fft_order(mask),
-                        oracle.to_fft_order(sup), starts=2, start=sf, schedule=schedule)
+                        oracle.to_fft_order(sup), starts=2, start=sf, schedule=schedule, radix=radix)
         errs = R.run_abi(s, ops)
         xs = s.read_state()
         for k in range(2):
@@ -133,9 +134,10 @@ def test_agrees_with_oracle_short_recipe(qpch, shape, m, schedule):
     s.close()
 
 
-@pytest.mark.parametrize("shape", [(64, 64), (256, 256), (32, 32, 32), (8, 128, 128)])
+@pytest.mark.parametrize("shape,radix", [((64, 64), 0), ((256, 256), 0), ((32, 32, 32), 0), ((8, 128, 128), 0),
+                                         ((256, 256), 8), ((8, 128, 128), 8)])
 @pytest.mark.parametrize("storage,tol,floor", [("f16", 3e-3, 5e-4), ("bf16", 2e-2, 4e-3)])
-def test_low_precision_iteration(qpch, shape, storage, tol, floor):
+def test_low_precision_iteration(qpch, shape, radix, storage, tol, floor):
     """One RAAR and one ER iteration in fp16 / bf16 storage against the FP64 oracle from
     the same start: errors within the storage precision (fp32 arithmetic throughout); the
     floor is the storage's unit roundoff, below which an error value is rounding noise."""
@@ -146,7 +148,7 @@ def test_low_precision_iteration(qpch, shape, storage, tol, floor):
     o.run(ops)
     want = o.results()[0][2]
     s = qpch.Solver(oracle.to_fft_order(amp), oracle.to_fft_order(mask), oracle.to_fft_order(sup), starts=1,
-                    start=np.stack([oracle.to_fft_order(start[0])]), storage=storage, mixed=False)
+                    start=np.stack([oracle.to_fft_order(start[0])]), storage=storage, mixed=False, radix=radix)
     got = R.run_abi(s, ops)[:, 0]
     s.close()
     assert np.all(np.abs(got - want) <= tol * want + floor), (got, want)

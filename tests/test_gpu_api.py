@@ -187,8 +187,8 @@ def test_ensemble_equals_separate_runs(qpc, variant):
 
 
 def test_launch_configuration_does_not_change_results(qpc, qpch):
-    """Plane vs line passes, XCD-affine vs plain numbering, and starts per launch give the
-    same iterates and errors."""
+    """Plane vs line passes, XCD-affine vs plain numbering, starts per launch and launch
+    groups run side by side on their own streams give the same iterates and errors."""
     import recipes as R
 
     obj, sup, inten = synthetic.triangle((32, 32, 32), 8)
@@ -196,8 +196,9 @@ def test_launch_configuration_does_not_change_results(qpc, qpch):
     f = oracle.to_fft_order
     ops = R.flat(R.raar(6), R.shrinkwrap(method="rms"), R.er(4))
     runs = []
-    for plane, xcd, batch in [(1, 0, 0), (2, 0, 0), (1, 2, 0), (1, 0, 4)]:
-        s = qpch.Solver(f(amp), f(mask), f(sup), starts=8, seed=2, plane=plane, xcd=xcd, batch=batch)
+    for plane, xcd, batch, lanes in [(1, 0, 0, 1), (2, 0, 0, 1), (1, 2, 0, 1), (1, 0, 4, 1), (1, 0, 2, 4),
+                                     (2, 0, 4, 2)]:
+        s = qpch.Solver(f(amp), f(mask), f(sup), starts=8, seed=2, plane=plane, xcd=xcd, batch=batch, lanes=lanes)
         errs = R.run_abi(s, ops)
         runs.append((errs, s.read_state(), s.get_support()))
         s.close()

@@ -116,15 +116,16 @@ __device__ __forceinline__ void load_twiddles(float2* tw) {
 // One element of the modulus projection: w is the (scaled) spectrum value, a its measured
 // amplitude or -1 when masked.  Usable pixels add (|w| - a)^2 to acc and take magnitude
 // a; masked ones keep their own.  Reference semantics: src/cpu/kernels.cpp:29-39.
+// The magnitude and the ratio use the hardware square root and reciprocal (each within
+// about one ulp) rather than the correctly rounded sequences, which cost as many
+// instructions as a radix-16 stage; the selects keep the lanes branch-free.
 __device__ __forceinline__ float2 project(float2 w, float a, double& acc) {
-  const float mag = sqrtf(w.x * w.x + w.y * w.y);
-  float target = mag;
-  if (a >= 0.0f) {
-    const float d = mag - a;
-    acc += static_cast<double>(d) * static_cast<double>(d);
-    target = a;
-  }
-  const float f = target / (mag + 1e-12f);
+  const float mag = __builtin_amdgcn_sqrtf(w.x * w.x + w.y * w.y);
+  const bool usable = a >= 0.0f;
+  const float d = usable ? mag - a : 0.0f;
+  acc += static_cast<double>(d) * static_cast<double>(d);
+  const float target = usable ? a : mag;
+  const float f = target * __builtin_amdgcn_rcpf(mag + 1e-12f);
   return {w.x * f, w.y * f};
 }
 

@@ -50,26 +50,7 @@ def log(*a):
 def workload(name):
     from qoraphasecu_amd import workloads
 
-    if name == "c2":
-        obj, inten, mask = workloads.problem_2d(256, seed=1000, photons=1e7, beamstop_radius=6)
-        recipe = workloads.raar_shrinkwrap_recipe(10, 50, 200, method="max")
-        return dict(name="c2", shape=(256, 256), starts=32, intensity=inten, mask=mask, recipe=recipe,
-                    desc="256^2, 32 starts, repeat(10, RAAR(50, 0.9), Shrinkwrap(max)) + ER(200), "
-                         "Poisson 1e7 photons, 6 px beamstop, autocorrelation support")
-    if name == "c4":
-        obj, inten, mask = workloads.problem_3d(128, seed=2000)
-        recipe = workloads.raar_shrinkwrap_recipe(8, 50, 200, method="rms")
-        return dict(name="c4", shape=(128, 128, 128), starts=8, intensity=inten, mask=mask, recipe=recipe,
-                    desc="128^3, 8 starts, repeat(8, RAAR(50, 0.9), Shrinkwrap(rms)) + ER(200), "
-                         "noiseless, 1% random-pixel mask, autocorrelation support")
-    if name == "c3":
-        problems = workloads.ensemble_uneven(per_size=16, sizes=(128, 256, 512), seed=0)
-        from qoraphasecu_amd.recipe import default_recipe
-
-        return dict(name="c3", shape=None, starts=8, problems=problems, recipe=default_recipe(),
-                    desc="48 problems (16 each at 128^2, 256^2, 512^2), 8 starts each, default_recipe, "
-                         "Poisson 1e6-1e8 photons, 4-10 px beamstops, autocorrelation supports")
-    raise ValueError(name)
+    return workloads.config(name)
 
 
 def time_ensemble(w, variant, steps, warmup):
@@ -219,6 +200,79 @@ def numpy_fft_pair(shape, reps=20):
     return (time.perf_counter() - t0) / reps * 1e3
 
 
+def input_hashes(w):
+    """SHA-256 of the generated inputs (float32 intensity, uint8 mask), as the committed
+    manifest tests/golden/workloads_manifest.json records them."""
+    import hashlib
+
+    def h(a):
+        return None if a is None else hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+    if "problems" in w:
+        return {"intensity": [h(np.asarray(p[1], np.float32)) for p in w["problems"]],
+                "mask": [h(np.asarray(p[2], np.uint8)) for p in w["problems"]]}
+    return {"intensity": h(np.asarray(w["intensity"], np.float32)),
+            "mask": h(None if w["mask"] is None else np.asarray(w["mask"], np.uint8))}
+
+
+def rocfft_floor(shape, starts, reps=50):
+    """A forward + inverse complex64 transform of every start (torch.fft, i.e. rocFFT),
+    captured in a graph and replayed: the floor of one iteration's transforms, timed on
+    the capturing stream with HIP events."""
+    import torch
+
+    x = torch.randn((starts, *shape), dtype=torch.complex64, device="cuda")
+    dims = tuple(range(1, len(shape) + 1))
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        for _ in range(3):
+            y = torch.fft.ifftn(torch.fft.fftn(x, dim=dims), dim=dims)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    mode = "graph"
+    try:
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            y = torch.fft.ifftn(torch.fft.fftn(x, dim=dims), dim=dims)
+        run = g.replay
+    except Exception as e:  # timing floor only: fall back to eager launches
+        mode = f"eager ({type(e).__name__})"
+
+        def run():
+            torch.fft.ifftn(torch.fft.fftn(x, dim=dims), dim=dims)
+    run()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        run()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return dict(ms_per_pair=ms, starts=starts, mode=mode, start_pairs_per_s=starts / (ms * 1e-3))
+
+
+def cpu_full_run(w, threads):
+    """The oracle (float, OpenMP) running the whole recipe on every start, after a warm-up
+    run of the same recipe (bench/bench.cpp:33-45)."""
+    import oracle
+
+    oracle.set_threads(threads)
+    amp, mask, sup = oracle.make_problem(w["intensity"], None, w["mask"])
+    ops = w["recipe"].ops()
+    iters = w["recipe"].iterations()
+    warm = oracle.Solver(amp, mask, sup, starts=w["starts"], seed=0)
+    warm.run(ops)
+    solver = oracle.Solver(amp, mask, sup, starts=w["starts"], seed=0)
+    t0 = time.perf_counter()
+    solver.run(ops)
+    wall = time.perf_counter() - t0
+    best = min(float(r[2][-1]) for r in solver.results())
+    return dict(value=w["starts"] * iters / wall, seconds=wall, threads=threads, iterations=iters,
+                starts=w["starts"], best_final_error=best)
+
+
 def traffic_for(kernel_key):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, when present."""
     path = os.path.join(HERE, "profiles", "traffic.json")
@@ -235,10 +289,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--configs", default="c2,c4,c3,c5")
+    ap.add_argument("--configs", default="c2,c4,c3,c5,c1")
     ap.add_argument("--variants", default="graph,mixed,mixed_bf16")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-passes", action="store_true", help="skip the per-pass timing")
+    ap.add_argument("--no-floor", action="store_true", help="skip the rocFFT transform floor")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB 10 times (the reference point of the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -270,9 +325,25 @@ def main():
             details["c5"] = time_refinement()
             continue
         w = workload(name)
+        if name == "c1":
+            # the reference's CPU-runnable case: the oracle in full (all threads and one), and
+            # the device on the same problem
+            res = {"workload": w["desc"], "iterations": w["recipe"].iterations(), "starts": w["starts"],
+                   "inputs_sha256": input_hashes(w)}
+            for variant in ("graph", "batched"):
+                res[variant] = time_variant(w, variant, 1 if variant == "graph" else 0, args.steps, args.warmup)
+            if not args.no_cpu:
+                threads = min(16, os.cpu_count() or 1)
+                res["cpu_oracle"] = cpu_full_run(w, threads)
+                res["cpu_oracle_1_thread"] = cpu_full_run(w, 1)
+                log(f"  c1 cpu oracle: {res['cpu_oracle']['value']:.4g} start-it/s ({threads} threads), "
+                    f"{res['cpu_oracle_1_thread']['value']:.4g} (1 thread); best final error "
+                    f"{res['cpu_oracle']['best_final_error']:.3e}")
+            details["c1"] = res
+            continue
         if name == "c3":
             res = {"workload": w["desc"], "iterations": w["recipe"].iterations(), "starts": w["starts"],
-                   "problems": len(w["problems"])}
+                   "problems": len(w["problems"]), "inputs_sha256": input_hashes(w)}
             res["ensemble_mixed"] = time_ensemble(w, "mixed", args.steps, args.warmup)
             res["ensemble_batched"] = time_ensemble(w, "batched", args.steps, args.warmup)
             res["per_problem_batched"] = time_per_problem(w, "batched", args.steps)
@@ -284,7 +355,8 @@ def main():
             details["c3"] = res
             continue
         log(f"{name}: {w['desc']}")
-        res = {"workload": w["desc"], "iterations": w["recipe"].iterations(), "starts": w["starts"]}
+        res = {"workload": w["desc"], "iterations": w["recipe"].iterations(), "starts": w["starts"],
+               "inputs_sha256": input_hashes(w)}
         for variant in [v for v in args.variants.split(",") if v]:
             batch = 1 if variant in ("graph", "fused", "naive", "callback") else 0
             res[variant] = time_variant(w, variant, batch, args.steps, args.warmup)
@@ -312,6 +384,12 @@ def main():
             log(f"    f16 pass axis {p['axis']} {p['mode']:>8}: {p['ms'] * 1e3:8.1f} us  {p['achieved_gbs']:8.0f} GB/s")
         for p in passes32:
             log(f"    f32 pass axis {p['axis']} {p['mode']:>8}: {p['ms'] * 1e3:8.1f} us  {p['achieved_gbs']:8.0f} GB/s")
+        if not args.no_floor:
+            fl = rocfft_floor(w["shape"], w["starts"])
+            res["rocfft_pair_floor"] = fl
+            it_us = {v: res[v]["ms_per_run"] * 1e3 / w["recipe"].iterations() for v in ("graph", "mixed") if v in res}
+            log(f"    rocFFT fwd+inv pair over all {w['starts']} starts ({fl['mode']}): {fl['ms_per_pair'] * 1e3:.1f} us; "
+                + ", ".join(f"{v} iteration {t:.1f} us" for v, t in it_us.items()))
         if not args.no_cpu:
             threads = min(16, os.cpu_count() or 1)
             cb = cpu_baseline(w, threads)

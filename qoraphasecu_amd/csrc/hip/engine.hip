@@ -146,11 +146,13 @@ bool eager_launches() {
   return v && v[0] == '1';
 }
 
+// The fused passes address one start's arrays with 32-bit byte offsets (buffer
+// resources): a start of complex64 must stay below 2 GiB.
 bool fusable(const Dims& d) {
   for (int a = 0; a < d.nd; ++a) {
     if (!is_pow2(d.d[a]) || d.d[a] < 2 || d.d[a] > kMaxFused) return false;
   }
-  return true;
+  return d.n * static_cast<long long>(sizeof(float2)) < (1LL << 31);
 }
 
 int grid_for(long long work, int threads = 256, int cap = 2048) {

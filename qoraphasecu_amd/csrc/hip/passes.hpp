@@ -178,41 +178,44 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St), R>::kThreads))
     valid = o < p.outer;
     base = static_cast<long long>(o) * N;
   }
-  const long long es = COL ? p.inner : 1;
+  // This start's arrays as buffer resources: register m of this thread is element
+  // e0 + m * em of the start, e0 per lane, em the same for the whole launch.  Loads are
+  // unconditional so that all of a thread's loads are in flight at once (a load under a
+  // branch gets its own wait): a lane past the last line of a ragged tile reads line 0 of
+  // its start instead, and its results are never stored.
+  const int es = COL ? p.inner : 1;
+  const int e0 = static_cast<int>(valid ? base : 0) + i * es;
+  const int em = kT * es;
+  constexpr int EB = static_cast<int>(sizeof(St));
   const long long soff = static_cast<long long>(start) * p.n;
-  // Loads are unconditional so that all of a thread's loads are in flight at once (a load
-  // under `valid ?` becomes a branch with its own wait): a lane past the last line of a
-  // ragged tile reads line 0 of its start instead and its results are never stored.
-  const long long lb = soff + (valid ? base : 0);
+  const long long nb = p.n * EB;
   float2* line = lds + b * G::kStride;
 
-  St* w = static_cast<St*>(p.w);
-  const St* src = p.src ? static_cast<const St*>(p.src) : w;
+  St* w = static_cast<St*>(p.w) + soff;
+  const Rsrc rw = make_rsrc(w, nb);
   float2 v[kR];
 
   if constexpr (M == Mode::kFwd || M == Mode::kInv) {
+    const Rsrc rsrc_in = p.src ? make_rsrc(static_cast<const St*>(p.src) + soff, nb) : rw;
 #pragma unroll
-    for (int m = 0; m < kR; ++m) {
-      v[m] = Io::ld(src, lb + (i + m * kT) * es);
-    }
+    for (int m = 0; m < kR; ++m) v[m] = Io::bld(rsrc_in, e0 * EB, m * em * EB);
     load_twiddles<N>(tw);  // while the loads are in flight
     __syncthreads();
     fft_line<N, R, M == Mode::kFwd>(v, i, line, tw);
     if (valid) {
 #pragma unroll
       for (int m = 0; m < kR; ++m) {
-        Io::st(w, soff + base + (i + m * kT) * es,
-               float2{v[m].x * p.out_scale, v[m].y * p.out_scale});
+        Io::bst(rw, e0 * EB, m * em * EB, float2{v[m].x * p.out_scale, v[m].y * p.out_scale});
       }
     }
   } else if constexpr (M == Mode::kModulus) {
 #pragma unroll
-    for (int m = 0; m < kR; ++m) v[m] = Io::ld(w, lb + (i + m * kT) * es);
+    for (int m = 0; m < kR; ++m) v[m] = Io::bld(rw, e0 * EB, m * em * EB);
     float amp[kR];
     const long long arow = p.amp_stride * ((p.err_offset + start) / (p.amp_group > 1 ? p.amp_group : 1));
-    const long long ab = arow + (valid ? base : 0);
+    const Rsrc ra = make_rsrc(p.amp + arow, p.n * 4);
 #pragma unroll
-    for (int m = 0; m < kR; ++m) amp[m] = p.amp[ab + (i + m * kT) * es];
+    for (int m = 0; m < kR; ++m) amp[m] = buf_f32(ra, e0 * 4, m * em * 4);
     load_twiddles<N>(tw);
     __syncthreads();
     fft_line<N, R, true>(v, i, line, tw);
@@ -229,7 +232,7 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St), R>::kThreads))
     fft_line<N, R, false>(v, i, line, tw);
     if (valid) {
 #pragma unroll
-      for (int m = 0; m < kR; ++m) Io::st(w, soff + base + (i + m * kT) * es, v[m]);
+      for (int m = 0; m < kR; ++m) Io::bst(rw, e0 * EB, m * em * EB, v[m]);
     }
     const double total = block_sum(acc, red);
     if (tid == 0) {
@@ -237,35 +240,35 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St), R>::kThreads))
       if (tile == 0 && start == 0) atomicAdd(p.counter, 1);
     }
   } else if constexpr (M == Mode::kGrad) {
-    const St* x = static_cast<const St*>(p.x);
+    const Rsrc rx = make_rsrc(static_cast<const St*>(p.x) + soff, nb);
 #pragma unroll
-    for (int m = 0; m < kR; ++m) v[m] = Io::ld(w, lb + (i + m * kT) * es);
+    for (int m = 0; m < kR; ++m) v[m] = Io::bld(rw, e0 * EB, m * em * EB);
     float2 xo[kR];
 #pragma unroll
-    for (int m = 0; m < kR; ++m) xo[m] = Io::ld(x, lb + (i + m * kT) * es);
+    for (int m = 0; m < kR; ++m) xo[m] = Io::bld(rx, e0 * EB, m * em * EB);
     load_twiddles<N>(tw);
     __syncthreads();
     fft_line<N, R, false>(v, i, line, tw);
     if (valid) {
-      St* out = static_cast<St*>(p.out);
+      const Rsrc ro = make_rsrc(static_cast<St*>(p.out) + soff, nb);
 #pragma unroll
       for (int m = 0; m < kR; ++m) {
         const float2 pm = {v[m].x * p.in_scale, v[m].y * p.in_scale};
-        Io::st(out, soff + base + (i + m * kT) * es, float2{2.0f * (xo[m].x - pm.x), 2.0f * (xo[m].y - pm.y)});
+        Io::bst(ro, e0 * EB, m * em * EB, float2{2.0f * (xo[m].x - pm.x), 2.0f * (xo[m].y - pm.y)});
       }
     }
   } else {  // kUpdate
-    St* x = static_cast<St*>(p.x);
+    const Rsrc rx = make_rsrc(static_cast<St*>(p.x) + soff, nb);
+    const Rsrc rs = make_rsrc(p.support + soff, p.n);
 #pragma unroll
-    for (int m = 0; m < kR; ++m) v[m] = Io::ld(w, lb + (i + m * kT) * es);
+    for (int m = 0; m < kR; ++m) v[m] = Io::bld(rw, e0 * EB, m * em * EB);
     // the old iterate and the support do not depend on the transform: fetch them now
     float2 xo[kR];
     bool inside[kR];
 #pragma unroll
     for (int m = 0; m < kR; ++m) {
-      const long long at = lb + (i + m * kT) * es;
-      inside[m] = p.support[at] != 0;
-      xo[m] = Io::ld(x, at);
+      inside[m] = buf_u8(rs, e0, m * em) != 0;
+      xo[m] = Io::bld(rx, e0 * EB, m * em * EB);
     }
     load_twiddles<N>(tw);
     __syncthreads();
@@ -273,11 +276,10 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St), R>::kThreads))
     if (valid) {
 #pragma unroll
       for (int m = 0; m < kR; ++m) {
-        const long long at = soff + base + (i + m * kT) * es;
         const float2 pm = {v[m].x * p.in_scale, v[m].y * p.in_scale};
         float2 nx = pm;
         if (!inside[m]) nx = {p.a * xo[m].x + p.b * pm.x, p.a * xo[m].y + p.b * pm.y};
-        Io::st(x, at, nx);
+        Io::bst(rx, e0 * EB, m * em * EB, nx);
         v[m] = nx;
       }
     }
@@ -285,8 +287,7 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St), R>::kThreads))
     if (valid) {
 #pragma unroll
       for (int m = 0; m < kR; ++m) {
-        Io::st(w, soff + base + (i + m * kT) * es,
-               float2{v[m].x * p.out_scale, v[m].y * p.out_scale});
+        Io::bst(rw, e0 * EB, m * em * EB, float2{v[m].x * p.out_scale, v[m].y * p.out_scale});
       }
     }
     if (tile == 0 && tid < 64 && p.errs) {

@@ -121,3 +121,41 @@ def raar_shrinkwrap_recipe(repeats, raar_n, er_n, method="max", beta=0.9):
     from .recipe import ER, RAAR, Recipe, Shrinkwrap, repeat
 
     return Recipe(repeat(repeats, RAAR(raar_n, beta), Shrinkwrap(method=method)), ER(er_n))
+
+
+def config(name):
+    """The benchmark workloads by name, as dicts (shape, starts, intensity, mask, recipe,
+    desc; "problems" for the uneven ensemble):
+      c1  64^2 disc, 8 starts, default_recipe, noiseless, no mask
+      c2  256^2 ellipses, 32 starts, repeat(10, RAAR(50, 0.9), Shrinkwrap max) + ER(200),
+          Poisson 1e7 photons, 6 px beamstop
+      c3  48 problems (16 each at 128^2, 256^2, 512^2), 8 starts each, default_recipe
+      c4  128^3 ellipsoids, 8 starts, repeat(8, RAAR(50, 0.9), Shrinkwrap rms) + ER(200),
+          noiseless, 1% random-pixel mask
+    Every problem's support comes from the autocorrelation (threshold 0.1)."""
+    from .recipe import default_recipe
+
+    if name == "c1":
+        obj, inten = problem_disc(64, radius=16, seed=1000)
+        return dict(name="c1", shape=(64, 64), starts=8, object=obj, intensity=inten, mask=None,
+                    recipe=default_recipe(),
+                    desc="64^2 disc (radius 16), 8 starts, default_recipe, noiseless, no mask, "
+                         "autocorrelation support")
+    if name == "c2":
+        obj, inten, mask = problem_2d(256, seed=1000, photons=1e7, beamstop_radius=6)
+        return dict(name="c2", shape=(256, 256), starts=32, object=obj, intensity=inten, mask=mask,
+                    recipe=raar_shrinkwrap_recipe(10, 50, 200, method="max"),
+                    desc="256^2, 32 starts, repeat(10, RAAR(50, 0.9), Shrinkwrap(max)) + ER(200), "
+                         "Poisson 1e7 photons, 6 px beamstop, autocorrelation support")
+    if name == "c4":
+        obj, inten, mask = problem_3d(128, seed=2000)
+        return dict(name="c4", shape=(128, 128, 128), starts=8, object=obj, intensity=inten, mask=mask,
+                    recipe=raar_shrinkwrap_recipe(8, 50, 200, method="rms"),
+                    desc="128^3, 8 starts, repeat(8, RAAR(50, 0.9), Shrinkwrap(rms)) + ER(200), "
+                         "noiseless, 1% random-pixel mask, autocorrelation support")
+    if name == "c3":
+        problems = ensemble_uneven(per_size=16, sizes=(128, 256, 512), seed=0)
+        return dict(name="c3", shape=None, starts=8, problems=problems, recipe=default_recipe(),
+                    desc="48 problems (16 each at 128^2, 256^2, 512^2), 8 starts each, default_recipe, "
+                         "Poisson 1e6-1e8 photons, 4-10 px beamstops, autocorrelation supports")
+    raise ValueError(f"unknown workload {name!r}")

@@ -129,6 +129,55 @@ __device__ __forceinline__ float2 project(float2 w, float a, double& acc) {
   return {w.x * f, w.y * f};
 }
 
+// One thread's strided view of one start's array: register m is element e0 + m * em.
+// Column passes (BUF) address it through a buffer resource: the stride em is a runtime
+// value that the SGPR offset carries, so a load costs no per-lane 64-bit address
+// arithmetic.  Row passes use the pointer: there em is a compile-time constant that folds
+// into the instructions' immediate offsets.
+template <class T, bool BUF>
+struct View {
+  T* p;
+  Rsrc r;
+  int e0, em;
+  __device__ __forceinline__ View(T* base, long long count, int first, int step)
+      : p(base), r(make_rsrc(base, count * static_cast<long long>(sizeof(T)))), e0(first), em(step) {}
+};
+
+template <class St, bool BUF>
+__device__ __forceinline__ float2 vld(const View<St, BUF>& v, int m) {
+  constexpr int EB = static_cast<int>(sizeof(St));
+  if constexpr (BUF) {
+    return Storage<St>::bld(v.r, v.e0 * EB, m * v.em * EB);
+  } else {
+    return Storage<St>::ld(v.p, static_cast<size_t>(v.e0 + m * v.em));
+  }
+}
+template <class St, bool BUF>
+__device__ __forceinline__ void vst(const View<St, BUF>& v, int m, float2 x) {
+  constexpr int EB = static_cast<int>(sizeof(St));
+  if constexpr (BUF) {
+    Storage<St>::bst(v.r, v.e0 * EB, m * v.em * EB, x);
+  } else {
+    Storage<St>::st(v.p, static_cast<size_t>(v.e0 + m * v.em), x);
+  }
+}
+template <bool BUF>
+__device__ __forceinline__ float vld_f32(const View<const float, BUF>& v, int m) {
+  if constexpr (BUF) {
+    return buf_f32(v.r, v.e0 * 4, m * v.em * 4);
+  } else {
+    return v.p[v.e0 + m * v.em];
+  }
+}
+template <bool BUF>
+__device__ __forceinline__ bool vld_flag(const View<const std::uint8_t, BUF>& v, int m) {
+  if constexpr (BUF) {
+    return buf_u8(v.r, v.e0, m * v.em) != 0;
+  } else {
+    return v.p[v.e0 + m * v.em] != 0;
+  }
+}
+
 // Block numbering: with p.slots > 0 (a multiple of 8) block L handles start L % slots and
 // tile L / slots, so every block of a start lands on the same XCD (blocks are dealt to the
 // eight XCDs round-robin) and the start's arrays stay in that XCD's L2 from one pass to the
@@ -137,7 +186,6 @@ template <int N, bool COL, Mode M, class St, int R>
 __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St), R>::kThreads))
     line_pass(PassArgs p) {
   using G = PassGeom<N, COL, sizeof(St), R>;
-  using Io = Storage<St>;
   constexpr int kT = G::kT;
   constexpr int kR = G::kR;
   constexpr int kB = G::kB;
@@ -178,44 +226,38 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St), R>::kThreads))
     valid = o < p.outer;
     base = static_cast<long long>(o) * N;
   }
-  // This start's arrays as buffer resources: register m of this thread is element
-  // e0 + m * em of the start, e0 per lane, em the same for the whole launch.  Loads are
+  // Register m of this thread is element e0 + m * em of its start.  Loads are
   // unconditional so that all of a thread's loads are in flight at once (a load under a
   // branch gets its own wait): a lane past the last line of a ragged tile reads line 0 of
   // its start instead, and its results are never stored.
   const int es = COL ? p.inner : 1;
   const int e0 = static_cast<int>(valid ? base : 0) + i * es;
   const int em = kT * es;
-  constexpr int EB = static_cast<int>(sizeof(St));
   const long long soff = static_cast<long long>(start) * p.n;
-  const long long nb = p.n * EB;
   float2* line = lds + b * G::kStride;
-
-  St* w = static_cast<St*>(p.w) + soff;
-  const Rsrc rw = make_rsrc(w, nb);
+  using SV = View<St, COL>;
+  const SV vw(static_cast<St*>(p.w) + soff, p.n, e0, em);
   float2 v[kR];
 
   if constexpr (M == Mode::kFwd || M == Mode::kInv) {
-    const Rsrc rsrc_in = p.src ? make_rsrc(static_cast<const St*>(p.src) + soff, nb) : rw;
+    const SV vin(p.src ? static_cast<St*>(const_cast<void*>(p.src)) + soff : vw.p, p.n, e0, em);
 #pragma unroll
-    for (int m = 0; m < kR; ++m) v[m] = Io::bld(rsrc_in, e0 * EB, m * em * EB);
+    for (int m = 0; m < kR; ++m) v[m] = vld(vin, m);
     load_twiddles<N>(tw);  // while the loads are in flight
     __syncthreads();
     fft_line<N, R, M == Mode::kFwd>(v, i, line, tw);
     if (valid) {
 #pragma unroll
-      for (int m = 0; m < kR; ++m) {
-        Io::bst(rw, e0 * EB, m * em * EB, float2{v[m].x * p.out_scale, v[m].y * p.out_scale});
-      }
+      for (int m = 0; m < kR; ++m) vst(vw, m, float2{v[m].x * p.out_scale, v[m].y * p.out_scale});
     }
   } else if constexpr (M == Mode::kModulus) {
 #pragma unroll
-    for (int m = 0; m < kR; ++m) v[m] = Io::bld(rw, e0 * EB, m * em * EB);
+    for (int m = 0; m < kR; ++m) v[m] = vld(vw, m);
     float amp[kR];
     const long long arow = p.amp_stride * ((p.err_offset + start) / (p.amp_group > 1 ? p.amp_group : 1));
-    const Rsrc ra = make_rsrc(p.amp + arow, p.n * 4);
+    const View<const float, COL> va(p.amp + arow, p.n, e0, em);
 #pragma unroll
-    for (int m = 0; m < kR; ++m) amp[m] = buf_f32(ra, e0 * 4, m * em * 4);
+    for (int m = 0; m < kR; ++m) amp[m] = vld_f32(va, m);
     load_twiddles<N>(tw);
     __syncthreads();
     fft_line<N, R, true>(v, i, line, tw);
@@ -232,7 +274,7 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St), R>::kThreads))
     fft_line<N, R, false>(v, i, line, tw);
     if (valid) {
 #pragma unroll
-      for (int m = 0; m < kR; ++m) Io::bst(rw, e0 * EB, m * em * EB, v[m]);
+      for (int m = 0; m < kR; ++m) vst(vw, m, v[m]);
     }
     const double total = block_sum(acc, red);
     if (tid == 0) {
@@ -240,35 +282,35 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St), R>::kThreads))
       if (tile == 0 && start == 0) atomicAdd(p.counter, 1);
     }
   } else if constexpr (M == Mode::kGrad) {
-    const Rsrc rx = make_rsrc(static_cast<const St*>(p.x) + soff, nb);
+    const SV vx(static_cast<St*>(p.x) + soff, p.n, e0, em);
 #pragma unroll
-    for (int m = 0; m < kR; ++m) v[m] = Io::bld(rw, e0 * EB, m * em * EB);
+    for (int m = 0; m < kR; ++m) v[m] = vld(vw, m);
     float2 xo[kR];
 #pragma unroll
-    for (int m = 0; m < kR; ++m) xo[m] = Io::bld(rx, e0 * EB, m * em * EB);
+    for (int m = 0; m < kR; ++m) xo[m] = vld(vx, m);
     load_twiddles<N>(tw);
     __syncthreads();
     fft_line<N, R, false>(v, i, line, tw);
     if (valid) {
-      const Rsrc ro = make_rsrc(static_cast<St*>(p.out) + soff, nb);
+      const SV vo(static_cast<St*>(p.out) + soff, p.n, e0, em);
 #pragma unroll
       for (int m = 0; m < kR; ++m) {
         const float2 pm = {v[m].x * p.in_scale, v[m].y * p.in_scale};
-        Io::bst(ro, e0 * EB, m * em * EB, float2{2.0f * (xo[m].x - pm.x), 2.0f * (xo[m].y - pm.y)});
+        vst(vo, m, float2{2.0f * (xo[m].x - pm.x), 2.0f * (xo[m].y - pm.y)});
       }
     }
   } else {  // kUpdate
-    const Rsrc rx = make_rsrc(static_cast<St*>(p.x) + soff, nb);
-    const Rsrc rs = make_rsrc(p.support + soff, p.n);
+    const SV vx(static_cast<St*>(p.x) + soff, p.n, e0, em);
+    const View<const std::uint8_t, COL> vs(p.support + soff, p.n, e0, em);
 #pragma unroll
-    for (int m = 0; m < kR; ++m) v[m] = Io::bld(rw, e0 * EB, m * em * EB);
+    for (int m = 0; m < kR; ++m) v[m] = vld(vw, m);
     // the old iterate and the support do not depend on the transform: fetch them now
     float2 xo[kR];
     bool inside[kR];
 #pragma unroll
     for (int m = 0; m < kR; ++m) {
-      inside[m] = buf_u8(rs, e0, m * em) != 0;
-      xo[m] = Io::bld(rx, e0 * EB, m * em * EB);
+      inside[m] = vld_flag(vs, m);
+      xo[m] = vld(vx, m);
     }
     load_twiddles<N>(tw);
     __syncthreads();
@@ -279,16 +321,14 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St), R>::kThreads))
         const float2 pm = {v[m].x * p.in_scale, v[m].y * p.in_scale};
         float2 nx = pm;
         if (!inside[m]) nx = {p.a * xo[m].x + p.b * pm.x, p.a * xo[m].y + p.b * pm.y};
-        Io::bst(rx, e0 * EB, m * em * EB, nx);
+        vst(vx, m, nx);
         v[m] = nx;
       }
     }
     fft_line<N, R, true>(v, i, line, tw);
     if (valid) {
 #pragma unroll
-      for (int m = 0; m < kR; ++m) {
-        Io::bst(rw, e0 * EB, m * em * EB, float2{v[m].x * p.out_scale, v[m].y * p.out_scale});
-      }
+      for (int m = 0; m < kR; ++m) vst(vw, m, float2{v[m].x * p.out_scale, v[m].y * p.out_scale});
     }
     if (tile == 0 && tid < 64 && p.errs) {
       // fixed-order sum of the previous modulus pass's partials of this start

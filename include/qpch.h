@@ -69,8 +69,10 @@ typedef struct {
   int schedule;        /* QPCH_UNFUSED / QPCH_FUSED / QPCH_GRAPH */
   int problems;        /* independent problems of this shape stacked in one solver (0 = 1) */
   int plane;           /* 3D: 0 = automatic, 1 = xy-plane update pass, 2 = line passes only */
-  int xcd;             /* 0 = XCD-affine block numbering when starts per launch are a multiple of 8,
-                          2 = plain numbering */
+  int xcd;             /* 0 = XCD-affine block numbering when starts per launch are a multiple of 8
+                          (a start's blocks share one XCD's L2), 2 = plain numbering (a tile of
+                          every start on one XCD), 3 = plain for the frequency-axis pass only
+                          (its amplitude is shared by all starts) */
   int lanes;           /* launch groups in flight at once, each on its own stream (0 or 1 = one
                           after another; at most 8) */
   int radix;           /* FFT radix of the line passes: 0 or 16 = radix 16, 8 = radix 8 for axes

@@ -533,7 +533,7 @@ struct SolverImpl {
     bool plane = false;  // 3D: the update on whole xy-planes (axes 1 and 2 at once)
   };
   bool plane3d = false;  // 3D iterations use the plane pass
-  bool xcd_off = false;  // keep the plain block numbering even for multiples of 8 starts
+  int xcd_mode = 0;  // 0 XCD-affine by start, 2 plain numbering, 3 plain for the modulus pass only
   int radix = 16;        // FFT radix of the line passes (8 or 16)
 
   std::vector<PassSpec> iteration_passes(bool lowp) const {
@@ -588,7 +588,7 @@ struct SolverImpl {
     p.counter = counters.get() + g;
     p.a = a;
     p.b = b;
-    p.no_xcd = xcd_off ? 1 : 0;
+    p.no_xcd = (xcd_mode == 2 || (xcd_mode == 3 && ps.mode == Mode::kModulus)) ? 1 : 0;
     p.radix = radix;
     p.in_scale = ps.in_scale;
     p.out_scale = ps.out_scale;
@@ -932,7 +932,8 @@ struct SolverImpl {
     if (schedule == QPCH_GRAPH && eager_launches()) schedule = QPCH_FUSED;
     fused = fusable(dims) && schedule != QPCH_UNFUSED;
     plane3d = fused && dims.nd == 3 && plane_supported(dims.d[1], dims.d[2]) && p.plane != 2;
-    xcd_off = p.xcd == 2;
+    if (p.xcd != 0 && p.xcd != 2 && p.xcd != 3) throw InvalidArgument("xcd must be 0, 2 or 3");
+    xcd_mode = p.xcd;
     if (p.radix != 0 && p.radix != 8 && p.radix != 16) throw InvalidArgument("radix must be 0, 8 or 16");
     radix = p.radix == 8 ? 8 : 16;
     if (storage != QPCH_F32 && !fusable(dims)) {
@@ -1528,7 +1529,7 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
       qpch_tuning r = t;
       r.us_per_iteration = ms * 1e3 / 16.0;
       line += " [batch " + std::to_string(t.batch) + (t.lanes > 1 ? " x" + std::to_string(t.lanes) + " lanes" : "") +
-              (t.plane ? (t.plane == 1 ? " plane" : " lines") : "") + (t.xcd == 2 ? " xcd-off" : "") +
+              (t.plane ? (t.plane == 1 ? " plane" : " lines") : "") + (t.xcd == 2 ? " xcd-off" : t.xcd == 3 ? " xcd-off-modulus" : "") +
               (t.radix == 8 ? " radix-8" : "") + ": " + std::to_string(r.us_per_iteration) + " us/it]";
       if (r.us_per_iteration < best.us_per_iteration) best = r;
     };
@@ -1541,8 +1542,8 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
     }
     const qpch_tuning split = best;
     for (int pl : planes) {
-      for (int xcd : {0, 2}) {
-        if (xcd == 2 && !(split.batch >= 8 && split.batch % 8 == 0)) continue;  // numbering only differs then
+      for (int xcd : {0, 2, 3}) {
+        if (xcd != 0 && !(split.batch >= 8 && split.batch % 8 == 0)) continue;  // numbering only differs then
         for (int rx : radices) {
           if (pl == split.plane && xcd == split.xcd && rx == split.radix) continue;
           trial(qpch_tuning{split.batch, pl, xcd, 0.0, split.lanes, rx});

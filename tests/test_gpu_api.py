@@ -196,8 +196,8 @@ def test_launch_configuration_does_not_change_results(qpc, qpch):
     f = oracle.to_fft_order
     ops = R.flat(R.raar(6), R.shrinkwrap(method="rms"), R.er(4))
     runs = []
-    for plane, xcd, batch, lanes in [(1, 0, 0, 1), (2, 0, 0, 1), (1, 2, 0, 1), (1, 0, 4, 1), (1, 0, 2, 4),
-                                     (2, 0, 4, 2)]:
+    for plane, xcd, batch, lanes in [(1, 0, 0, 1), (2, 0, 0, 1), (1, 2, 0, 1), (1, 3, 0, 1), (2, 3, 0, 1),
+                                     (1, 0, 4, 1), (1, 0, 2, 4), (2, 0, 4, 2)]:
         s = qpch.Solver(f(amp), f(mask), f(sup), starts=8, seed=2, plane=plane, xcd=xcd, batch=batch, lanes=lanes)
         errs = R.run_abi(s, ops)
         runs.append((errs, s.read_state(), s.get_support()))

@@ -70,7 +70,6 @@ template <int NY, int NX, class St>
 __global__ void __launch_bounds__((PlaneGeom<NY, NX>::kThreads))
     plane_update(PassArgs p) {
   using G = PlaneGeom<NY, NX>;
-  using Io = Storage<St>;
   __shared__ float2 lds[G::kLds];
   __shared__ double red[16];
   float2* twx = lds + G::kScratch;
@@ -90,13 +89,17 @@ __global__ void __launch_bounds__((PlaneGeom<NY, NX>::kThreads))
   const int y = t / G::kTX, i = t % G::kTX;  // row layout
   const int x = t % NX, j = t / NX;          // column layout
   const long long plane = static_cast<long long>(start) * p.n + static_cast<long long>(tile) * NY * NX;
-  St* w = static_cast<St*>(p.w);
-  St* xs = static_cast<St*>(p.x);
-  const std::uint8_t* sup = p.support;
+  // The block's plane as buffer resources: a 32-bit lane offset plus compile-time
+  // per-register offsets (the column layout steps by kTY rows, beyond the reach of a global
+  // load's immediate offset), so no access needs 64-bit address arithmetic.
+  constexpr int kN = NY * NX;
+  const View<St, true> wrow(static_cast<St*>(p.w) + plane, kN, y * NX + i, G::kTX);
+  const View<St, true> xcol(static_cast<St*>(p.x) + plane, kN, j * NX + x, G::kTY * NX);
+  const View<const std::uint8_t, true> scol(p.support + plane, kN, j * NX + x, G::kTY * NX);
 
   float2 v[16];
 #pragma unroll
-  for (int m = 0; m < 16; ++m) v[m] = Io::ld(w, plane + static_cast<long long>(y) * NX + i + m * G::kTX);
+  for (int m = 0; m < 16; ++m) v[m] = vld(wrow, m);
   load_twiddles<NX>(twx);
   load_twiddles<NY>(twy);
   __syncthreads();
@@ -107,27 +110,25 @@ __global__ void __launch_bounds__((PlaneGeom<NY, NX>::kThreads))
   fft_line<NY, 16, false>(v, j, lds + x * G::kColStride, twy);
   // (the old iterate and the support are read here rather than prefetched: at 128 x 128
   // the 1024-thread block has 128 registers per thread, and holding them across the
-  // transforms would spill.  They are read kG elements at a time, unconditionally, so
-  // that 2 kG loads are in flight together.)
-  constexpr int kG = sizeof(St) == 4 ? 4 : 2;
+  // transforms would spill.  They are read eight elements at a time, unconditionally, so
+  // that sixteen loads are in flight together; sixteen at a time spills at 128 x 128.)
+  constexpr int kG = 8;
 #pragma unroll
   for (int h = 0; h < 16; h += kG) {
     float2 xo[kG];
     bool inside[kG];
 #pragma unroll
     for (int q = 0; q < kG; ++q) {
-      const long long at = plane + static_cast<long long>(j + (h + q) * G::kTY) * NX + x;
-      inside[q] = sup[at] != 0;
-      xo[q] = Io::ld(xs, at);
+      inside[q] = vld_flag(scol, h + q);
+      xo[q] = vld(xcol, h + q);
     }
 #pragma unroll
     for (int q = 0; q < kG; ++q) {
       const int m = h + q;
-      const long long at = plane + static_cast<long long>(j + m * G::kTY) * NX + x;
       const float2 pm = {v[m].x * p.in_scale, v[m].y * p.in_scale};
       float2 nx = pm;
       if (!inside[q]) nx = {p.a * xo[q].x + p.b * pm.x, p.a * xo[q].y + p.b * pm.y};
-      Io::st(xs, at, nx);
+      vst(xcol, m, nx);
       v[m] = nx;
     }
   }
@@ -136,10 +137,7 @@ __global__ void __launch_bounds__((PlaneGeom<NY, NX>::kThreads))
   cols_to_rows<NY, NX>(v, y, i, x, j, lds);
   fft_line<NX, 16, true>(v, i, lds + y * G::SX::kLine, twx);
 #pragma unroll
-  for (int m = 0; m < 16; ++m) {
-    Io::st(w, plane + static_cast<long long>(y) * NX + i + m * G::kTX,
-           float2{v[m].x * p.out_scale, v[m].y * p.out_scale});
-  }
+  for (int m = 0; m < 16; ++m) vst(wrow, m, float2{v[m].x * p.out_scale, v[m].y * p.out_scale});
 
   if (tile == 0 && t < 64 && p.errs) {
     double s = 0.0;

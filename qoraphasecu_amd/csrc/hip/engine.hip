@@ -235,6 +235,27 @@ void launch_pass(int N, bool col, Mode m, const PassArgs& a, int starts, hipStre
   }
 }
 
+// Error partials per start of a modulus pass: one per wave of every work-group.
+template <int N, bool COL, int EB, int R = 16>
+int pass_partials_t(int outer, int inner) {
+  return pass_tiles_t<N, COL, EB, R>(outer, inner) * dev::PassGeom<N, COL, EB, R>::kWaves;
+}
+
+template <int EB>
+int pass_partials(int N, bool col, int outer, int inner, int radix = 16) {
+  const bool r8 = radix_for(N, radix) == 8;
+  switch (N) {
+#define QPCH_CASE(V)                                                                                 \
+  case V:                                                                                            \
+    if (r8) return col ? pass_partials_t<V, true, EB, 8>(outer, inner) : pass_partials_t<V, false, EB, 8>(outer, inner); \
+    return col ? pass_partials_t<V, true, EB>(outer, inner) : pass_partials_t<V, false, EB>(outer, inner);
+    QPCH_FOR_EACH_N(QPCH_CASE)
+#undef QPCH_CASE
+    default:
+      throw InvalidArgument("no fused pass for axis length " + std::to_string(N));
+  }
+}
+
 template <int EB>
 int pass_tiles(int N, bool col, int outer, int inner, int radix = 16) {
   const bool r8 = radix_for(N, radix) == 8;
@@ -578,8 +599,8 @@ struct SolverImpl {
     p.amp = amp_signed.get();
     p.amp_stride = problems > 1 ? n : 0;
     p.amp_group = per_problem;
-    p.partial_stride = pass_tiles<sizeof(St)>(dims.d[0], dims.inner(0) > 1, static_cast<int>(dims.outer(0)),
-                                              static_cast<int>(dims.inner(0)), radix);
+    p.partial_stride = pass_partials<sizeof(St)>(dims.d[0], dims.inner(0) > 1, static_cast<int>(dims.outer(0)),
+                                                 static_cast<int>(dims.inner(0)), radix);
     p.partials = partials.get() + static_cast<long long>(first) * p.partial_stride;  // own rows per group
     p.errs = errs.get();
     p.err_stride = starts;
@@ -1005,7 +1026,8 @@ struct SolverImpl {
     int pb = grid_for(n, 256, 1024);
     if (fused) {
       const int o = static_cast<int>(dims.outer(0)), in = static_cast<int>(dims.inner(0));
-      pb = std::max({pb, pass_tiles<8>(dims.d[0], in > 1, o, in, radix), pass_tiles<4>(dims.d[0], in > 1, o, in, radix)});
+      pb = std::max({pb, pass_partials<8>(dims.d[0], in > 1, o, in, radix),
+                     pass_partials<4>(dims.d[0], in > 1, o, in, radix)});
     }
     partials.resize(static_cast<size_t>(pb) * starts);  // groups may run at once: rows of their own
 
@@ -1597,8 +1619,8 @@ int qpch_amplitude_loss(const void* x, const void* amplitude, long long amplitud
     if (precision == 0 && qpch::fusable(d)) {
       auto* w = static_cast<float2*>(scratch.get(sizeof(float2) * total));
       const int o0 = static_cast<int>(d.outer(0)), in0 = static_cast<int>(d.inner(0));
-      const int tiles = qpch::pass_tiles<8>(d.d[0], in0 > 1, o0, in0);
-      auto* partials = static_cast<double*>(scratch.get(sizeof(double) * tiles * batch));
+      const int parts = qpch::pass_partials<8>(d.d[0], in0 > 1, o0, in0);
+      auto* partials = static_cast<double*>(scratch.get(sizeof(double) * parts * batch));
       auto* counter = static_cast<int*>(scratch.get(sizeof(int)));
       auto pass = [&](int axis, qpch::dev::Mode mode, const void* src, float in_scale, float out_scale) {
         qpch::dev::PassArgs p{};
@@ -1612,7 +1634,7 @@ int qpch_amplitude_loss(const void* x, const void* amplitude, long long amplitud
         p.amp = static_cast<const float*>(amplitude);
         p.amp_stride = amplitude_stride;
         p.partials = partials;
-        p.partial_stride = tiles;
+        p.partial_stride = parts;
         p.counter = counter;
         p.in_scale = in_scale;
         p.out_scale = out_scale;
@@ -1624,7 +1646,7 @@ int qpch_amplitude_loss(const void* x, const void* amplitude, long long amplitud
       pass(0, qpch::dev::Mode::kModulus, nullptr, static_cast<float>(s), static_cast<float>(s));
       if (d.nd == 3) pass(1, qpch::dev::Mode::kInv, nullptr, 1.0f, 1.0f);
       pass(last, qpch::dev::Mode::kGrad, nullptr, 1.0f, 1.0f);
-      hipLaunchKernelGGL(qpch::dev::sum_partials, dim3((batch + 63) / 64), dim3(64), 0, st, partials, tiles, loss,
+      hipLaunchKernelGGL(qpch::dev::sum_partials, dim3((batch + 63) / 64), dim3(64), 0, st, partials, parts, loss,
                          batch);
       QPCH_HIP(hipGetLastError());
       return;

@@ -18,8 +18,8 @@
 //   kGrad        inverse -> P_M x, out = 2 (x - P_M x): the last pass of the amplitude
 //                loss gradient (no reference counterpart)
 // The error numerator of a start is reduced in a fixed order: the modulus pass writes one
-// FP64 partial per work-group, the next update pass (work-group 0 of each start) sums
-// them and stores errs[slot][start]; slot = *counter - 1, the modulus pass having
+// FP64 partial per wave, the next update pass (work-group 0 of each start) sums them in
+// index order and stores errs[slot][start]; slot = *counter - 1, the modulus pass having
 // advanced the counter.
 #pragma once
 
@@ -46,7 +46,7 @@ struct PassArgs {
   const float* amp;             // kModulus: signed amplitude [n], -1 = masked
   long long amp_stride;         // 0: one amplitude for every start; n: one per problem
   int amp_group;                // consecutive starts per problem (0 or 1: one per start)
-  double* partials;             // [start][partial_stride]
+  double* partials;             // [start][partial_stride]: modulus pass, one per wave
   int partial_stride;
   double* errs;                 // [slot][err_stride]
   int err_stride;
@@ -81,6 +81,7 @@ struct PassGeom {
   static constexpr int kColB = 128 / EB;
   static constexpr int kB = clampB(COL ? (256 / kT > kColB ? 256 / kT : kColB) : (256 / kT > 1 ? 256 / kT : 1));
   static constexpr int kThreads = kB * kT;
+  static constexpr int kWaves = (kThreads + 63) / 64;  // error partials per work-group
   // Column passes put consecutive lanes on different lines: an odd line stride (in float2)
   // spreads them over the LDS banks; a row pass's lanes walk along one line.
   static constexpr int kStride = COL ? (S::kLine | 1) : S::kLine;
@@ -178,6 +179,27 @@ __device__ __forceinline__ bool vld_flag(const View<const std::uint8_t, BUF>& v,
   }
 }
 
+// The modulus projection of a pass whose input and output carry scales: v is the unscaled
+// spectrum value and the projection acts on u = in * v, so |u| = in |v|; the result,
+// out * P(u), is v times one real factor (io = in * out, uniform), which saves the separate
+// scalings of u and of the output.  Semantics as project().
+__device__ __forceinline__ float2 project_scaled(float2 v, float a, float in, float io, double& acc) {
+  const float mag = in * __builtin_amdgcn_sqrtf(v.x * v.x + v.y * v.y);
+  const bool usable = a >= 0.0f;
+  const float d = usable ? mag - a : 0.0f;
+  acc += static_cast<double>(d) * static_cast<double>(d);
+  const float target = usable ? a : mag;
+  const float g = io * target * __builtin_amdgcn_rcpf(mag + 1e-12f);
+  return {v.x * g, v.y * g};
+}
+
+// Sum over the 64 lanes of a wave in a fixed order; the result is valid in lane 0.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
 // Block numbering: with p.slots > 0 (a multiple of 8) block L handles start L % slots and
 // tile L / slots, so every block of a start lands on the same XCD (blocks are dealt to the
 // eight XCDs round-robin) and the start's arrays stay in that XCD's L2 from one pass to the
@@ -189,8 +211,8 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St), R>::kThreads))
   constexpr int kT = G::kT;
   constexpr int kR = G::kR;
   constexpr int kB = G::kB;
+  constexpr int kWaves = G::kWaves;
   __shared__ float2 lds[G::kTile + N];
-  __shared__ double red[16];
   float2* tw = lds + G::kTile;
 
   int tile, start;
@@ -263,23 +285,21 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St), R>::kThreads))
     fft_line<N, R, true>(v, i, line, tw);
     double acc = 0.0;
     if (valid) {
+      const float io = p.in_scale * p.out_scale;
 #pragma unroll
-      for (int m = 0; m < kR; ++m) {
-        const float a = amp[m];
-        const float2 u = {v[m].x * p.in_scale, v[m].y * p.in_scale};
-        const float2 q = project(u, a, acc);
-        v[m] = {q.x * p.out_scale, q.y * p.out_scale};
-      }
+      for (int m = 0; m < kR; ++m) v[m] = project_scaled(v[m], amp[m], p.in_scale, io, acc);
     }
+    // Each wave's share of the error numerator goes out now, one partial per wave: its
+    // shuffles overlap the inverse transform, and no block-wide reduction ends the pass.
+    const double wsum = wave_sum(acc);
+    if ((tid & 63) == 0) {
+      p.partials[static_cast<long long>(start) * p.partial_stride + tile * kWaves + (tid >> 6)] = wsum;
+    }
+    if (tid == 0 && tile == 0 && start == 0) atomicAdd(p.counter, 1);
     fft_line<N, R, false>(v, i, line, tw);
     if (valid) {
 #pragma unroll
       for (int m = 0; m < kR; ++m) vst(vw, m, v[m]);
-    }
-    const double total = block_sum(acc, red);
-    if (tid == 0) {
-      p.partials[static_cast<long long>(start) * p.partial_stride + tile] = total;
-      if (tile == 0 && start == 0) atomicAdd(p.counter, 1);
     }
   } else if constexpr (M == Mode::kGrad) {
     const SV vx(static_cast<St*>(p.x) + soff, p.n, e0, em);

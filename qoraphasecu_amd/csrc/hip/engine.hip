@@ -968,9 +968,11 @@ struct SolverImpl {
       }
     }
     if (!p.amplitude || !p.mask || !p.support) throw InvalidArgument("amplitude, mask and support are required");
+    if (p.lanes < 0 || p.lanes > 8) throw InvalidArgument("lanes must be between 0 and 8");
+    if (p.plane < 0 || p.plane > 2) throw InvalidArgument("plane must be 0, 1 or 2");
     QPCH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     on = stream;
-    lanes = std::max(1, std::min(p.lanes, 8));
+    lanes = std::max(1, p.lanes);
     if (fused && lanes > 1 && groups > 1) {
       for (int l = 0; l < std::min(lanes, groups); ++l) {
         hipStream_t st{};

@@ -152,3 +152,11 @@ def test_low_precision_iteration(qpch, shape, radix, storage, tol, floor):
     got = R.run_abi(s, ops)[:, 0]
     s.close()
     assert np.all(np.abs(got - want) <= tol * want + floor), (got, want)
+
+
+@pytest.mark.parametrize("kw", [dict(radix=5), dict(xcd=1), dict(lanes=9), dict(lanes=-1), dict(plane=3)])
+def test_bad_launch_choices_are_rejected(qpch, kw):
+    """Launch choices outside their documented ranges are argument errors (QPCH_EINVAL)."""
+    obj, amp, mask, sup = _problem((64, 64), 16)
+    with pytest.raises(ValueError):
+        qpch.Solver(oracle.to_fft_order(amp), oracle.to_fft_order(mask), oracle.to_fft_order(sup), starts=2, **kw)

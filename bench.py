@@ -273,6 +273,60 @@ def cpu_full_run(w, threads):
                 starts=w["starts"], best_final_error=best)
 
 
+def time_next_rows(steps):
+    """The rows after the hot path: a non-power-of-two problem (240^2 = 2^4 3 5, 8 starts,
+    RAAR(50)) through the generic mixed-radix kernels, with the oracle's rate on the same
+    problem beside it; and the device make_problem (autocorrelation support) and
+    aligned_error at 256^2."""
+    import qoraphasecu_amd as qpc
+    from qoraphasecu_amd import _core, workloads
+
+    obj, inten, mask = workloads.problem_2d(240, seed=1000, photons=1e7, beamstop_radius=6, box=90)
+    ops = qpc.Recipe(qpc.RAAR(50)).ops()
+    res = {"workload": "240^2, 8 starts, RAAR(50), Poisson 1e7 photons, 6 px beamstop"}
+    for variant in ("graph", "batched"):
+        s = _core.Solver(inten, None, mask, 8, 1 if variant == "graph" else 0, 0, None, 0.1, "hip", variant, 10)
+        s.run(ops)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.run(ops)
+        wall = (time.perf_counter() - t0) / steps
+        res[variant] = dict(ms_per_run=wall * 1e3, start_iterations_per_s=8 * 50 / wall)
+    log(f"  240^2 (generic radix 2/3/5): graph {res['graph']['start_iterations_per_s']:.4g} start-it/s, "
+        f"batched {res['batched']['start_iterations_per_s']:.4g}")
+    obj2, inten2, mask2 = workloads.problem_2d(256, seed=1000, photons=1e7, beamstop_radius=6)
+    _core.make_problem(inten2, None, mask2, 0.1)
+    t0 = time.perf_counter()
+    for _ in range(20):
+        _core.make_problem(inten2, None, mask2, 0.1)
+    res["make_problem_256_ms"] = (time.perf_counter() - t0) / 20 * 1e3
+    a = obj2 * np.exp(0.3j)
+    qpc.aligned_error(a, obj2)
+    t0 = time.perf_counter()
+    for _ in range(20):
+        qpc.aligned_error(a, obj2)
+    res["aligned_error_256_ms"] = (time.perf_counter() - t0) / 20 * 1e3
+    log(f"  make_problem 256^2: {res['make_problem_256_ms']:.2f} ms; aligned_error 256^2: "
+        f"{res['aligned_error_256_ms']:.2f} ms")
+    return res
+
+
+def cpu_next_rows(threads):
+    """The oracle on the 240^2 problem of time_next_rows (RAAR(5) on all 8 starts)."""
+    import oracle
+    from qoraphasecu_amd import workloads
+
+    oracle.set_threads(threads)
+    obj, inten, mask = workloads.problem_2d(240, seed=1000, photons=1e7, beamstop_radius=6, box=90)
+    amp, m, sup = oracle.make_problem(inten, None, mask)
+    solver = oracle.Solver(amp, m, sup, starts=8, seed=0)
+    solver.run([("raar", 1, 0.9, 0.0, 0.0, "max", 0)])
+    t0 = time.perf_counter()
+    solver.run([("raar", 5, 0.9, 0.0, 0.0, "max", 0)])
+    wall = time.perf_counter() - t0
+    return dict(value=8 * 5 / wall, threads=threads)
+
+
 def traffic_for(kernel_key):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, when present."""
     path = os.path.join(HERE, "profiles", "traffic.json")
@@ -289,7 +343,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--configs", default="c2,c4,c3,c5,c1")
+    ap.add_argument("--configs", default="c2,c4,c3,c5,c1,next")
     ap.add_argument("--variants", default="graph,mixed,mixed_bf16")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-passes", action="store_true", help="skip the per-pass timing")
@@ -323,6 +377,13 @@ def main():
     for name in [c for c in args.configs.split(",") if c]:
         if name == "c5":
             details["c5"] = time_refinement()
+            continue
+        if name == "next":
+            res = time_next_rows(args.steps)
+            if not args.no_cpu:
+                res["cpu_oracle_240"] = cpu_next_rows(min(16, os.cpu_count() or 1))
+                log(f"  240^2 cpu oracle: {res['cpu_oracle_240']['value']:.4g} start-it/s")
+            details["next"] = res
             continue
         w = workload(name)
         if name == "c1":
@@ -414,6 +475,9 @@ def main():
                 "avg_launch_ms": dom["ms"],
             }
 
+    from qoraphasecu_amd import _qpch
+
+    details["autotune_log"] = _qpch.autotune_log().splitlines()
     if headline is None:
         print(json.dumps({"details": details}))
         return

@@ -272,6 +272,9 @@ class HipEnsemble final : public MultiSolver {
       q.mixed = plan.mixed;
       q.schedule = QPCH_GRAPH;
       q.problems = static_cast<int>(members.size());
+      // a member's launch groups run one after another on its branch of the group's graph
+      // (the shapes already run side by side), so concurrent lanes are not on offer
+      q.lanes = 1;
       tune(q);
       qpch_check(qpch_solver_create(&q, &c.h));
       c.energies.resize(members.size());

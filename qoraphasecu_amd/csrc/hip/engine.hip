@@ -1509,9 +1509,8 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
     if (problem->batch > 0) {
       batches.push_back(problem->batch);  // fixed by the caller: tune the rest
     } else {
-      for (int b = total; b >= 1 && batches.size() < 4; b /= 2) {
+      for (int b = total; b >= 1 && batches.size() < 4; b /= 2) {  // all, 1/2, 1/4, 1/8
         if (total % b == 0) batches.push_back(b);
-        if (b < 8) break;
       }
     }
     std::vector<int> planes{0};

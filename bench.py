@@ -174,17 +174,20 @@ def pass_roofline(w, storage, reps=50):
     return out
 
 
-def cpu_baseline(w, threads):
-    """The oracle (float, OpenMP) on a bounded sample: max(5, iters/20) RAAR iterations for
-    every start of the workload, after one untimed iteration (bench/bench.cpp:148)."""
+def cpu_baseline(w, threads, seconds=10.0):
+    """The oracle (float, OpenMP) on a bounded sample of the workload: RAAR iterations on
+    every start, as many as fill about `seconds` of CPU time (at least the reference's
+    max(5, iters/20), bench/bench.cpp:148), sized from one timed warm-up iteration."""
     import oracle
 
     oracle.set_threads(threads)
-    iters = max(5, w["recipe"].iterations() // 20)
     amp, mask, sup = oracle.make_problem(w["intensity"], None, w["mask"])
     starts = w["starts"]
     solver = oracle.Solver(amp, mask, sup, starts=starts, seed=0)
+    t0 = time.perf_counter()
     solver.run([("raar", 1, 0.9, 0.0, 0.0, "max", 0)])
+    one = time.perf_counter() - t0
+    iters = max(5, w["recipe"].iterations() // 20, int(seconds / max(one, 1e-6)))
     t0 = time.perf_counter()
     solver.run([("raar", iters, 0.9, 0.0, 0.0, "max", 0)])
     wall = time.perf_counter() - t0

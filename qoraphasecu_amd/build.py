@@ -91,12 +91,26 @@ def build_core() -> str:
     return out
 
 
+def cpp_test() -> str:
+    return os.path.join(ROOT, "tests", "cpp", "test_hip_backend")
+
+
+def build_cpp_test() -> str:
+    """The C++ API test program (tests/cpp/test_hip_backend.cpp), linked to the host library."""
+    out = cpp_test()
+    tmp = out + ".tmp"
+    _run([CXX, "-O2", "-std=c++20", "-Wall", f"-I{INCLUDE}", os.path.join(ROOT, "tests", "cpp", "test_hip_backend.cpp"),
+          "-o", tmp, f"-L{HERE}", "-lqoraphasecu", "-Wl,-rpath,$ORIGIN/../../qoraphasecu_amd"])
+    os.replace(tmp, out)
+    return out
+
+
 def build() -> list[str]:
-    """Compile the engine and the host library in parallel, then the Python module."""
+    """Compile the engine, then the host library, the Python module and the C++ API test."""
     with ThreadPoolExecutor(max_workers=1) as pool:
         eng = pool.submit(build_engine)
         eng.result()
-    built = [engine_lib(), build_host(), build_core()]
+    built = [engine_lib(), build_host(), build_core(), build_cpp_test()]
     return built
 
 

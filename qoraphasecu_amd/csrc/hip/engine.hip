@@ -1629,7 +1629,6 @@ int qpch_amplitude_loss(const void* x, const void* amplitude, long long amplitud
       const int o0 = static_cast<int>(d.outer(0)), in0 = static_cast<int>(d.inner(0));
       const int parts = qpch::pass_partials<8>(d.d[0], in0 > 1, o0, in0);
       auto* partials = static_cast<double*>(scratch.get(sizeof(double) * parts * batch));
-      auto* counter = static_cast<int*>(scratch.get(sizeof(int)));
       auto pass = [&](int axis, qpch::dev::Mode mode, const void* src, float in_scale, float out_scale) {
         qpch::dev::PassArgs p{};
         p.outer = static_cast<int>(d.outer(axis));
@@ -1643,7 +1642,6 @@ int qpch_amplitude_loss(const void* x, const void* amplitude, long long amplitud
         p.amp_stride = amplitude_stride;
         p.partials = partials;
         p.partial_stride = parts;
-        p.counter = counter;
         p.in_scale = in_scale;
         p.out_scale = out_scale;
         qpch::launch_pass<float2>(d.d[axis], p.inner > 1, mode, p, batch, st);

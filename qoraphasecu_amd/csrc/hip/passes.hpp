@@ -52,7 +52,7 @@ struct PassArgs {
   int err_stride;
   int err_offset;               // global index of this launch's first start
   int err_cap;                  // slots available
-  int* counter;
+  int* counter;                 // error slot counter (null: no history, as in the loss)
   int starts;                   // starts in this launch
   int slots;                    // > 0: XCD-affine grid, block = tile * slots + start
   int no_xcd;                   // launcher input: 1 keeps the plain block numbering
@@ -297,7 +297,7 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St), R>::kThreads))
     if ((tid & 63) == 0) {
       p.partials[static_cast<long long>(start) * p.partial_stride + tile * kWaves + (tid >> 6)] = wsum;
     }
-    if (tid == 0 && tile == 0 && start == 0) atomicAdd(p.counter, 1);
+    if (tid == 0 && tile == 0 && start == 0 && p.counter) atomicAdd(p.counter, 1);
     fft_line<N, R, false>(v, i, line, tw);
     if (valid) {
 #pragma unroll

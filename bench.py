@@ -151,6 +151,36 @@ def time_variant(w, variant, batch, steps, warmup):
                 best_final_error=min(final), final_errors=final)
 
 
+def time_plain_radix8(w, steps):
+    """The plain fp32 path (one start per launch, graph-replayed) at radix 8: faster than the
+    default radix 16 for one-start launches of 256-point lines, but a different rounding,
+    so the API keeps radix 16 for every variant (a start's result must not depend on the
+    launch split).  RAAR(0.9) iterations of the recipe's count, without Shrinkwrap."""
+    from qoraphasecu_amd import _core, _qpch
+
+    amp, mask, sup = _core.make_problem(w["intensity"], None, w["mask"], 0.1)
+    f = np.fft.ifftshift
+    iters = w["recipe"].iterations() // 10 * 10
+    s = _qpch.Solver(f(amp).copy(), f(mask).copy(), f(sup).copy(), starts=w["starts"], batch=1, radix=8,
+                     schedule="graph")
+
+    def run():
+        s.begin_run(iters)
+        for _ in range(iters // 10):
+            s.iterate(10, 0.9, -0.8, False)
+        s.synchronize()
+
+    run()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    wall = (time.perf_counter() - t0) / steps
+    s.close()
+    rate = w["starts"] * iters / wall
+    log(f"  {w['name']} plain fp32 radix 8: {wall * 1e3:9.2f} ms/run  {rate:12.4g} start-it/s  (RAAR only)")
+    return dict(ms_per_run=wall * 1e3, start_iterations_per_s=rate, iterations=iters)
+
+
 def pass_roofline(w, storage, reps=50):
     """Average duration of each pass of one batched iteration, HIP events on the solver stream."""
     from qoraphasecu_amd import _core, _qpch
@@ -428,6 +458,11 @@ def main():
             for v in ("mixed", "mixed_bf16", "half", "bf16", "batched"):
                 if v in res:
                     res[f"speedup_{v}_vs_plain"] = res[v]["start_iterations_per_s"] / res["graph"]["start_iterations_per_s"]
+            if len(w["shape"]) == 2:
+                res["graph_radix8"] = time_plain_radix8(w, args.steps)
+                if "mixed" in res:
+                    res["speedup_mixed_vs_plain_radix8"] = (res["mixed"]["start_iterations_per_s"]
+                                                           / res["graph_radix8"]["start_iterations_per_s"])
         n = int(np.prod(w["shape"]))
         for v in ("mixed", "mixed_bf16", "batched", "graph"):
             if v not in res:

@@ -721,6 +721,47 @@ struct SolverImpl {
     QPCH_HIP(hipGraphLaunch(it->second, on));
   }
 
+  // Launch groups side by side in one graph per chunk: a branch per lane, forked from and
+  // joined to the solver's stream inside the capture, so one launch starts every group.
+  template <class St>
+  void lanes_chunk(int L, int len, float a, float b, bool need_prologue) {
+    if (need_prologue) {
+      for (int g = 0; g < groups; ++g) fused_prologue<St>(g);
+    }
+    const auto key = std::make_tuple(-L, len, a, b, !std::is_same_v<St, float2>);
+    auto it = graphs.find(key);
+    if (it == graphs.end()) {
+      QPCH_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      hipGraph_t graph{};
+      try {
+        QPCH_HIP(hipEventRecord(lane_fork, stream));
+        for (int l = 0; l < L; ++l) QPCH_HIP(hipStreamWaitEvent(lane_streams[l], lane_fork, 0));
+        for (int g = 0; g < groups; ++g) {
+          on = lane_streams[g % L];
+          fused_iterations<St>(g, len, a, b);
+        }
+        on = stream;
+        for (int l = 0; l < L; ++l) {
+          QPCH_HIP(hipEventRecord(lane_done[l], lane_streams[l]));
+          QPCH_HIP(hipStreamWaitEvent(stream, lane_done[l], 0));
+        }
+      } catch (...) {
+        on = stream;
+        hipGraph_t dummy{};
+        (void)hipStreamEndCapture(stream, &dummy);
+        if (dummy) (void)hipGraphDestroy(dummy);
+        throw;
+      }
+      QPCH_HIP(hipStreamEndCapture(stream, &graph));
+      hipGraphExec_t exec{};
+      const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      QPCH_HIP(e);
+      it = graphs.emplace(key, exec).first;
+    }
+    QPCH_HIP(hipGraphLaunch(it->second, stream));
+  }
+
   // --- group support: the precision switch and prologue run eagerly; the chunk's passes
   // for every launch group are then enqueued on `stream` (inside a capture by the group).
   bool prepare(bool refine) {
@@ -754,6 +795,17 @@ struct SolverImpl {
     set_precision(lowp);
     const bool need_prologue = !wave_valid;
     const int L = fused ? std::min<int>(static_cast<int>(lane_streams.size()), groups) : 0;
+    if (L > 1 && schedule == QPCH_GRAPH) {
+      if (!lowp) {
+        lanes_chunk<float2>(L, len, a, b, need_prologue);
+      } else if (storage == QPCH_F16) {
+        lanes_chunk<__half2>(L, len, a, b, need_prologue);
+      } else {
+        lanes_chunk<__hip_bfloat162>(L, len, a, b, need_prologue);
+      }
+      wave_valid = true;
+      return;
+    }
     if (L > 1) {
       // launch groups run side by side on the lanes, forked from and joined to the stream
       QPCH_HIP(hipEventRecord(lane_fork, stream));
@@ -1515,11 +1567,10 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
     }
     std::vector<int> planes{0};
     if (d.nd == 3 && qpch::plane_supported(d.d[1], d.d[2])) planes = {1, 2};
-    bool r8 = false;
-    for (int a = 0; a < d.nd; ++a) r8 = r8 || qpch::radix_for(d.d[a], 8) == 8;
-    std::vector<int> radices{16};
-    if (r8 && problem->radix == 0) radices.push_back(8);
-    if (problem->radix != 0) radices = {problem->radix};
+    // The radix is not tuned: it changes the arithmetic (rounding), and a start's result
+    // must not depend on how the starts are split into launches (batched == looped).  A
+    // caller may still fix it in problem->radix.
+    const std::vector<int> radices{problem->radix != 0 ? problem->radix : 16};
     std::string line = "qpch autotune " + key + ":";
     qpch_tuning best{};
     best.us_per_iteration = 1e300;

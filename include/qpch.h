@@ -153,8 +153,9 @@ int qpch_solver_pass_info(qpch_solver* solver, int low, int* axes, int* modes, i
  * when QPCH_TUNE_LOG=1).  No reference counterpart: the reference leaves Options::batch
  * to the user (include/qoraphasecu/solver.hpp:44-46).  A problem->batch > 0 is kept and
  * only the other choices are tuned; problem->lanes == 1 keeps launch groups sequential.  The
- * radix is not tuned (it changes rounding, and results must not depend on the split of the
- * starts into launches): problem->radix, 16 when 0. */
+ * radix and the 3D plane / line choice are not tuned (they change rounding, and results must
+ * not depend on the split of the starts into launches): problem->radix (16 when 0) and
+ * problem->plane (the plane pass where supported when 0) are kept. */
 int qpch_autotune(const qpch_problem* problem, qpch_tuning* out);
 const char* qpch_autotune_log(void);
 

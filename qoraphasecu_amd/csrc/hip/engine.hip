@@ -1565,8 +1565,11 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
         if (total % b == 0) batches.push_back(b);
       }
     }
-    std::vector<int> planes{0};
-    if (d.nd == 3 && qpch::plane_supported(d.d[1], d.d[2])) planes = {1, 2};
+    // Plane vs line passes (3D) changes the order of the arithmetic, so like the radix it
+    // is not tuned: the plane pass where supported (it was the faster one in every
+    // measurement), unless the caller asks for line passes.
+    const bool plane_ok = d.nd == 3 && qpch::plane_supported(d.d[1], d.d[2]);
+    const std::vector<int> planes{problem->plane != 0 ? problem->plane : (plane_ok ? 1 : 0)};
     // The radix is not tuned: it changes the arithmetic (rounding), and a start's result
     // must not depend on how the starts are split into launches (batched == looped).  A
     // caller may still fix it in problem->radix.

@@ -7,8 +7,8 @@ the slow side).  With QPCH_REPORT=<file> the numbers are written as JSON.
 HIO / RAAR with Shrinkwrap and detwin is chaotic: from the same start, two runs that differ
 only in rounding separate after some iterations and end in different local minima.  The
 yardstick is the oracle's own float run against its double run: the device fp32 paths
-must follow the FP64 error history (to 1e-3 relative) about as long as the CPU float
-reference does, the fp16 / bf16-storage paths must agree with it over the first
+must follow the FP64 error history (to 1e-3 relative) for at least 80% as many iterations
+as the CPU float reference does, the fp16 / bf16-storage paths must agree with it over the first
 iterations to their storage precision, and where the dynamics converge (the noiseless
 128^3 volume) the final error must agree with the FP64 run's."""
 
@@ -92,7 +92,9 @@ def test_against_fp64_oracle(name, starts, variants, final_bar):
         if low:
             assert early <= LOW_TOL, (variant, early)
         else:
-            assert min(track) >= min(yard_track) - 2, (variant, track, yard_track)
+            # the iterations to divergence grow with the log of the per-iteration rounding:
+            # 80% of the CPU float reference's count allows a few times its rounding error
+            assert min(track) >= 0.8 * min(yard_track), (variant, track, yard_track)
         if final_bar is not None:
             assert abs(best_gpu - best_ref) <= final_bar[variant], (variant, best_gpu, best_ref)
     _REPORT[name] = rep

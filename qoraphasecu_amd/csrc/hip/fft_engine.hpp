@@ -62,50 +62,79 @@ __device__ __forceinline__ constexpr float cos16(int t) {
 }
 __device__ __forceinline__ constexpr float sin16(int t) { return cos16(t + 12); }
 
-// v * W_R^t with W_R = exp(-+ 2 pi i / R), R a power of two <= 16.
-template <int R, bool FWD>
-__device__ __forceinline__ float2 twiddle_const(float2 v, int t) {
-  const int u = (t * (16 / R)) & 15;  // index in sixteenths of a turn
-  if (u == 0) return v;
-  if (u == 4) return FWD ? mul_mi(v) : mul_pi(v);
-  if (u == 8) return {-v.x, -v.y};
-  if (u == 12) return FWD ? mul_pi(v) : mul_mi(v);
-  const float c = cos16(u);
-  const float s = FWD ? -sin16(u) : sin16(u);
-  return {v.x * c - v.y * s, v.x * s + v.y * c};
+template <int R>
+__host__ __device__ constexpr int bitrev(int k) {
+  int r = 0;
+  for (int b = 0; b < ilog2(R); ++b) r |= ((k >> b) & 1) << (ilog2(R) - 1 - b);
+  return r;
 }
 
-// In-register DFT of size R (power of two <= 16): natural order in, natural order out.
-// Radix-2 decimation in frequency; the bit reversal is a compile-time renaming.
-template <int R, bool FWD>
-__device__ __forceinline__ void dft_regs(float2* v) {
+// The butterfly (a, b) -> (a + w b, a - w b) with w = (c, s).  a + w b takes four FMAs
+// (each product lands in one), and a - w b = 2a - (a + w b) one more per component: six
+// instructions where a separate product and sum take eight.
+__device__ __forceinline__ void bfly_w(float2& a, float2& b, float c, float s) {
+  const float2 y0 = {fmaf(b.x, c, fmaf(-b.y, s, a.x)), fmaf(b.x, s, fmaf(b.y, c, a.y))};
+  b = {fmaf(2.0f, a.x, -y0.x), fmaf(2.0f, a.y, -y0.y)};
+  a = y0;
+}
+
+// In-register DFT of size R (power of two <= 16) with W_R = exp(-+ 2 pi i / R): natural
+// order in and out.  Radix-2 decimation in time over the bit-reversed input (the bit
+// reversal is a compile-time renaming), so every twiddle multiplies a butterfly's second
+// input and fuses into it (bfly_w).  With TW the inputs are first multiplied by w[q]
+// (conjugated for the inverse), q >= 1, input 0 untwiddled: the products of the first
+// stage's second inputs fuse the same way.
+template <int R, bool FWD, bool TW = false>
+__device__ __forceinline__ void dft_regs(float2* v, const float2* w = nullptr) {
   if constexpr (R == 1) {
     return;
   } else {
+    float2 u[R];
 #pragma unroll
-    for (int span = R / 2; span >= 1; span /= 2) {
+    for (int k = 0; k < R; ++k) u[k] = v[bitrev<R>(k)];
+    // span 1: the pairs (v[q], v[q + R/2]), q = bitrev(k)
+#pragma unroll
+    for (int k = 0; k < R; k += 2) {
+      float2 a = u[k], b = u[k + 1];
+      if constexpr (TW) {
+        const int q = bitrev<R>(k);
+        if (q != 0) a = FWD ? cmul(a, w[q]) : cmulc(a, w[q]);
+        const float2 t = w[q + R / 2];
+        bfly_w(a, b, t.x, FWD ? t.y : -t.y);
+      } else {
+        const float2 y0 = cadd(a, b);
+        b = csub(a, b);
+        a = y0;
+      }
+      u[k] = a;
+      u[k + 1] = b;
+    }
+#pragma unroll
+    for (int span = 2; span < R; span *= 2) {
 #pragma unroll
       for (int start = 0; start < R; start += 2 * span) {
 #pragma unroll
         for (int j = 0; j < span; ++j) {
-          const float2 a = v[start + j];
-          const float2 b = v[start + j + span];
-          v[start + j] = cadd(a, b);
-          // twiddle W_{2 span}^j = W_R^{j R / (2 span)}
-          v[start + j + span] = twiddle_const<R, FWD>(csub(a, b), j * (R / (2 * span)));
+          // twiddle W_{2 span}^j, in sixteenths of a turn
+          const int t = j * (16 / (2 * span));
+          float2& a = u[start + j];
+          float2& b = u[start + j + span];
+          if (t == 0) {
+            const float2 y0 = cadd(a, b);
+            b = csub(a, b);
+            a = y0;
+          } else if (t == 4) {
+            const float2 wb = FWD ? mul_mi(b) : mul_pi(b);
+            b = csub(a, wb);
+            a = cadd(a, wb);
+          } else {
+            bfly_w(a, b, cos16(t), FWD ? -sin16(t) : sin16(t));
+          }
         }
       }
     }
-    float2 t[R];
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
-      int r = 0;
-#pragma unroll
-      for (int b = 0; b < ilog2(R); ++b) r |= ((k >> b) & 1) << (ilog2(R) - 1 - b);
-      t[k] = v[r];
-    }
-#pragma unroll
-    for (int k = 0; k < R; ++k) v[k] = t[k];
+    for (int k = 0; k < R; ++k) v[k] = u[k];
   }
 }
 
@@ -153,15 +182,15 @@ __device__ __forceinline__ void fft_line(float2* v, int i, float2* line, const f
   for (int s = 0; s < S::kFull; ++s) {
     const int k = i & (L - 1);
     if (s > 0) {
-      // twiddle x_q by W_{L R}^{q k} = W_N^{q k N / (L R)}
+      // x_q twiddled by W_{L R}^{q k} = W_N^{q k N / (L R)}, inside the DFT's first stage
       const int step = k * (N / (L * kR));
+      float2 w[kR];
 #pragma unroll
-      for (int q = 1; q < kR; ++q) {
-        const float2 w = tw[q * step];
-        v[q] = FWD ? cmul(v[q], w) : cmulc(v[q], w);
-      }
+      for (int q = 1; q < kR; ++q) w[q] = tw[q * step];
+      dft_regs<kR, FWD, true>(v, w);
+    } else {
+      dft_regs<kR, FWD>(v);
     }
-    dft_regs<kR, FWD>(v);
     const bool last = (s == S::kFull - 1) && S::kR2 == 1;
     if (!last) {
       const int base = (i - k) * kR + k;
@@ -182,16 +211,14 @@ __device__ __forceinline__ void fft_line(float2* v, int i, float2* line, const f
 #pragma unroll
     for (int j = 0; j < kG; ++j) {
       float2 u[kR2];
+      float2 w[kR2];
       const int ip = i + j * kT;
 #pragma unroll
       for (int q = 0; q < kR2; ++q) {
         u[q] = v[j + q * kG];
-        if (q > 0) {
-          const float2 w = tw[q * ip];
-          u[q] = FWD ? cmul(u[q], w) : cmulc(u[q], w);
-        }
+        if (q > 0) w[q] = tw[q * ip];
       }
-      dft_regs<kR2, FWD>(u);
+      dft_regs<kR2, FWD, true>(u, w);
 #pragma unroll
       for (int q = 0; q < kR2; ++q) v[j + q * kG] = u[q];
     }

@@ -10,7 +10,10 @@ yardstick is the oracle's own float run against its double run: the device fp32 
 must follow the FP64 error history (to 1e-3 relative) for at least 80% as many iterations
 as the CPU float reference does, the fp16 / bf16-storage paths must agree with it over the first
 iterations to their storage precision, and where the dynamics converge (the noiseless
-128^3 volume) the final error must agree with the FP64 run's."""
+128^3 volume) the final error must be as close to the FP64 run's as the CPU float run's is:
+within twice that distance, and never required closer than a floor (1e-4 for fp32 storage,
+1e-3 for fp16 / bf16).  At 128^3 the CPU float run itself ends 4.6e-4 from the FP64 run
+(final errors 0.02919 and 0.02873)."""
 
 import json
 import os
@@ -22,7 +25,8 @@ import oracle
 
 pytestmark = pytest.mark.gpu
 
-# (config, starts, variants, final-error bar (absolute) where the run converges, or None)
+# (config, starts, variants, floor of the final-error bar (absolute) where the run converges,
+# or None)
 CASES = [("c1", 8, ("graph", "batched", "mixed"), None),
          ("c2", 4, ("graph", "mixed", "mixed_bf16"), None),
          ("c4", 1, ("graph", "mixed", "mixed_bf16"), {"graph": 1e-4, "mixed": 1e-3, "mixed_bf16": 1e-3})]
@@ -62,10 +66,12 @@ def test_against_fp64_oracle(name, starts, variants, final_bar):
     best_ref = min(float(h[-1]) for h in ref_hist)
     yard = oracle.Solver(amp, mask, sup, starts=starts, seed=0, prec="f")
     yard.run(w["recipe"].ops())
-    yard_track = [_tracking(r[2], h) for r, h in zip(yard.results(), ref_hist)]
+    yard_res = yard.results()
+    yard_track = [_tracking(r[2], h) for r, h in zip(yard_res, ref_hist)]
+    yard_final = abs(min(float(r[2][-1]) for r in yard_res) - best_ref)
     rep = {"starts": starts, "iterations": w["recipe"].iterations(),
            "fp64_final_errors": [float(h[-1]) for h in ref_hist], "fp64_best_final_error": best_ref,
-           "oracle_float_iterations_within_1e-3": yard_track}
+           "oracle_float_iterations_within_1e-3": yard_track, "oracle_float_best_final_abs_diff": yard_final}
     print(f"{name}: the oracle's float run tracks its double run to 1e-3 for {min(yard_track)}-{max(yard_track)} "
           f"iterations")
     for variant in variants:
@@ -96,5 +102,6 @@ def test_against_fp64_oracle(name, starts, variants, final_bar):
             # 80% of the CPU float reference's count allows a few times its rounding error
             assert min(track) >= 0.8 * min(yard_track), (variant, track, yard_track)
         if final_bar is not None:
-            assert abs(best_gpu - best_ref) <= final_bar[variant], (variant, best_gpu, best_ref)
+            bar = max(final_bar[variant], 2.0 * yard_final)
+            assert abs(best_gpu - best_ref) <= bar, (variant, best_gpu, best_ref, yard_final)
     _REPORT[name] = rep

@@ -156,11 +156,8 @@ __global__ void __launch_bounds__(256)
     }
   }
   if (blockIdx.x == 0 && threadIdx.x < 64 && errs) {
-    double s = 0.0;
-    const double* part = partials + static_cast<long long>(blockIdx.y) * partial_stride;
-    for (int k = threadIdx.x; k < partial_stride; k += 64) s += part[k];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    const double s = partial_sum(partials + static_cast<long long>(blockIdx.y) * partial_stride,
+                                 partial_stride, threadIdx.x);
     if (threadIdx.x == 0) {
       const int slot = *counter - 1;
       if (slot >= 0 && slot < err_cap) {

@@ -202,6 +202,25 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// The error numerator of one start from its partials, in a fixed order: lane l adds
+// part[l], part[l + 64], ... in turn, then the wave's shuffle tree; valid in lane 0.  The
+// loads go out eight at a time, so a long row (2048 partials per start at 128^3) costs a
+// few round trips rather than one per element, with the same order of additions.
+__device__ __forceinline__ double partial_sum(const double* part, int count, int lane) {
+  constexpr int kU = 8;
+  double s = 0.0;
+  int k = lane;
+  for (; k + (kU - 1) * 64 < count; k += kU * 64) {
+    double a[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) a[u] = part[k + u * 64];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) s += a[u];
+  }
+  for (; k < count; k += 64) s += part[k];
+  return wave_sum(s);
+}
+
 // Block numbering: with p.slots > 0 (a multiple of 8) block L handles start L % slots and
 // tile L / slots, so every block of a start lands on the same XCD (blocks are dealt to the
 // eight XCDs round-robin) and the start's arrays stay in that XCD's L2 from one pass to the
@@ -354,11 +373,8 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St), R>::kThreads))
     }
     if (tile == 0 && tid < 64 && p.errs) {
       // fixed-order sum of the previous modulus pass's partials of this start
-      double s = 0.0;
-      const double* part = p.partials + static_cast<long long>(start) * p.partial_stride;
-      for (int k = tid; k < p.partial_stride; k += 64) s += part[k];
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+      const double s = partial_sum(p.partials + static_cast<long long>(start) * p.partial_stride,
+                                   p.partial_stride, tid);
       if (tid == 0) {
         const int slot = *p.counter - 1;
         if (slot >= 0 && slot < p.err_cap) {

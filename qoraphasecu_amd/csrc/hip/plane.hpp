@@ -71,7 +71,6 @@ __global__ void __launch_bounds__((PlaneGeom<NY, NX>::kThreads))
     plane_update(PassArgs p) {
   using G = PlaneGeom<NY, NX>;
   __shared__ float2 lds[G::kLds];
-  __shared__ double red[16];
   float2* twx = lds + G::kScratch;
   float2* twy = twx + NX;
 
@@ -108,10 +107,11 @@ __global__ void __launch_bounds__((PlaneGeom<NY, NX>::kThreads))
   fft_line<NX, 16, false>(v, i, lds + y * G::SX::kLine, twx);
   rows_to_cols<NY, NX>(v, y, i, x, j, lds);
   fft_line<NY, 16, false>(v, j, lds + x * G::kColStride, twy);
-  // (the old iterate and the support are read here rather than prefetched: at 128 x 128
-  // the 1024-thread block has 128 registers per thread, and holding them across the
-  // transforms would spill.  They are read eight elements at a time, unconditionally, so
-  // that sixteen loads are in flight together; sixteen at a time spills at 128 x 128.)
+  // (the old iterate and the support are read here rather than before the transforms: at
+  // 128 x 128 the 1024-thread block has 128 registers per thread, holding them across the
+  // transforms spills with fp32 storage, and with fp16 storage it measured no faster.  They
+  // are read eight elements at a time, unconditionally, so that sixteen loads are in flight
+  // together; sixteen at a time spills at 128 x 128.)
   constexpr int kG = 8;
 #pragma unroll
   for (int h = 0; h < 16; h += kG) {
@@ -140,11 +140,8 @@ __global__ void __launch_bounds__((PlaneGeom<NY, NX>::kThreads))
   for (int m = 0; m < 16; ++m) vst(wrow, m, float2{v[m].x * p.out_scale, v[m].y * p.out_scale});
 
   if (tile == 0 && t < 64 && p.errs) {
-    double s = 0.0;
-    const double* part = p.partials + static_cast<long long>(start) * p.partial_stride;
-    for (int k = t; k < p.partial_stride; k += 64) s += part[k];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    const double s = partial_sum(p.partials + static_cast<long long>(start) * p.partial_stride,
+                                 p.partial_stride, t);
     if (t == 0) {
       const int slot = *p.counter - 1;
       if (slot >= 0 && slot < p.err_cap) {
@@ -152,7 +149,6 @@ __global__ void __launch_bounds__((PlaneGeom<NY, NX>::kThreads))
       }
     }
   }
-  (void)red;
 }
 
 }  // namespace qpch::dev

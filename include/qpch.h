@@ -147,7 +147,9 @@ int qpch_solver_pass_info(qpch_solver* solver, int low, int* axes, int* modes, i
 /* --- autotuner --------------------------------------------------------------------------
  * Times candidate launch configurations of `problem` (starts per launch, 3D plane or line
  * passes, XCD-affine or plain block numbering) on trial solvers with random starts, a few
- * graph-replayed iterations each, and returns the fastest.  The configurations give the
+ * graph-replayed iterations each, and returns the fastest (a candidate replaces the best so
+ * far only when it is 2% faster, the first candidate being the default: all starts in one
+ * launch, one lane, XCD-affine numbering).  The configurations give the
  * same results; only the time differs.  Cached per process by shape, total starts and
  * storage; qpch_autotune_log() returns one line per tuned key (also printed to stderr
  * when QPCH_TUNE_LOG=1).  No reference counterpart: the reference leaves Options::batch

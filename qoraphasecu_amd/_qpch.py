@@ -246,7 +246,7 @@ class Solver:
         p.storage = STORAGE[storage]
         p.mixed = int(mixed)
         p.schedule = SCHEDULE[schedule]
-        if tuned:
+        if tuned and os.environ.get("QPCH_TUNE") != "0":  # QPCH_TUNE=0: the defaults, as in the C++ host
             t = QpchTuning()
             check(lib().qpch_autotune(ctypes.byref(p), ctypes.byref(t)))
             p.batch, p.plane, p.xcd, p.lanes, p.radix = t.batch, t.plane, t.xcd, t.lanes, t.radix

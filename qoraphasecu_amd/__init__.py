@@ -29,6 +29,7 @@ from .api import (
     reconstruct_many,
     simulate_intensity,
 )
+from .interop import from_qoraphase
 from .recipe import ER, HIO, RAAR, Recipe, Shrinkwrap, default_recipe, detwin, repeat
 
 __all__ = [
@@ -44,6 +45,7 @@ __all__ = [
     "cuda_available",
     "default_recipe",
     "detwin",
+    "from_qoraphase",
     "hip_available",
     "hip_device_info",
     "reconstruct",

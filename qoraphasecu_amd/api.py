@@ -12,7 +12,7 @@ from typing import Literal
 
 import numpy as np
 
-from . import _core
+from . import _core, interop
 from .recipe import default_recipe
 
 NEGATIVE_WARN_FRACTION = 0.01
@@ -39,6 +39,10 @@ class Reconstruction:
         """Final Fourier error; the ranking key."""
         return float(self.errors[-1]) if self.errors.size else float("nan")
 
+    def to_qoraphase(self):
+        """The same run as a ``qoraphase.Reconstruction``; ``errors`` becomes ``metrics``."""
+        return interop.reconstruction_to_qoraphase(self)
+
 
 @dataclass(frozen=True)
 class Ensemble:
@@ -63,6 +67,11 @@ class Ensemble:
     @property
     def best(self) -> Reconstruction:
         return self.runs[self.ranking[0]]
+
+    def to_qoraphase(self, problem=None):
+        """A ``qoraphase.Ensemble`` of the runs, for its ``align`` and ``average``; pass the
+        measurement's ``qoraphase.CDI`` as *problem* to have ``average`` scored against it."""
+        return interop.ensemble_to_qoraphase(self.runs, problem)
 
 
 def cuda_available() -> bool:
@@ -140,7 +149,8 @@ def reconstruct(
         support: boolean, ``True`` inside the object; ``None`` derives an
             autocorrelation support.
         mask: boolean, ``True`` marks *unusable* pixels.
-        recipe: a :class:`Recipe`; ``None`` uses :func:`default_recipe`.
+        recipe: a :class:`Recipe` or a QoraPhase recipe (converted by
+            :func:`from_qoraphase`); ``None`` uses :func:`default_recipe`.
         starts: number of random starts; more than one returns an :class:`Ensemble`.
         batch: starts per launch in the batched and reduced-precision variants;
             ``None`` puts all of them in one.
@@ -164,6 +174,8 @@ def reconstruct(
         backend = "hip" if hip_available() else "cpu"
     if recipe is None:
         recipe = default_recipe()
+    elif interop.is_qoraphase(recipe):
+        recipe = interop.from_qoraphase(recipe)
     runs = _core.reconstruct(
         intensity,
         _flag("support", support, intensity.shape),
@@ -211,6 +223,8 @@ def reconstruct_many(
         backend = "hip" if hip_available() else "cpu"
     if recipe is None:
         recipe = default_recipe()
+    elif interop.is_qoraphase(recipe):
+        recipe = interop.from_qoraphase(recipe)
 
     def flags(name, arrays):
         if arrays is None:

@@ -204,15 +204,16 @@ def pass_roofline(w, storage, reps=50):
     return out
 
 
-def cpu_baseline(w, threads, seconds=10.0):
+def cpu_baseline(w, threads, seconds=10.0, starts=None):
     """The oracle (float, OpenMP) on a bounded sample of the workload: RAAR iterations on
-    every start, as many as fill about `seconds` of CPU time (at least the reference's
-    max(5, iters/20), bench/bench.cpp:148), sized from one timed warm-up iteration."""
+    every start (or on `starts` of them), as many as fill about `seconds` of CPU time (at
+    least the reference's max(5, iters/20), bench/bench.cpp:148), sized from one timed
+    warm-up iteration."""
     import oracle
 
     oracle.set_threads(threads)
     amp, mask, sup = oracle.make_problem(w["intensity"], None, w["mask"])
-    starts = w["starts"]
+    starts = w["starts"] if starts is None else starts
     solver = oracle.Solver(amp, mask, sup, starts=starts, seed=0)
     t0 = time.perf_counter()
     solver.run([("raar", 1, 0.9, 0.0, 0.0, "max", 0)])
@@ -493,9 +494,11 @@ def main():
             threads = min(16, os.cpu_count() or 1)
             cb = cpu_baseline(w, threads)
             res["cpu_oracle"] = cb
+            # one thread on one start, as the reference's single-thread CPU timing
+            res["cpu_oracle_1thread"] = cpu_baseline(w, 1, seconds=3.0, starts=1)
             res["numpy_fft_pair_ms"] = numpy_fft_pair(w["shape"])
-            log(f"    cpu oracle ({threads} threads): {cb['value']:.4g} start-it/s; numpy fft pair "
-                f"{res['numpy_fft_pair_ms']:.2f} ms")
+            log(f"    cpu oracle ({threads} threads): {cb['value']:.4g} start-it/s; 1 thread: "
+                f"{res['cpu_oracle_1thread']['value']:.4g}; numpy fft pair {res['numpy_fft_pair_ms']:.2f} ms")
         details[name] = res
         if name == "c2":
             headline = res

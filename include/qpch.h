@@ -124,6 +124,12 @@ int qpch_solver_get_support(qpch_solver* solver, uint8_t* host /* starts * n */)
 int qpch_solver_set_support(qpch_solver* solver, const uint8_t* host);
 int qpch_solver_save_support(qpch_solver* solver);
 int qpch_solver_restore_support(qpch_solver* solver);
+/* Detwin on the device: every start's support cut at its centre of mass along `axis`
+ * (centre-origin coordinates), bit-identical to the host computation of the reference's
+ * half_support (src/core/problem.cpp:101-127).  Replaces the reference's host round trip
+ * (get support, half_support per start, set support: src/cpu/cpu_solver.cpp:70-80,
+ * src/cuda/cuda_solver.cu:256-276); save the support first to restore it at the end. */
+int qpch_solver_half_support(qpch_solver* solver, int axis);
 
 /* errors[t * starts + s], t in [first, first + count) of the current run */
 int qpch_solver_read_errors(qpch_solver* solver, int first, int count, double* host);

@@ -95,6 +95,7 @@ def lib():
         L.qpch_solver_set_support.argtypes = [vp, vp]
         L.qpch_solver_save_support.argtypes = [vp]
         L.qpch_solver_restore_support.argtypes = [vp]
+        L.qpch_solver_half_support.argtypes = [vp, ctypes.c_int]
         L.qpch_solver_read_errors.argtypes = [vp, i, i, vp]
         L.qpch_solver_read_state.argtypes = [vp, vp]
         L.qpch_solver_energy.argtypes = [vp]
@@ -283,6 +284,10 @@ class Solver:
 
     def restore_support(self):
         check(lib().qpch_solver_restore_support(self._h))
+
+    def half_support(self, axis):
+        """Detwin on the device: each start's support cut at its centre of mass along axis."""
+        check(lib().qpch_solver_half_support(self._h, axis))
 
     def read_errors(self, first, count):
         out = np.empty((count, self.starts), np.float64)

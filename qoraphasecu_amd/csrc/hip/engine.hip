@@ -506,6 +506,7 @@ struct SolverImpl {
 
   DevBuf<float> amp_signed, amp_true;
   DevBuf<std::uint8_t> supp, saved;
+  DevBuf<unsigned long long> detwin_sums;  // [start][2]: support moment and count along the detwin axis
   DevBuf<float2> x32, w32;
   DevBuf<std::uint32_t> x16, w16;  // half2 / bfloat162 storage, 4 bytes per element
   DevBuf<double> partials, errs, sw_part;
@@ -1341,6 +1342,23 @@ int qpch_solver_save_support(qpch_solver* s) {
   return guarded([&] {
     auto& m = s->impl;
     QPCH_HIP(hipMemcpyAsync(m.saved.get(), m.supp.get(), m.supp.size(), hipMemcpyDeviceToDevice, m.stream));
+  });
+}
+
+int qpch_solver_half_support(qpch_solver* s, int axis) {
+  return guarded([&] {
+    auto& m = s->impl;
+    if (axis < 0 || axis >= m.dims.nd) throw qpch::InvalidArgument("detwin axis out of range");
+    if (m.detwin_sums.size() < static_cast<size_t>(2 * m.starts)) m.detwin_sums.resize(2 * m.starts);
+    const long long inner = m.dims.inner(axis);
+    const int len = m.dims.d[axis];
+    const dim3 grid(static_cast<unsigned>(std::min<long long>((m.dims.n + 255) / 256, 256)), m.starts);
+    hipLaunchKernelGGL(qpch::dev::fill_u64, dim3(1), dim3(256), 0, m.stream, m.detwin_sums.get(), 2 * m.starts);
+    hipLaunchKernelGGL(qpch::dev::support_moments, grid, dim3(256), 0, m.stream, m.supp.get(), m.dims.n, inner, len,
+                       m.detwin_sums.get());
+    hipLaunchKernelGGL(qpch::dev::support_cut, grid, dim3(256), 0, m.stream, m.supp.get(), m.dims.n, inner, len,
+                       m.detwin_sums.get());
+    QPCH_HIP(hipGetLastError());
   });
 }
 

@@ -69,6 +69,54 @@ __global__ void fill_double(double* p, long long count, double value) {
   }
 }
 
+__global__ void fill_u64(unsigned long long* p, int count) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) p[k] = 0ull;
+}
+
+// --- Detwin: the support cut at its centre of mass along one axis ----------------------
+// Reference: src/core/problem.cpp:101-127, applied to the centre-origin support; here the
+// support is in FFT order, where index f along the axis is centre-origin coordinate
+// (f + len / 2) % len.  The moment and the count are sums of integers, exact in 64-bit
+// integers and in the host's doubles alike, so the centre moment / max(count, 1) is the
+// host's bit for bit whatever the order of the additions.
+
+__global__ void __launch_bounds__(256)
+    support_moments(const std::uint8_t* support, long long n, long long inner, int len,
+                    unsigned long long* sums /* [start][2]: moment, count */) {
+  const long long soff = static_cast<long long>(blockIdx.y) * n;
+  unsigned long long moment = 0, count = 0;
+  for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < n;
+       e += static_cast<long long>(gridDim.x) * blockDim.x) {
+    if (support[soff + e]) {
+      const int f = static_cast<int>((e / inner) % len);
+      moment += static_cast<unsigned long long>((f + len / 2) % len);
+      count += 1;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    moment += __shfl_down(moment, o, 64);
+    count += __shfl_down(count, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0 && count) {
+    atomicAdd(sums + 2 * blockIdx.y, moment);
+    atomicAdd(sums + 2 * blockIdx.y + 1, count);
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    support_cut(std::uint8_t* support, long long n, long long inner, int len, const unsigned long long* sums) {
+  const long long soff = static_cast<long long>(blockIdx.y) * n;
+  const double count = static_cast<double>(sums[2 * blockIdx.y + 1]);
+  const double centre = static_cast<double>(sums[2 * blockIdx.y]) / (count > 1.0 ? count : 1.0);
+  for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < n;
+       e += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const int f = static_cast<int>((e / inner) % len);
+    const int c = (f + len / 2) % len;
+    support[soff + e] = (support[soff + e] != 0 && static_cast<double>(c) < centre) ? 1 : 0;
+  }
+}
+
 // --- Shrinkwrap ------------------------------------------------------------------------
 // The Gaussian transfer exp(-2 pi^2 sigma^2 |f|^2) of the reference (src/cpu/kernels.cpp:
 // 52-75) factorises over the axes, so the blur is a separable circular convolution with

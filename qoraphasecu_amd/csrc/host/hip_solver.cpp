@@ -151,18 +151,12 @@ class HipSolver final : public Solver {
     qpch_check(qpch_solver_shrinkwrap(h_, o.threshold, o.sigma, o.method == Threshold::max ? 0 : 1));
   }
 
-  // Once per recipe: a host round trip is cheaper than a kernel for it.
+  // The support cut at its centre of mass, on the device: the reference's host round trip
+  // (every support to the host, half_support per start, back) costs two transfers of all
+  // supports and a CPU pass over them.
   void apply(const DetwinBegin& o) {
     qpch_check(qpch_solver_save_support(h_));
-    std::vector<std::uint8_t> host(n_ * static_cast<std::size_t>(starts_));
-    qpch_check(qpch_solver_get_support(h_, host.data()));
-    for (int k = 0; k < starts_; ++k) {
-      const auto part = std::span(host).subspan(static_cast<std::size_t>(k) * n_, n_);
-      const auto centred = to_centered<std::uint8_t>(part, shape_);
-      const auto half = to_fft_order<std::uint8_t>(half_support(centred, shape_, o.axis), shape_);
-      std::copy(half.begin(), half.end(), part.begin());
-    }
-    qpch_check(qpch_solver_set_support(h_, host.data()));
+    qpch_check(qpch_solver_half_support(h_, o.axis));
   }
 
   void apply(const DetwinEnd&) { qpch_check(qpch_solver_restore_support(h_)); }
@@ -352,16 +346,7 @@ class HipEnsemble final : public MultiSolver {
   void apply(const DetwinBegin& o) {
     for (auto& c : classes_) {
       qpch_check(qpch_solver_save_support(c.h));
-      const std::size_t total = c.members.size() * static_cast<std::size_t>(options_.starts);
-      std::vector<std::uint8_t> host(c.n * total);
-      qpch_check(qpch_solver_get_support(c.h, host.data()));
-      for (std::size_t k = 0; k < total; ++k) {
-        const auto part = std::span(host).subspan(k * c.n, c.n);
-        const auto centred = to_centered<std::uint8_t>(part, c.shape);
-        const auto half = to_fft_order<std::uint8_t>(half_support(centred, c.shape, o.axis), c.shape);
-        std::copy(half.begin(), half.end(), part.begin());
-      }
-      qpch_check(qpch_solver_set_support(c.h, host.data()));
+      qpch_check(qpch_solver_half_support(c.h, o.axis));
     }
   }
 

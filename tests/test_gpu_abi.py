@@ -160,3 +160,28 @@ def test_bad_launch_choices_are_rejected(qpch, kw):
     obj, amp, mask, sup = _problem((64, 64), 16)
     with pytest.raises(ValueError):
         qpch.Solver(oracle.to_fft_order(amp), oracle.to_fft_order(mask), oracle.to_fft_order(sup), starts=2, **kw)
+
+
+@pytest.mark.parametrize("shape", [(64, 64), (33, 40), (16, 12, 20), (15, 16, 17), (128, 128, 128)])
+def test_device_detwin_matches_half_support(qpch, shape):
+    """qpch_solver_half_support against the oracle's half_support (the reference's
+    src/core/problem.cpp:101-127) applied to each start's centre-origin support: bit-exact
+    on every axis, odd lengths included, with an empty support (centre 0: all cut) and a
+    full one among the starts."""
+    rng = np.random.default_rng(7)
+    starts = 4
+    obj, sup, inten = synthetic.triangle(shape, max(2, min(shape) // 4))
+    amp, mask, sup = oracle.make_problem(inten, sup)
+    f = oracle.to_fft_order
+    s = qpch.Solver(f(amp), f(mask), f(sup), starts=starts, seed=1)
+    for axis in range(len(shape)):
+        centred = (rng.random((starts, *shape)) < 0.3).astype(np.uint8)
+        centred[1] = 0
+        centred[2] = 1
+        s.set_support(np.stack([f(c) for c in centred]))
+        s.half_support(axis)
+        got = s.get_support()
+        for k in range(starts):
+            want = f(oracle.half_support(centred[k], axis))
+            assert np.array_equal(got[k], want), (shape, axis, k)
+    s.close()

@@ -185,3 +185,67 @@ def test_device_detwin_matches_half_support(qpch, shape):
             want = f(oracle.half_support(centred[k], axis))
             assert np.array_equal(got[k], want), (shape, axis, k)
     s.close()
+
+
+PLANES = [(16, 64), (16, 128), (32, 32), (32, 64), (32, 128), (64, 16), (64, 32), (64, 64), (64, 128),
+          (128, 16), (128, 32), (128, 64), (128, 128)]
+
+
+@pytest.mark.parametrize("plane", PLANES)
+def test_plane_pass_every_shape(qpch, plane):
+    """Every xy-plane shape the plane pass is built for (z = 8), one RAAR and one ER
+    iteration from the same start against the oracle: fp32 within 1e-5, as the line
+    passes, and fp16 storage within its precision."""
+    shape = (8, *plane)
+    obj, amp, mask, sup = _problem(shape, max(4, min(plane) // 4), masked=True)
+    start = np.stack([synthetic.random_field(obj.size, 21).reshape(shape)])
+    f = oracle.to_fft_order
+    ops = [R.raar(1), R.er(1)]
+    o = oracle.Solver(amp, mask, sup, starts=1, start=start)
+    o.run(ops)
+    ores = o.results()[0]
+    s = qpch.Solver(f(amp), f(mask), f(sup), starts=1, start=np.stack([f(start[0])]), plane=1)
+    errs = R.run_abi(s, ops)[:, 0]
+    x = s.read_state()[0]
+    s.close()
+    assert np.all(np.abs(errs - ores[2]) <= 1e-5 * ores[2]), (errs, ores[2])
+    assert rel(x, f(ores[0])) < 1e-5
+    od = oracle.Solver(amp, mask, sup, starts=1, start=start, prec="d")
+    od.run(ops)
+    want = od.results()[0][2]
+    s = qpch.Solver(f(amp), f(mask), f(sup), starts=1, start=np.stack([f(start[0])]), storage="f16", plane=1)
+    got = R.run_abi(s, ops)[:, 0]
+    s.close()
+    assert np.all(np.abs(got - want) <= 3e-3 * want + 5e-4), (got, want)
+
+
+@pytest.mark.parametrize("shape", [(8, 4096), (4096, 8), (32, 2048), (2048, 16), (1024, 32), (16, 1024),
+                                   (512, 64), (2, 64), (64, 2), (4, 4), (2, 4, 512)])
+@pytest.mark.parametrize("radix", [16, 8])
+def test_line_lengths_2_to_4096(qpch, shape, radix):
+    """The fused line passes at every built line length (2 to 4096, both as row and as
+    column axis, radix 16 and radix 8 where built): one RAAR and one ER iteration from the
+    same start against the oracle within 1e-5, and fp16 storage within its precision.  The
+    errors are normalised by the amplitude's norm, so a value carries an absolute FP32
+    rounding floor of a few 1e-7 (|Fx| rounded relative to that norm): the tiny objects
+    of these shapes reach errors of 3e-4 after one RAAR, where that floor is 3e-4 relative."""
+    obj, amp, mask, sup = _problem(shape, max(1, min(shape) // 4), masked=True)
+    start = np.stack([synthetic.random_field(obj.size, 31).reshape(shape)])
+    f = oracle.to_fft_order
+    ops = [R.raar(1), R.er(1)]
+    o = oracle.Solver(amp, mask, sup, starts=1, start=start)
+    o.run(ops)
+    ores = o.results()[0]
+    s = qpch.Solver(f(amp), f(mask), f(sup), starts=1, start=np.stack([f(start[0])]), radix=radix)
+    errs = R.run_abi(s, ops)[:, 0]
+    x = s.read_state()[0]
+    s.close()
+    assert np.all(np.abs(errs - ores[2]) <= 1e-5 * ores[2] + 3e-7), (errs, ores[2])
+    assert rel(x, f(ores[0])) < 1e-5
+    od = oracle.Solver(amp, mask, sup, starts=1, start=start, prec="d")
+    od.run(ops)
+    want = od.results()[0][2]
+    s = qpch.Solver(f(amp), f(mask), f(sup), starts=1, start=np.stack([f(start[0])]), storage="f16", radix=radix)
+    got = R.run_abi(s, ops)[:, 0]
+    s.close()
+    assert np.all(np.abs(got - want) <= 3e-3 * want + 5e-4), (got, want)

@@ -19,6 +19,7 @@
 /* kEps, src/core/pointwise.hpp:16 */
 #define OR_EPS 1e-12
 #define OR_MAXR 64
+#define OR_V 8 /* lines per SIMD block of the FFT */
 #define OR_BLOCK 4096
 
 static int g_threads = 0;

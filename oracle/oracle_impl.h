@@ -162,6 +162,97 @@ static void F(fft_line)(const F(plan_t) * p, R* x, R* w, int forward) {
   if (src != x) memcpy(x, src, sizeof(R) * 2 * n);
 }
 
+/* The same transform on OR_V lines at once, stored lane-minor: value k of line v at
+ * x[2 (k OR_V + v)] (re) and + 1 (im).  Every lane runs exactly the scalar fft_line's
+ * operations in the same order (the loops over v only repeat them), so the results are
+ * bit-identical; the lane loops let the compiler use SIMD across lines, as pocketfft's
+ * multi-line iteration does.  Plans without a factorisation (nfac == 0) are not passed here. */
+static void F(dft_small_v)(const F(plan_t) * p, int r, int forward, const R* vr, const R* vi, R* yr, R* yi) {
+  /* vr[q * OR_V + v] */
+  if (r == 2) {
+    for (int v = 0; v < OR_V; ++v) {
+      yr[v] = vr[v] + vr[OR_V + v]; yi[v] = vi[v] + vi[OR_V + v];
+      yr[OR_V + v] = vr[v] - vr[OR_V + v]; yi[OR_V + v] = vi[v] - vi[OR_V + v];
+    }
+    return;
+  }
+  if (r == 4) {
+    for (int v = 0; v < OR_V; ++v) {
+      const R t0r = vr[v] + vr[2 * OR_V + v], t0i = vi[v] + vi[2 * OR_V + v];
+      const R t1r = vr[v] - vr[2 * OR_V + v], t1i = vi[v] - vi[2 * OR_V + v];
+      const R t2r = vr[OR_V + v] + vr[3 * OR_V + v], t2i = vi[OR_V + v] + vi[3 * OR_V + v];
+      const R t3r = vr[OR_V + v] - vr[3 * OR_V + v], t3i = vi[OR_V + v] - vi[3 * OR_V + v];
+      yr[v] = t0r + t2r; yi[v] = t0i + t2i;
+      yr[2 * OR_V + v] = t0r - t2r; yi[2 * OR_V + v] = t0i - t2i;
+      if (forward) {
+        yr[OR_V + v] = t1r + t3i; yi[OR_V + v] = t1i - t3r;
+        yr[3 * OR_V + v] = t1r - t3i; yi[3 * OR_V + v] = t1i + t3r;
+      } else {
+        yr[OR_V + v] = t1r - t3i; yi[OR_V + v] = t1i + t3r;
+        yr[3 * OR_V + v] = t1r + t3i; yi[3 * OR_V + v] = t1i - t3r;
+      }
+    }
+    return;
+  }
+  const size_t step = p->n / (size_t)r;
+  for (int j = 0; j < r; ++j) {
+    R sr[OR_V], si[OR_V];
+    for (int v = 0; v < OR_V; ++v) { sr[v] = vr[v]; si[v] = vi[v]; }
+    for (int q = 1; q < r; ++q) {
+      const size_t t = ((size_t)j * (size_t)q % (size_t)r) * step;
+      const R c = p->tw[2 * t];
+      const R s = forward ? p->tw[2 * t + 1] : -p->tw[2 * t + 1];
+      for (int v = 0; v < OR_V; ++v) {
+        sr[v] += vr[q * OR_V + v] * c - vi[q * OR_V + v] * s;
+        si[v] += vr[q * OR_V + v] * s + vi[q * OR_V + v] * c;
+      }
+    }
+    for (int v = 0; v < OR_V; ++v) { yr[j * OR_V + v] = sr[v]; yi[j * OR_V + v] = si[v]; }
+  }
+}
+
+static void F(fft_lines_v)(const F(plan_t) * p, R* x, R* w, int forward) {
+  const size_t n = p->n;
+  R* src = x;
+  R* dst = w;
+  size_t L = 1;
+  R vr[OR_MAXR * OR_V], vi[OR_MAXR * OR_V], yr[OR_MAXR * OR_V], yi[OR_MAXR * OR_V];
+  for (int s = 0; s < p->nfac; ++s) {
+    const int r = p->fac[s];
+    const size_t m = n / (L * (size_t)r);
+    const size_t nr = n / (size_t)r;
+    for (size_t i = 0; i < nr; ++i) {
+      const size_t k = i % L;
+      for (int q = 0; q < r; ++q) {
+        const R* in = src + 2 * (i + (size_t)q * nr) * OR_V;
+        const size_t t = (size_t)q * k * m;
+        if (t == 0) {
+          for (int v = 0; v < OR_V; ++v) { vr[q * OR_V + v] = in[2 * v]; vi[q * OR_V + v] = in[2 * v + 1]; }
+        } else {
+          const R c = p->tw[2 * t];
+          const R sn = forward ? p->tw[2 * t + 1] : -p->tw[2 * t + 1];
+          for (int v = 0; v < OR_V; ++v) {
+            const R a = in[2 * v], b = in[2 * v + 1];
+            vr[q * OR_V + v] = a * c - b * sn;
+            vi[q * OR_V + v] = a * sn + b * c;
+          }
+        }
+      }
+      F(dft_small_v)(p, r, forward, vr, vi, yr, yi);
+      const size_t base = (i - k) * (size_t)r + k;
+      for (int q = 0; q < r; ++q) {
+        R* out = dst + 2 * (base + (size_t)q * L) * OR_V;
+        for (int v = 0; v < OR_V; ++v) { out[2 * v] = yr[q * OR_V + v]; out[2 * v + 1] = yi[q * OR_V + v]; }
+      }
+    }
+    R* t = src;
+    src = dst;
+    dst = t;
+    L *= (size_t)r;
+  }
+  if (src != x) memcpy(x, src, sizeof(R) * 2 * n * OR_V);
+}
+
 /* Unnormalised transform along every axis, then one unitary scale 1/sqrt(N), like the
  * reference's pocketfft::c2c(..., fct = 1/sqrt(N)) over all axes (src/cpu/fft.cpp:12-35). */
 void F(or_fft)(R* data, int ndim, const size_t* shape, int forward) {
@@ -175,14 +266,48 @@ void F(or_fft)(R* data, int ndim, const size_t* shape, int forward) {
     const size_t outer = total / (len * inner);
     F(plan_t)* plan = F(plan_create)(len);
     const long long lines = (long long)(outer * inner);
+    /* Lines go in blocks of OR_V through fft_lines_v where the plan factorises: OR_V
+     * adjacent columns (inner > 1, within one outer block) or OR_V consecutive rows; the
+     * rest one at a time through fft_line.  Both give the same bits per line. */
+    const int blocked = plan->nfac > 0;
+    const long long cols_v = inner > 1 ? (long long)(inner / OR_V) : 0;  /* column blocks per outer */
+    const long long nblocks = blocked ? (inner > 1 ? (long long)outer * cols_v : lines / OR_V) : 0;
 #pragma omp parallel num_threads(or_threads())
     {
-      R* line = (R*)malloc(sizeof(R) * 2 * len);
-      R* work = (R*)malloc(sizeof(R) * 2 * len);
+      R* line = (R*)malloc(sizeof(R) * 2 * len * OR_V);
+      R* work = (R*)malloc(sizeof(R) * 2 * len * OR_V);
+#pragma omp for schedule(static)
+      for (long long bl = 0; bl < nblocks; ++bl) {
+        if (inner > 1) {
+          const size_t o = (size_t)(bl / cols_v);
+          const size_t in = (size_t)(bl % cols_v) * OR_V;
+          R* base = data + 2 * (o * len * inner + in);
+          for (size_t k = 0; k < len; ++k) memcpy(line + 2 * k * OR_V, base + 2 * k * inner, sizeof(R) * 2 * OR_V);
+          F(fft_lines_v)(plan, line, work, forward);
+          for (size_t k = 0; k < len; ++k) memcpy(base + 2 * k * inner, line + 2 * k * OR_V, sizeof(R) * 2 * OR_V);
+        } else {
+          R* base = data + 2 * (size_t)bl * OR_V * len;
+          for (int v = 0; v < OR_V; ++v) {
+            for (size_t k = 0; k < len; ++k) {
+              line[2 * (k * OR_V + v)] = base[2 * ((size_t)v * len + k)];
+              line[2 * (k * OR_V + v) + 1] = base[2 * ((size_t)v * len + k) + 1];
+            }
+          }
+          F(fft_lines_v)(plan, line, work, forward);
+          for (int v = 0; v < OR_V; ++v) {
+            for (size_t k = 0; k < len; ++k) {
+              base[2 * ((size_t)v * len + k)] = line[2 * (k * OR_V + v)];
+              base[2 * ((size_t)v * len + k) + 1] = line[2 * (k * OR_V + v) + 1];
+            }
+          }
+        }
+      }
+      /* the lines no block covered */
 #pragma omp for schedule(static)
       for (long long l = 0; l < lines; ++l) {
         const size_t o = (size_t)l / inner;
         const size_t in = (size_t)l % inner;
+        if (blocked && (inner > 1 ? (long long)in < cols_v * OR_V : l < nblocks * OR_V)) continue;
         R* base = data + 2 * (o * len * inner + in);
         if (inner == 1) {
           memcpy(line, base, sizeof(R) * 2 * len);

@@ -151,19 +151,21 @@ int qpch_solver_time_passes(qpch_solver* solver, int low, int reps, float* ms, i
 int qpch_solver_pass_info(qpch_solver* solver, int low, int* axes, int* modes, int cap, int* count);
 
 /* --- autotuner --------------------------------------------------------------------------
- * Times candidate launch configurations of `problem` (starts per launch, 3D plane or line
- * passes, XCD-affine or plain block numbering) on trial solvers with random starts, a few
- * graph-replayed iterations each, and returns the fastest (a candidate replaces the best so
- * far only when it is 2% faster, the first candidate being the default: all starts in one
- * launch, one lane, XCD-affine numbering).  The configurations give the
- * same results; only the time differs.  Cached per process by shape, total starts and
+ * Times candidate launch configurations of `problem` (FFT radix, starts per launch,
+ * concurrent launch groups, XCD-affine or plain block numbering) on trial solvers with
+ * random starts, a few graph-replayed iterations each, and returns the fastest (a candidate
+ * replaces the best so far only when it is 2% faster, the first candidate being the
+ * default: all starts in one launch, one lane, XCD-affine numbering).  For one radix the
+ * configurations give the same results; only the time differs.  Cached per process by shape, total starts and
  * storage; qpch_autotune_log() returns one line per tuned key (also printed to stderr
  * when QPCH_TUNE_LOG=1).  No reference counterpart: the reference leaves Options::batch
  * to the user (include/qoraphasecu/solver.hpp:44-46).  A problem->batch > 0 is kept and
  * only the other choices are tuned; problem->lanes == 1 keeps launch groups sequential.  The
- * radix and the 3D plane / line choice are not tuned (they change rounding, and results must
- * not depend on the split of the starts into launches): problem->radix (16 when 0) and
- * problem->plane (the plane pass where supported when 0) are kept. */
+ * FFT radix changes rounding, and results must not depend on the split of the starts into
+ * launches, so it is tuned once per (shape, storage, mixed) in the process, at one canonical
+ * configuration (all starts in one launch), and kept for every split of that shape (radix 8
+ * must be 2% faster than 16); a nonzero problem->radix is kept.  The 3D plane / line choice
+ * is not tuned: problem->plane (the plane pass where supported when 0) is kept. */
 int qpch_autotune(const qpch_problem* problem, qpch_tuning* out);
 const char* qpch_autotune_log(void);
 

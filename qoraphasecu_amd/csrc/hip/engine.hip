@@ -1545,6 +1545,7 @@ int qpch_gaussian_blur(const float* host_in, float* host_out, int ndim, const si
 namespace {
 std::mutex g_tune_mu;
 std::map<std::string, qpch_tuning> g_tune_cache;
+std::map<std::string, int> g_radix_cache;  // (shape, storage, mixed) -> radix
 std::string g_tune_log;
 
 std::string tune_key(const qpch_problem& p) {
@@ -1583,20 +1584,43 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
         if (total % b == 0) batches.push_back(b);
       }
     }
-    // Plane vs line passes (3D) changes the order of the arithmetic, so like the radix it
-    // is not tuned: the plane pass where supported (it was the faster one in every
-    // measurement), unless the caller asks for line passes.
+    // Plane vs line passes (3D) change the order of the arithmetic and are not tuned: the
+    // plane pass where supported (it was the faster one in every measurement), unless the
+    // caller asks for line passes.
     const bool plane_ok = d.nd == 3 && qpch::plane_supported(d.d[1], d.d[2]);
     const std::vector<int> planes{problem->plane != 0 ? problem->plane : (plane_ok ? 1 : 0)};
-    // The radix is not tuned: it changes the arithmetic (rounding), and a start's result
-    // must not depend on how the starts are split into launches (batched == looped).  A
-    // caller may still fix it in problem->radix.
-    const std::vector<int> radices{problem->radix != 0 ? problem->radix : 16};
+    // The radix changes the arithmetic (rounding), and a start's result must not depend on
+    // how the starts are split into launches (batched == looped).  So it is tuned once per
+    // (shape, storage, mixed) at one canonical configuration (all of the first caller's
+    // starts in one launch, one lane, XCD-affine numbering) and then kept for every split
+    // of that shape; radix 8 must be 2% faster to replace 16.  A caller may fix it in
+    // problem->radix.
+    bool r8_possible = qpch::fusable(d);
+    if (r8_possible) {
+      bool any = false;
+      for (int a = 0; a < d.nd; ++a) any = any || qpch::has_radix8(d.d[a]);
+      r8_possible = any;
+    }
+    std::string rkey = std::to_string(d.nd);
+    for (int a = 0; a < d.nd; ++a) rkey += "x" + std::to_string(d.d[a]);
+    rkey += "/st" + std::to_string(problem->storage) + "/m" + std::to_string(problem->mixed);
+    int tuned_radix = 16;
+    bool radix_known = problem->radix != 0 || !r8_possible;
+    if (problem->radix != 0) tuned_radix = problem->radix;
+    if (!radix_known) {
+      std::lock_guard<std::mutex> lock(g_tune_mu);
+      auto it = g_radix_cache.find(rkey);
+      if (it != g_radix_cache.end()) {
+        tuned_radix = it->second;
+        radix_known = true;
+      }
+    }
+    std::vector<int> radices{tuned_radix};
     std::string line = "qpch autotune " + key + ":";
     qpch_tuning best{};
     best.us_per_iteration = 1e300;
     const bool lowp = problem->storage != QPCH_F32;
-    auto trial = [&](const qpch_tuning& t) {
+    auto measure = [&](const qpch_tuning& t) -> double {
       qpch_problem q = *problem;
       q.batch = t.batch;
       q.lanes = t.lanes;
@@ -1634,8 +1658,11 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
       if (lowp && q.mixed) us = (2.0 * us + per_iteration(true)) / 3.0;
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
+      return us;
+    };
+    auto trial = [&](const qpch_tuning& t) {
       qpch_tuning r = t;
-      r.us_per_iteration = us;
+      r.us_per_iteration = measure(t);
       line += " [batch " + std::to_string(t.batch) + (t.lanes > 1 ? " x" + std::to_string(t.lanes) + " lanes" : "") +
               (t.plane ? (t.plane == 1 ? " plane" : " lines") : "") + (t.xcd == 2 ? " xcd-off" : t.xcd == 3 ? " xcd-off-modulus" : "") +
               (t.radix == 8 ? " radix-8" : "") + ": " + std::to_string(r.us_per_iteration) + " us/it]";
@@ -1644,6 +1671,21 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
       // lane, XCD-affine numbering), which then stays unless another is clearly faster.
       if (r.us_per_iteration < 0.98 * best.us_per_iteration) best = r;
     };
+    if (!radix_known) {
+      const qpch_tuning canon16{total, planes[0], 0, 0.0, 1, 16};
+      qpch_tuning canon8 = canon16;
+      canon8.radix = 8;
+      const double t16 = measure(canon16), t8 = measure(canon8);
+      tuned_radix = t8 < 0.98 * t16 ? 8 : 16;
+      line += " [radix for " + rkey + ", all " + std::to_string(total) + " starts in one launch: 16: " +
+              std::to_string(t16) + " us/it, 8: " + std::to_string(t8) + " us/it -> " + std::to_string(tuned_radix) +
+              "]";
+      std::lock_guard<std::mutex> lock(g_tune_mu);
+      auto [it, fresh] = g_radix_cache.emplace(rkey, tuned_radix);
+      tuned_radix = it->second;  // a concurrent tuner's choice stands
+      (void)fresh;
+      radices[0] = tuned_radix;
+    }
     // first the split of the starts into launches (default kernels), then the kernel
     // choices for the best split
     for (int b : batches) {

@@ -69,7 +69,7 @@ struct Storage<__hip_bfloat162> {
   static __device__ __forceinline__ float2 bld(Rsrc r, int vo, int so) {
     const unsigned u = __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0);
     __hip_bfloat162 h;
-    __builtin_memcpy(&h, &u, sizeof(u));
+    __builtin_memcpy(static_cast<void*>(&h), &u, sizeof(u));
     return __bfloat1622float2(h);
   }
   static __device__ __forceinline__ void bst(Rsrc r, int vo, int so, float2 v) {

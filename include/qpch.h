@@ -25,8 +25,11 @@
  *
  * Conventions: every function returns QPCH_OK or an error code and leaves a message for
  * qpch_last_error() (thread-local).  One solver handle must not be used from two threads
- * at once.  Solver calls run on the solver's own non-blocking stream and return after
- * the work is queued, except those that hand data back to the host.
+ * at once; different handles may run in different threads at the same time.  Solver calls
+ * run on the solver's own non-blocking stream and return after the work is queued, except
+ * those that hand data back to the host.  No call touches the null stream: host copies go
+ * through the solver's stream (the standalone operators create one of their own), so one
+ * thread's copies never wait on, or break, another thread's graph capture.
  */
 #ifndef QPCH_H
 #define QPCH_H

@@ -96,12 +96,36 @@ class DevBuf {
   size_t n_ = 0;
 };
 
+// Host <-> device copies go through a stream of the caller's own (non-blocking) and wait
+// for it: never through the legacy stream, which a graph capture running in another
+// thread forbids (and which would not order the copy after that stream's queued work).
+void copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s) {
+  if (!bytes) return;
+  QPCH_HIP(hipMemcpyAsync(dst, src, bytes, kind, s));
+  QPCH_HIP(hipStreamSynchronize(s));
+}
+
 template <class T>
-DevBuf<T> upload(const T* host, size_t count) {
+DevBuf<T> upload(const T* host, size_t count, hipStream_t s) {
   DevBuf<T> d(count);
-  if (count) QPCH_HIP(hipMemcpy(d.get(), host, count * sizeof(T), hipMemcpyHostToDevice));
+  copy_sync(d.get(), host, count * sizeof(T), hipMemcpyHostToDevice, s);
   return d;
 }
+
+// A non-blocking stream for one synchronous entry point, drained and destroyed on exit.
+// Declared before the entry point's device buffers, so they are freed first.
+struct OwnStream {
+  hipStream_t s{};
+  OwnStream() { QPCH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+  ~OwnStream() {
+    if (s) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+  }
+  OwnStream(const OwnStream&) = delete;
+  OwnStream& operator=(const OwnStream&) = delete;
+};
 
 bool is_pow2(long long v) { return v >= 1 && (v & (v - 1)) == 0; }
 
@@ -857,7 +881,7 @@ struct SolverImpl {
     auto it = taps_cache.find(key);
     if (it == taps_cache.end()) {
       Taps t = blur_taps(N, sigma);
-      DevBuf<float> d = upload(t.taps.data(), t.taps.size());
+      DevBuf<float> d = upload(t.taps.data(), t.taps.size(), stream);
       it = taps_cache.emplace(key, std::make_pair(t.J, std::move(d))).first;
     }
     return it->second;
@@ -876,7 +900,7 @@ struct SolverImpl {
         const double f = static_cast<double>(sj) / N;
         g[j] = static_cast<float>(std::exp(k * f * f));
       }
-      DevBuf<float> d = upload(g.data(), g.size());
+      DevBuf<float> d = upload(g.data(), g.size(), stream);
       it = taps_cache.emplace(key, std::make_pair(0, std::move(d))).first;
     }
     return it->second.second.get();
@@ -1061,14 +1085,14 @@ struct SolverImpl {
     }
     energy = energies[0];
     amp_scale = amp_scales[0];
-    amp_true = upload(amp.data(), np);
-    amp_signed = upload(signed_amp.data(), np);
+    amp_true = upload(amp.data(), np, stream);
+    amp_signed = upload(signed_amp.data(), np, stream);
     std::vector<std::uint8_t> sup(static_cast<size_t>(n) * starts);
     for (int s = 0; s < starts; ++s) {
       const std::uint8_t* src = p.support + (s / per_problem) * n;
       for (long long i = 0; i < n; ++i) sup[s * n + i] = src[i] != 0;
     }
-    supp = upload(sup.data(), sup.size());
+    supp = upload(sup.data(), sup.size(), stream);
     saved.resize(sup.size());
     x32.resize(static_cast<size_t>(n) * starts);
     w32.resize(static_cast<size_t>(n) * starts);
@@ -1093,7 +1117,7 @@ struct SolverImpl {
         const float sc = amp_scales[s / per_problem];
         if (sc != 1.0f) for (long long i = 0; i < 2 * n; ++i) host[2 * n * s + i] /= sc;
       }
-      QPCH_HIP(hipMemcpy(w32.get(), host.data(), sizeof(float2) * n * starts, hipMemcpyHostToDevice));
+      copy_sync(w32.get(), host.data(), sizeof(float2) * n * starts, hipMemcpyHostToDevice, stream);
     } else {
       for (int g = 0; g < groups; ++g) {
         float2* w = w32.get() + static_cast<long long>(g) * batch * n;
@@ -1156,7 +1180,7 @@ struct SolverImpl {
       // low-precision iteration converts back.
     }
     std::vector<float> tmp;
-    QPCH_HIP(hipMemcpy(host, x32.get(), sizeof(float2) * n * starts, hipMemcpyDeviceToHost));
+    copy_sync(host, x32.get(), sizeof(float2) * n * starts, hipMemcpyDeviceToHost, stream);
     for (int s = 0; s < starts; ++s) {
       const float sc = amp_scales[s / per_problem];
       if (sc != 1.0f) for (long long i = 0; i < 2 * n; ++i) host[2 * n * s + i] *= sc;
@@ -1325,16 +1349,14 @@ int qpch_solver_shrinkwrap(qpch_solver* s, float threshold, float sigma, int met
 int qpch_solver_get_support(qpch_solver* s, uint8_t* host) {
   return guarded([&] {
     auto& m = s->impl;
-    QPCH_HIP(hipStreamSynchronize(m.stream));
-    QPCH_HIP(hipMemcpy(host, m.supp.get(), m.supp.size(), hipMemcpyDeviceToHost));
+    qpch::copy_sync(host, m.supp.get(), m.supp.size(), hipMemcpyDeviceToHost, m.stream);
   });
 }
 
 int qpch_solver_set_support(qpch_solver* s, const uint8_t* host) {
   return guarded([&] {
     auto& m = s->impl;
-    QPCH_HIP(hipStreamSynchronize(m.stream));
-    QPCH_HIP(hipMemcpy(m.supp.get(), host, m.supp.size(), hipMemcpyHostToDevice));
+    qpch::copy_sync(m.supp.get(), host, m.supp.size(), hipMemcpyHostToDevice, m.stream);
   });
 }
 
@@ -1374,10 +1396,8 @@ int qpch_solver_read_errors(qpch_solver* s, int first, int count, double* host) 
     auto& m = s->impl;
     if (first < 0 || count < 0 || first + count > m.err_cap) throw qpch::InvalidArgument("error slots out of range");
     QPCH_HIP(hipStreamSynchronize(m.stream));
-    if (count) {
-      QPCH_HIP(hipMemcpy(host, m.errs.get() + static_cast<size_t>(first) * m.starts,
-                         sizeof(double) * count * m.starts, hipMemcpyDeviceToHost));
-    }
+    qpch::copy_sync(host, m.errs.get() + static_cast<size_t>(first) * m.starts, sizeof(double) * count * m.starts,
+                    hipMemcpyDeviceToHost, m.stream);
   });
 }
 
@@ -1460,14 +1480,10 @@ int qpch_solver_synchronize(qpch_solver* s) {
 int qpch_fft(float* host, int ndim, const size_t* shape, int forward) {
   return guarded([&] {
     const qpch::Dims d = qpch::make_dims(ndim, shape);
-    qpch::DevBuf<float2> buf = qpch::upload(reinterpret_cast<const float2*>(host), d.n);
-    hipStream_t st{};
-    QPCH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    qpch::device_fft(buf.get(), d, 1, forward != 0, static_cast<float>(1.0 / std::sqrt(static_cast<double>(d.n))), st);
-    const hipError_t e = hipStreamSynchronize(st);
-    (void)hipStreamDestroy(st);
-    QPCH_HIP(e);
-    QPCH_HIP(hipMemcpy(host, buf.get(), sizeof(float2) * d.n, hipMemcpyDeviceToHost));
+    const qpch::OwnStream st;
+    qpch::DevBuf<float2> buf = qpch::upload(reinterpret_cast<const float2*>(host), d.n, st.s);
+    qpch::device_fft(buf.get(), d, 1, forward != 0, static_cast<float>(1.0 / std::sqrt(static_cast<double>(d.n))), st.s);
+    qpch::copy_sync(host, buf.get(), sizeof(float2) * d.n, hipMemcpyDeviceToHost, st.s);
   });
 }
 
@@ -1477,40 +1493,37 @@ int qpch_modulus_project(float* host_x, const float* amplitude, const uint8_t* m
     const qpch::Dims d = qpch::make_dims(ndim, shape);
     std::vector<float> signed_amp(d.n);
     for (long long i = 0; i < d.n; ++i) signed_amp[i] = mask[i] ? -1.0f : amplitude[i];
-    qpch::DevBuf<float> amp = qpch::upload(signed_amp.data(), d.n);
-    qpch::DevBuf<float2> buf = qpch::upload(reinterpret_cast<const float2*>(host_x), d.n);
+    const qpch::OwnStream st;
+    qpch::DevBuf<float> amp = qpch::upload(signed_amp.data(), d.n, st.s);
+    qpch::DevBuf<float2> buf = qpch::upload(reinterpret_cast<const float2*>(host_x), d.n, st.s);
     const int grid = qpch::grid_for(d.n, 256, 1024);
     qpch::DevBuf<double> part(grid);
     qpch::DevBuf<int> counter(1);
-    hipStream_t st{};
-    QPCH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     const float s = static_cast<float>(1.0 / std::sqrt(static_cast<double>(d.n)));
-    qpch::device_fft(buf.get(), d, 1, true, 1.0f, st);
-    hipLaunchKernelGGL(qpch::dev::modulus_elementwise, dim3(grid, 1), dim3(256), 0, st, buf.get(), amp.get(),
+    qpch::device_fft(buf.get(), d, 1, true, 1.0f, st.s);
+    hipLaunchKernelGGL(qpch::dev::modulus_elementwise, dim3(grid, 1), dim3(256), 0, st.s, buf.get(), amp.get(),
                        d.n, s, s, part.get(), counter.get());
-    qpch::device_fft(buf.get(), d, 1, false, 1.0f, st);
-    const hipError_t e = hipStreamSynchronize(st);
-    (void)hipStreamDestroy(st);
-    QPCH_HIP(e);
+    QPCH_HIP(hipGetLastError());
+    qpch::device_fft(buf.get(), d, 1, false, 1.0f, st.s);
     std::vector<double> hp(grid);
-    QPCH_HIP(hipMemcpy(hp.data(), part.get(), sizeof(double) * grid, hipMemcpyDeviceToHost));
+    qpch::copy_sync(hp.data(), part.get(), sizeof(double) * grid, hipMemcpyDeviceToHost, st.s);
     double num = 0.0;
     for (double v : hp) num += v;
     *numerator = num;
-    QPCH_HIP(hipMemcpy(host_x, buf.get(), sizeof(float2) * d.n, hipMemcpyDeviceToHost));
+    qpch::copy_sync(host_x, buf.get(), sizeof(float2) * d.n, hipMemcpyDeviceToHost, st.s);
   });
 }
 
 int qpch_support_project(float* host_x, const uint8_t* support, size_t n) {
   return guarded([&] {
     const long long nn = static_cast<long long>(n);
-    qpch::DevBuf<float2> buf = qpch::upload(reinterpret_cast<const float2*>(host_x), n);
-    qpch::DevBuf<std::uint8_t> sup = qpch::upload(support, n);
-    hipLaunchKernelGGL((qpch::dev::convert<float2, float2>), dim3(qpch::grid_for(nn)), dim3(256), 0, nullptr,
+    const qpch::OwnStream st;
+    qpch::DevBuf<float2> buf = qpch::upload(reinterpret_cast<const float2*>(host_x), n, st.s);
+    qpch::DevBuf<std::uint8_t> sup = qpch::upload(support, n, st.s);
+    hipLaunchKernelGGL((qpch::dev::convert<float2, float2>), dim3(qpch::grid_for(nn)), dim3(256), 0, st.s,
                        buf.get(), buf.get(), nn, 1.0f, sup.get());
     QPCH_HIP(hipGetLastError());
-    QPCH_HIP(hipDeviceSynchronize());
-    QPCH_HIP(hipMemcpy(host_x, buf.get(), sizeof(float2) * n, hipMemcpyDeviceToHost));
+    qpch::copy_sync(host_x, buf.get(), sizeof(float2) * n, hipMemcpyDeviceToHost, st.s);
   });
 }
 
@@ -1518,23 +1531,23 @@ int qpch_gaussian_blur(const float* host_in, float* host_out, int ndim, const si
   return guarded([&] {
     if (!(sigma >= 0.0f)) throw qpch::InvalidArgument("sigma must be >= 0");
     const qpch::Dims d = qpch::make_dims(ndim, shape);
-    qpch::DevBuf<float> a = qpch::upload(host_in, d.n);
+    const qpch::OwnStream st;
+    qpch::DevBuf<float> a = qpch::upload(host_in, d.n, st.s);
     qpch::DevBuf<float> b(d.n);
     float* cur = a.get();
     float* other = b.get();
     std::vector<qpch::DevBuf<float>> keep;
     for (int ax = d.nd - 1; ax >= 0; --ax) {
       const qpch::Taps t = qpch::blur_taps(d.d[ax], sigma);
-      keep.push_back(qpch::upload(t.taps.data(), t.taps.size()));
+      keep.push_back(qpch::upload(t.taps.data(), t.taps.size(), st.s));
       qpch::dev::BlurAxis bx{d.d[ax], static_cast<int>(d.inner(ax)), static_cast<int>(t.taps.size()), t.J,
                              keep.back().get()};
       hipLaunchKernelGGL((qpch::dev::blur_axis<float2, false>), dim3(qpch::grid_for(d.n), 1), dim3(256), 0,
-                         nullptr, cur, other, d.n, bx);
+                         st.s, cur, other, d.n, bx);
       QPCH_HIP(hipGetLastError());
       std::swap(cur, other);
     }
-    QPCH_HIP(hipDeviceSynchronize());
-    QPCH_HIP(hipMemcpy(host_out, cur, sizeof(float) * d.n, hipMemcpyDeviceToHost));
+    qpch::copy_sync(host_out, cur, sizeof(float) * d.n, hipMemcpyDeviceToHost, st.s);
   });
 }
 

@@ -94,3 +94,35 @@ def test_auto_backend_is_hip(qpc, triangle):
     auto = qpc.reconstruct(intensity, support=support, recipe=recipe, seed=3)
     hip = qpc.reconstruct(intensity, support=support, recipe=recipe, seed=3, backend="hip")
     np.testing.assert_array_equal(auto.errors, hip.errors)
+
+
+@pytest.mark.gpu
+def test_two_threads_run_independent_solvers(qpc, triangle):
+    """The API releases the GIL for a run, and each solver keeps its own stream, graphs and
+    error state, so two threads may reconstruct at once (one solver per thread, as the
+    reference's contract states): the results equal the sequential runs bit for bit."""
+    import threading
+
+    _, support, intensity = triangle
+    recipe = qpc.Recipe(qpc.HIO(30), qpc.Shrinkwrap(), qpc.ER(20))
+    kw = dict(support=support, recipe=recipe, starts=4, backend="hip")
+    want = [qpc.reconstruct(intensity, seed=s, variant=v, **kw) for s, v in ((1, "batched"), (2, "mixed"))]
+    got = [None, None]
+    errors = []
+
+    def work(k, seed, variant):
+        try:
+            got[k] = qpc.reconstruct(intensity, seed=seed, variant=variant, **kw)
+        except Exception as e:  # surfaced below
+            errors.append(e)
+
+    threads = [threading.Thread(target=work, args=(0, 1, "batched")), threading.Thread(target=work, args=(1, 2, "mixed"))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not errors, errors
+    for g, w in zip(got, want):
+        for a, b in zip(g, w):
+            np.testing.assert_array_equal(a.object, b.object)
+            np.testing.assert_array_equal(a.errors, b.errors)

@@ -170,6 +170,8 @@ int qpch_solver_pass_info(qpch_solver* solver, int low, int* axes, int* modes, i
  * must be 2% faster than 16); a nonzero problem->radix is kept.  The 3D plane / line choice
  * is not tuned: problem->plane (the plane pass where supported when 0) is kept. */
 int qpch_autotune(const qpch_problem* problem, qpch_tuning* out);
+/* The tuner's log so far: a copy owned by the calling thread, valid until that thread calls
+ * qpch_autotune_log() again.  Tuners in other threads may append meanwhile. */
 const char* qpch_autotune_log(void);
 
 /* --- groups: several solvers (e.g. one per problem shape) advanced by one graph launch --

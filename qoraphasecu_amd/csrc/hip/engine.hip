@@ -1573,7 +1573,12 @@ std::string tune_key(const qpch_problem& p) {
 
 extern "C" {
 
-const char* qpch_autotune_log(void) { return g_tune_log.c_str(); }
+const char* qpch_autotune_log(void) {
+  thread_local std::string copy;
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  copy = g_tune_log;
+  return copy.c_str();
+}
 
 int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
   return guarded([&] {

@@ -105,3 +105,62 @@ def test_against_fp64_oracle(name, starts, variants, final_bar):
             bar = max(final_bar[variant], 2.0 * yard_final)
             assert abs(best_gpu - best_ref) <= bar, (variant, best_gpu, best_ref, yard_final)
     _REPORT[name] = rep
+
+
+ENSEMBLE_SAMPLE = (0, 16, 32)   # the first problem of each size (128^2, 256^2, 512^2)
+ENSEMBLE_STARTS = 2
+
+
+def test_ensemble_against_fp64_oracle():
+    """The uneven ensemble: a problem of each size, run together (one device solver per shape,
+    one graph launch per chunk) on the fp32 and fp16-storage paths, each problem against its
+    own FP64 oracle run from the same starts.  The recipe (the default one: RAAR, detwin,
+    Shrinkwrap, ER) is chaotic, so the bars are those of the 64^2 and 256^2 workloads."""
+    import qoraphasecu_amd as qpc
+    from qoraphasecu_amd import workloads
+
+    w = workloads.config("c3")
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    ops = w["recipe"].ops()
+    picked = [w["problems"][k] for k in ENSEMBLE_SAMPLE]
+    refs = []
+    for _, intensity, mask in picked:
+        amp, m, sup = oracle.make_problem(intensity, None, mask)
+        ref = oracle.Solver(amp, m, sup, starts=ENSEMBLE_STARTS, seed=0, prec="d")
+        ref.run(ops)
+        want = ref.results()
+        hist = [np.asarray(r[2], np.float64) for r in want]
+        yard = oracle.Solver(amp, m, sup, starts=ENSEMBLE_STARTS, seed=0, prec="f")
+        yard.run(ops)
+        yard_track = [_tracking(r[2], h) for r, h in zip(yard.results(), hist)]
+        refs.append(dict(sup=sup.astype(bool), want=want, hist=hist, yard_track=yard_track))
+    rep = {"problems": [f"{p[1].shape[0]}^2 (problem {k})" for k, p in zip(ENSEMBLE_SAMPLE, picked)],
+           "starts": ENSEMBLE_STARTS, "iterations": w["recipe"].iterations(),
+           "fp64_best_final_errors": [min(float(h[-1]) for h in r["hist"]) for r in refs],
+           "oracle_float_iterations_within_1e-3": [r["yard_track"] for r in refs]}
+    for variant in ("batched", "mixed"):
+        runs = qpc.reconstruct_many([p[1] for p in picked], supports=[r["sup"] for r in refs],
+                                    masks=[p[2].astype(bool) for p in picked], recipe=w["recipe"],
+                                    starts=ENSEMBLE_STARTS, seed=0, backend="hip", variant=variant)
+        rep[variant] = []
+        for (k, p), run, r in zip(zip(ENSEMBLE_SAMPLE, picked), runs, refs):
+            got = list(run)
+            errs = [float(g.error) for g in got]
+            assert all(np.isfinite(errs)), (variant, k, errs)
+            early = max(float(np.max(np.abs(g.errors[:LOW_ITERS] - h[:LOW_ITERS]) / h[:LOW_ITERS]))
+                        for g, h in zip(got, r["hist"]))
+            track = [_tracking(g.errors, h) for g, h in zip(got, r["hist"])]
+            obj = [float(qpc.aligned_error(g.object, o[0].astype(np.complex64))) for g, o in zip(got, r["want"])]
+            best_ref = min(float(h[-1]) for h in r["hist"])
+            rep[variant].append({"best_final_error": min(errs), "best_abs_diff": abs(min(errs) - best_ref),
+                                 f"max_rel_diff_first_{LOW_ITERS}_iterations": early,
+                                 "iterations_within_1e-3": track, "object_error_vs_fp64": obj})
+            print(f"c3 problem {k} ({p[1].shape[0]}^2) {variant}: best final error {min(errs):.4e} (fp64 "
+                  f"{best_ref:.4e}); first {LOW_ITERS} iterations within {early:.1e}; tracks the fp64 history to "
+                  f"1e-3 for {min(track)}-{max(track)} iterations (oracle float run: {min(r['yard_track'])}-"
+                  f"{max(r['yard_track'])}); object error vs fp64 median {float(np.median(obj)):.2e}")
+            if variant == "mixed":
+                assert early <= LOW_TOL, (variant, k, early)
+            else:
+                assert min(track) >= 0.8 * min(r["yard_track"]), (variant, k, track, r["yard_track"])
+    _REPORT["c3"] = rep

@@ -736,7 +736,7 @@ static void F(grow)(F(solver_t) * s, int more) {
   int cap = s->cap ? s->cap : 64;
   while (cap < need) cap *= 2;
   double* e = (double*)calloc((size_t)cap * (size_t)s->starts, sizeof(double));
-  for (int k = 0; k < s->starts; ++k) {
+  for (int k = 0; k < s->starts && s->iterations > 0; ++k) {  /* no history yet: errors is NULL */
     memcpy(e + (size_t)k * cap, s->errors + (size_t)k * s->cap, sizeof(double) * s->iterations);
   }
   free(s->errors);

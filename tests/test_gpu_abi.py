@@ -249,3 +249,33 @@ def test_line_lengths_2_to_4096(qpch, shape, radix):
     got = R.run_abi(s, ops)[:, 0]
     s.close()
     assert np.all(np.abs(got - want) <= 3e-3 * want + 5e-4), (got, want)
+
+
+@pytest.mark.parametrize("shape", [(4096, 4096), (256, 256, 256)])
+def test_largest_shapes(qpch, shape):
+    """The largest arrays of the fused schedules: 4096^2 (the longest built line on both
+    axes) and 256^3 (256 x 256 planes exceed the plane pass, so the four line passes run).
+    One RAAR and one ER iteration from the same start against the oracle within 1e-5, the
+    object within 1e-5, and fp16 storage against the FP64 oracle within its precision."""
+    obj, amp, mask, sup = _problem(shape, shape[-1] // 4, masked=True)
+    start = np.stack([synthetic.random_field(obj.size, 41).reshape(shape)])
+    f = oracle.to_fft_order
+    ops = [R.raar(1), R.er(1)]
+    o = oracle.Solver(amp, mask, sup, starts=1, start=start)
+    o.run(ops)
+    ores = o.results()[0]
+    print(f"{shape}: fp32 graph", flush=True)
+    s = qpch.Solver(f(amp), f(mask), f(sup), starts=1, start=np.stack([f(start[0])]))
+    errs = R.run_abi(s, ops)[:, 0]
+    x = s.read_state()[0]
+    s.close()
+    assert np.all(np.abs(errs - ores[2]) <= 1e-5 * ores[2]), (errs, ores[2])
+    assert rel(x, f(ores[0])) < 1e-5
+    od = oracle.Solver(amp, mask, sup, starts=1, start=start, prec="d")
+    od.run(ops)
+    want = od.results()[0][2]
+    print(f"{shape}: fp16 storage", flush=True)
+    s = qpch.Solver(f(amp), f(mask), f(sup), starts=1, start=np.stack([f(start[0])]), storage="f16")
+    got = R.run_abi(s, ops)[:, 0]
+    s.close()
+    assert np.all(np.abs(got - want) <= 3e-3 * want), (got, want)

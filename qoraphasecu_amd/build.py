@@ -22,7 +22,8 @@ INCLUDE = os.path.join(ROOT, "include")
 CSRC = os.path.join(HERE, "csrc")
 ARCH = os.environ.get("QPCH_OFFLOAD_ARCH", "gfx950")
 
-HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.environ.get("HIPCC", f"{ROCM}/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 
 HIP_FLAGS = [
@@ -73,8 +74,8 @@ def build_host() -> str:
     out = host_lib()
     tmp = out + ".tmp"
     srcs = [os.path.join(CSRC, "host", s) for s in HOST_SOURCES]
-    _run([CXX, "-O2", "-std=c++20", "-fPIC", "-shared", "-Wall", f"-I{INCLUDE}", *srcs, "-o", tmp,
-          f"-L{HERE}", "-lqpch", "-Wl,-rpath,$ORIGIN"])
+    _run([CXX, "-O2", "-std=c++20", "-fPIC", "-shared", "-Wall", f"-I{INCLUDE}", f"-I{ROCM}/include", *srcs, "-o", tmp,
+          f"-L{HERE}", "-lqpch", f"-L{ROCM}/lib", "-lrocprofiler-sdk-roctx", f"-Wl,-rpath,$ORIGIN:{ROCM}/lib"])
     os.replace(tmp, out)
     return out
 

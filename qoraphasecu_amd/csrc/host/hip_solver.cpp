@@ -14,6 +14,7 @@
 #include "abi.hpp"
 #include "backends.hpp"
 #include "layout.hpp"
+#include "trace.hpp"
 
 namespace qpcu::detail {
 
@@ -44,7 +45,10 @@ void tune(qpch_problem& q) {
     return;
   }
   qpch_tuning t{};
-  qpch_check(qpch_autotune(&q, &t));
+  {
+    const TraceRange range("qpch autotune");
+    qpch_check(qpch_autotune(&q, &t));
+  }
   q.batch = t.batch;
   q.plane = t.plane;
   q.xcd = t.xcd;
@@ -116,6 +120,7 @@ class HipSolver final : public Solver {
 
   void run(const Recipe& recipe) override {
     validate(recipe);
+    const TraceRange range("iterations");
     const int capacity = recipe.iterations();
     run_iterations_ = 0;
     if (capacity > 0) qpch_check(qpch_solver_begin_run(h_, capacity));
@@ -290,6 +295,7 @@ class HipEnsemble final : public MultiSolver {
 
   void run(const Recipe& recipe) override {
     validate(recipe);
+    const TraceRange range("iterations");
     const int capacity = recipe.iterations();
     run_iterations_ = 0;
     if (capacity > 0) {

@@ -402,10 +402,10 @@ void launch_generic(float2* data, long long n_per_start, int batch, int N, long 
   }
   const dev::GenericPlan plan = generic_plan(N);
   const long long lines = outer * inner;
-  const size_t lds = sizeof(float2) * 3 * static_cast<size_t>(N);
-  hipLaunchKernelGGL(dev::generic_fft<float>, dim3(static_cast<unsigned>(lines), batch),
-                     dim3(dev::kGenericThreads), lds, s, data, n_per_start,
-                     static_cast<int>(inner), plan, forward ? 1 : 0, scale);
+  const long long lpb = dev::generic_lines_per_block(N);
+  hipLaunchKernelGGL(dev::generic_fft<float>, dim3(static_cast<unsigned>((lines + lpb - 1) / lpb), batch),
+                     dim3(dev::kGenericThreads), dev::generic_lds_bytes<float2>(N), s, data, n_per_start,
+                     static_cast<int>(inner), lines, plan, forward ? 1 : 0, scale);
   QPCH_HIP(hipGetLastError());
 }
 
@@ -417,10 +417,10 @@ void device_fft_f64(double2* data, const Dims& d, int batch, bool forward, hipSt
     if (N > 1024) throw InvalidArgument("the FP64 transform supports axes up to 1024");
     const dev::GenericPlan plan = generic_plan(N);
     const long long lines = d.outer(a) * d.inner(a);
-    const size_t lds = sizeof(double2) * 3 * static_cast<size_t>(N);
-    hipLaunchKernelGGL(dev::generic_fft<double>, dim3(static_cast<unsigned>(lines), batch),
-                       dim3(dev::kGenericThreads), lds, s, data, d.n, static_cast<int>(d.inner(a)), plan,
-                       forward ? 1 : 0, 1.0);
+    const long long lpb = dev::generic_lines_per_block(N);
+    hipLaunchKernelGGL(dev::generic_fft<double>, dim3(static_cast<unsigned>((lines + lpb - 1) / lpb), batch),
+                       dim3(dev::kGenericThreads), dev::generic_lds_bytes<double2>(N), s, data, d.n,
+                       static_cast<int>(d.inner(a)), lines, plan, forward ? 1 : 0, 1.0);
     QPCH_HIP(hipGetLastError());
   }
 }

@@ -41,43 +41,128 @@ __device__ __forceinline__ C gmul(C a, C b) {
   return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 
-// One line per work-group, autosort mixed-radix stages ping-ponging between two LDS
-// buffers; radix-r butterflies are evaluated as O(r^2) sums with the line's own W_N table
-// (evaluated in double, rounded once to Real).  Lines are addressed like the fused passes:
-// element k of line (o, c) of start s at s*n + o*N*inner + k*inner + c, line = o*inner + c
-// = blockIdx.x.  Real = float (the unfused schedule) or double (the FP64 adjoint).
+// Lines per work-group for a line of length N: enough that short lines keep the threads
+// of a stage busy (a radix-r stage has N/r butterflies per line).
+__host__ __device__ constexpr int generic_lines_per_block(int N) {
+  return N >= 1024 ? 1 : (1024 / N < 16 ? 1024 / N : 16);
+}
+
+// Dynamic LDS of generic_fft: the W_N table, two line buffers per line, the line bases.
+template <class C>
+__host__ __device__ constexpr size_t generic_lds_bytes(int N) {
+  const int lpb = generic_lines_per_block(N);
+  return sizeof(C) * static_cast<size_t>(N) * (1 + 2 * lpb) + sizeof(long long) * lpb;
+}
+
+// One autosort stage of radix r over the nl lines of a block.  Output j of butterfly t
+// (k = t mod L) is  sum_q W_N^(q k m) W_r^(j q) src[t + q N/r]  (m = N / (L r)), written to
+// dst[(t - k) r + k + j L].  R > 0 fixes the radix at compile time (inputs held in
+// registers, W_r indices folded); R = 0 takes r at run time and advances (j q) mod r by j
+// per term instead of dividing.
+template <int R, class C>
+__device__ __forceinline__ void generic_stage(const C* src, C* dst, const C* tw, int N, int r, int L,
+                                              int nl) {
+  const int rr = R > 0 ? R : r;
+  const int nr = N / rr;
+  const int m = N / (L * rr);
+  for (int t = threadIdx.x; t < nl * nr; t += blockDim.x) {
+    const int l = t / nr;
+    const int tl = t - l * nr;
+    const int k = tl % L;
+    const int outb = (tl - k) * rr + k;
+    const C* sl = src + l * N;
+    C* dl = dst + l * N;
+    if constexpr (R > 0) {
+      C v[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        v[q] = sl[tl + q * nr];
+        const int tq = q * k * m;  // stage twiddle W_{L r}^{q k}
+        if (tq) v[q] = gmul(v[q], tw[tq]);
+      }
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        C acc = {0, 0};
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          const int e = (j * q) % R;  // W_r^{j q} = W_N^{(j q mod r) N / r}
+          acc = gadd(acc, e ? gmul(v[q], tw[e * nr]) : v[q]);
+        }
+        dl[outb + j * L] = acc;
+      }
+    } else {
+      for (int j = 0; j < rr; ++j) {
+        C acc = {0, 0};
+        int e = 0;  // (j q) mod r
+        for (int q = 0; q < rr; ++q) {
+          C v = sl[tl + q * nr];
+          const int tq = q * k * m;
+          if (tq) v = gmul(v, tw[tq]);
+          acc = gadd(acc, e ? gmul(v, tw[e * nr]) : v);
+          e += j;
+          if (e >= rr) e -= rr;
+        }
+        dl[outb + j * L] = acc;
+      }
+    }
+  }
+}
+
+// generic_lines_per_block(N) lines per work-group, autosort mixed-radix stages ping-ponging
+// between two LDS buffers; radix-r butterflies are evaluated as O(r^2) sums with the
+// line's own W_N table (evaluated in double, rounded once to Real).  Lines are addressed
+// like the fused passes: element k of line L = o*inner + c of start s at
+// s*n + o*N*inner + k*inner + c.  Loads and stores walk adjacent lines first when the
+// axis is strided (inner > 1) and along the line otherwise, so that neighbouring threads
+// touch neighbouring addresses.  Real = float (the unfused schedule) or double (the FP64
+// adjoint).
 template <class Real>
 __global__ void __launch_bounds__(kGenericThreads)
-    generic_fft(typename Cplx<Real>::T* data, long long n_per_start, int inner, GenericPlan plan,
-                int forward, Real scale) {
+    generic_fft(typename Cplx<Real>::T* data, long long n_per_start, int inner, long long lines,
+                GenericPlan plan, int forward, Real scale) {
   using C = typename Cplx<Real>::T;
   extern __shared__ unsigned char sm_raw[];
   C* sm = reinterpret_cast<C*>(sm_raw);
   const int N = plan.n;
-  C* bufa = sm;
-  C* bufb = sm + N;
-  C* tw = sm + 2 * N;
+  const int lpb = generic_lines_per_block(N);
+  C* tw = sm;
+  C* bufa = sm + N;
+  C* bufb = bufa + lpb * N;
+  long long* lbase = reinterpret_cast<long long*>(bufb + lpb * N);
   for (int j = threadIdx.x; j < N; j += blockDim.x) {
     double s, c;
     sincospi(2.0 * static_cast<double>(j) / static_cast<double>(N), &s, &c);
     tw[j] = {static_cast<Real>(c), static_cast<Real>(forward ? -s : s)};
   }
-  const int o = blockIdx.x / inner;
-  const int c = blockIdx.x % inner;
-  const long long base = static_cast<long long>(blockIdx.y) * n_per_start +
-                         static_cast<long long>(o) * N * inner + c;
-  for (int k = threadIdx.x; k < N; k += blockDim.x) bufa[k] = data[base + static_cast<long long>(k) * inner];
+  const long long line0 = static_cast<long long>(blockIdx.x) * lpb;
+  const int nl = static_cast<int>(lines - line0 < lpb ? lines - line0 : lpb);
+  if (static_cast<int>(threadIdx.x) < nl) {
+    const long long L = line0 + threadIdx.x;
+    lbase[threadIdx.x] = static_cast<long long>(blockIdx.y) * n_per_start + (L / inner) * N * inner + L % inner;
+  }
+  __syncthreads();
+  const bool strided = inner > 1;
+  for (int e = threadIdx.x; e < nl * N; e += blockDim.x) {
+    const int l = strided ? e % nl : e / N;
+    const int k = strided ? e / nl : e % N;
+    bufa[l * N + k] = data[lbase[l] + static_cast<long long>(k) * inner];
+  }
   __syncthreads();
   C* src = bufa;
   C* dst = bufb;
   if (plan.nfac == 0) {
-    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+    for (int t = threadIdx.x; t < nl * N; t += blockDim.x) {
+      const int l = t / N;
+      const int j = t - l * N;
+      const C* sl = src + l * N;
       C acc = {0, 0};
+      int idx = 0;  // (j q) mod N
       for (int q = 0; q < N; ++q) {
-        const C w = tw[static_cast<int>((static_cast<long long>(j) * q) % N)];
-        acc = gadd(acc, gmul(src[q], w));
+        acc = gadd(acc, gmul(sl[q], tw[idx]));
+        idx += j;
+        if (idx >= N) idx -= N;
       }
-      dst[j] = acc;
+      dst[t] = acc;
     }
     __syncthreads();
     C* t = src;
@@ -87,22 +172,12 @@ __global__ void __launch_bounds__(kGenericThreads)
     int L = 1;
     for (int s = 0; s < plan.nfac; ++s) {
       const int r = plan.fac[s];
-      const int m = N / (L * r);
-      const int nr = N / r;
-      for (int t = threadIdx.x; t < nr; t += blockDim.x) {
-        const int k = t % L;
-        const int outb = (t - k) * r + k;
-        for (int j = 0; j < r; ++j) {
-          C acc = {0, 0};
-          for (int q = 0; q < r; ++q) {
-            C v = src[t + q * nr];
-            const int tq = q * k * m;  // stage twiddle W_{L r}^{q k}
-            if (tq) v = gmul(v, tw[tq]);
-            const int e = ((j * q) % r) * nr;  // W_r^{j q} = W_N^{(j q mod r) N / r}
-            acc = gadd(acc, e ? gmul(v, tw[e]) : v);
-          }
-          dst[outb + j * L] = acc;
-        }
+      switch (r) {
+        case 2: generic_stage<2>(src, dst, tw, N, r, L, nl); break;
+        case 3: generic_stage<3>(src, dst, tw, N, r, L, nl); break;
+        case 4: generic_stage<4>(src, dst, tw, N, r, L, nl); break;
+        case 5: generic_stage<5>(src, dst, tw, N, r, L, nl); break;
+        default: generic_stage<0>(src, dst, tw, N, r, L, nl); break;
       }
       __syncthreads();
       C* tt = src;
@@ -111,9 +186,11 @@ __global__ void __launch_bounds__(kGenericThreads)
       L *= r;
     }
   }
-  for (int k = threadIdx.x; k < N; k += blockDim.x) {
-    const C v = src[k];
-    data[base + static_cast<long long>(k) * inner] = {v.x * scale, v.y * scale};
+  for (int e = threadIdx.x; e < nl * N; e += blockDim.x) {
+    const int l = strided ? e % nl : e / N;
+    const int k = strided ? e / nl : e % N;
+    const C v = src[l * N + k];
+    data[lbase[l] + static_cast<long long>(k) * inner] = {v.x * scale, v.y * scale};
   }
 }
 

@@ -26,7 +26,10 @@ def test_device_info(qpch):
 @pytest.mark.parametrize(
     "shape",
     [(64, 64), (256, 256), (512, 512), (128, 64), (8, 4096), (17, 24), (21, 15), (16, 20, 12),
-     (32, 32, 32), (128, 128, 16), (15, 16, 17), (2, 2)],
+     (32, 32, 32), (128, 128, 16), (15, 16, 17), (2, 2),
+     # the generic kernel: a prime above 64 (one plain DFT per line), radices 7 and 11,
+     # a 1536-point strided axis (one line per block) beside 6-point rows (16 lines per block)
+     (67, 24), (77, 30), (1536, 6), (9, 131, 10)],
 )
 @pytest.mark.parametrize("forward", [True, False])
 def test_fft_matches_oracle(qpch, shape, forward):

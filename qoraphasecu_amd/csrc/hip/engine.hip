@@ -179,6 +179,19 @@ bool fusable(const Dims& d) {
   return d.n * static_cast<long long>(sizeof(float2)) < (1LL << 31);
 }
 
+// The fused passes also cover fp32 shapes whose other axes are any length up to
+// kGenericMaxN: those axes run the generic pass kernel (generic.hpp).
+bool generic_fusable(const Dims& d) {
+  for (int a = 0; a < d.nd; ++a) {
+    const int N = d.d[a];
+    const bool pow2_ok = is_pow2(N) && N >= 2 && N <= kMaxFused;
+    if (!pow2_ok && (N < 2 || N > dev::kGenericMaxN)) return false;
+  }
+  return d.n * static_cast<long long>(sizeof(float2)) < (1LL << 31);
+}
+
+bool generic_axis(int N) { return !is_pow2(N) || N > kMaxFused; }
+
 int grid_for(long long work, int threads = 256, int cap = 2048) {
   long long g = (work + threads - 1) / threads;
   return static_cast<int>(std::max<long long>(1, std::min<long long>(g, cap)));
@@ -409,6 +422,36 @@ void launch_generic(float2* data, long long n_per_start, int batch, int N, long 
   QPCH_HIP(hipGetLastError());
 }
 
+// A fused pass (line_pass semantics) along an axis of any length up to kGenericMaxN, fp32.
+void launch_generic_pass(int N, Mode m, const PassArgs& a, int starts, hipStream_t s) {
+  if (N > dev::kGenericMaxN) throw InvalidArgument("no generic pass for axis length " + std::to_string(N));
+  const dev::GenericPlan plan = generic_plan(N);
+  const long long lines = static_cast<long long>(a.outer) * a.inner;
+  const long long lpb = dev::generic_lines_per_block(N);
+  const dim3 grid(static_cast<unsigned>((lines + lpb - 1) / lpb), starts);
+  const size_t lds = dev::generic_pass_lds_bytes(N);
+  switch (m) {
+    case Mode::kFwd: hipLaunchKernelGGL(dev::generic_pass<Mode::kFwd>, grid, dim3(dev::kGenericThreads), lds, s, a, plan, lines); break;
+    case Mode::kInv: hipLaunchKernelGGL(dev::generic_pass<Mode::kInv>, grid, dim3(dev::kGenericThreads), lds, s, a, plan, lines); break;
+    case Mode::kModulus: hipLaunchKernelGGL(dev::generic_pass<Mode::kModulus>, grid, dim3(dev::kGenericThreads), lds, s, a, plan, lines); break;
+    case Mode::kUpdate: hipLaunchKernelGGL(dev::generic_pass<Mode::kUpdate>, grid, dim3(dev::kGenericThreads), lds, s, a, plan, lines); break;
+    default: throw InvalidArgument("the generic pass has no gradient mode");
+  }
+  QPCH_HIP(hipGetLastError());
+}
+
+// Error partials per start of the modulus pass along axis 0 (line_pass: one per wave;
+// generic_pass: one per work-group).
+template <int EB>
+int axis0_partials(const Dims& d, int radix) {
+  const int o = static_cast<int>(d.outer(0)), in = static_cast<int>(d.inner(0));
+  if (generic_axis(d.d[0])) {
+    const long long lpb = dev::generic_lines_per_block(d.d[0]);
+    return static_cast<int>((static_cast<long long>(o) * in + lpb - 1) / lpb);
+  }
+  return pass_partials<EB>(d.d[0], in > 1, o, in, radix);
+}
+
 // Unnormalised FP64 transform along every axis (generic kernel, lines up to 1024).
 void device_fft_f64(double2* data, const Dims& d, int batch, bool forward, hipStream_t s) {
   for (int a = d.nd - 1; a >= 0; --a) {
@@ -515,7 +558,8 @@ struct SolverImpl {
   int storage = QPCH_F32;
   bool mixed = false;
   int schedule = QPCH_GRAPH;
-  bool fused = true;
+  bool fused = true;  // fused passes (generic ones on axes that are not powers of two)
+  bool pow2 = true;   // every axis a power of two <= 4096: FFT blur, stacked problems, fp16 / bf16
   hipStream_t stream = nullptr;
   hipStream_t on = nullptr;   // where run_pass and chunk launch: the solver's stream or a lane
   int lanes = 1;              // launch groups in flight at once (each on its own stream)
@@ -624,8 +668,7 @@ struct SolverImpl {
     p.amp = amp_signed.get();
     p.amp_stride = problems > 1 ? n : 0;
     p.amp_group = per_problem;
-    p.partial_stride = pass_partials<sizeof(St)>(dims.d[0], dims.inner(0) > 1, static_cast<int>(dims.outer(0)),
-                                                 static_cast<int>(dims.inner(0)), radix);
+    p.partial_stride = axis0_partials<sizeof(St)>(dims, radix);
     p.partials = partials.get() + static_cast<long long>(first) * p.partial_stride;  // own rows per group
     p.errs = errs.get();
     p.err_stride = starts;
@@ -642,6 +685,14 @@ struct SolverImpl {
       p.outer = dims.d[0];  // one tile per plane
       launch_plane<St>(dims.d[1], dims.d[2], p, count, on ? on : stream);
       return;
+    }
+    if (generic_axis(dims.d[ps.axis])) {
+      if constexpr (!std::is_same_v<St, float2>) {
+        throw InvalidArgument("fp16/bf16 storage needs power-of-two extents");
+      } else {
+        launch_generic_pass(dims.d[ps.axis], ps.mode, p, count, on ? on : stream);
+        return;
+      }
     }
     launch_pass<St>(dims.d[ps.axis], inner > 1, ps.mode, p, count, on ? on : stream);
   }
@@ -953,7 +1004,7 @@ struct SolverImpl {
 
   void shrinkwrap(float threshold, float sigma, int method) {
     if (!(threshold > 0.0f) || !(sigma >= 0.0f)) throw InvalidArgument("Shrinkwrap needs threshold > 0, sigma >= 0");
-    if (fused) {
+    if (fused && pow2) {
       const long long total = n * starts;
       if (sw_a.size() < static_cast<size_t>(total)) sw_a.resize(total);
       if (sw_c.size() < static_cast<size_t>(total)) sw_c.resize(total);
@@ -1028,7 +1079,8 @@ struct SolverImpl {
     // QPCH_EAGER=1 launches the graph schedule's kernels one by one (for kernel tracers
     // that do not follow graph launches); the kernels and results are unchanged.
     if (schedule == QPCH_GRAPH && eager_launches()) schedule = QPCH_FUSED;
-    fused = fusable(dims) && schedule != QPCH_UNFUSED;
+    pow2 = fusable(dims);
+    fused = (pow2 || (storage == QPCH_F32 && generic_fusable(dims))) && schedule != QPCH_UNFUSED;
     plane3d = fused && dims.nd == 3 && plane_supported(dims.d[1], dims.d[2]) && p.plane != 2;
     if (p.xcd != 0 && p.xcd != 2 && p.xcd != 3) throw InvalidArgument("xcd must be 0, 2 or 3");
     xcd_mode = p.xcd;
@@ -1038,7 +1090,7 @@ struct SolverImpl {
       throw InvalidArgument("fp16/bf16 storage needs power-of-two extents between 2 and 4096");
     }
     if (storage != QPCH_F32 && schedule == QPCH_UNFUSED) throw InvalidArgument("fp16/bf16 storage needs a fused schedule");
-    if (problems > 1 && !fused) throw InvalidArgument("several problems in one solver need power-of-two extents");
+    if (problems > 1 && !pow2) throw InvalidArgument("several problems in one solver need power-of-two extents");
     for (int a = 0; a < dims.nd; ++a) {
       if (!(is_pow2(dims.d[a]) && dims.d[a] <= kMaxFused) && dims.d[a] > dev::kGenericMaxN) {
         throw InvalidArgument("axis length " + std::to_string(dims.d[a]) + " is not supported");
@@ -1103,11 +1155,7 @@ struct SolverImpl {
     counters.resize(groups);
     // partial sums of the error: one per block of the modulus pass of each start
     int pb = grid_for(n, 256, 1024);
-    if (fused) {
-      const int o = static_cast<int>(dims.outer(0)), in = static_cast<int>(dims.inner(0));
-      pb = std::max({pb, pass_partials<8>(dims.d[0], in > 1, o, in, radix),
-                     pass_partials<4>(dims.d[0], in > 1, o, in, radix)});
-    }
+    if (fused) pb = std::max({pb, axis0_partials<8>(dims, radix), axis0_partials<4>(dims, radix)});
     partials.resize(static_cast<size_t>(pb) * starts);  // groups may run at once: rows of their own
 
     // starting objects

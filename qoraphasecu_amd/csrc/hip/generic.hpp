@@ -1,6 +1,8 @@
-// Fallback kernels for axes that are not powers of two (any length up to 2048), and the
-// element-wise kernels of the unfused iteration that goes with them.  These keep every
-// shape the reference accepts reconstructible; the fused passes are the fast path.
+// Kernels for axes that are not powers of two (any length up to 2048): the fused passes of
+// line_pass in a generic mixed-radix form (generic_pass, used by the fused schedules on
+// those axes), the plain line transform (generic_fft, for the standalone operators, the
+// unfused schedule and the FP64 adjoint), and the element-wise kernels of the unfused
+// iteration.  Power-of-two axes up to 4096 use the radix-16 passes of passes.hpp.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -54,12 +56,20 @@ __host__ __device__ constexpr size_t generic_lds_bytes(int N) {
   return sizeof(C) * static_cast<size_t>(N) * (1 + 2 * lpb) + sizeof(long long) * lpb;
 }
 
+// Twiddle j of a table, conjugated for an inverse transform read from a forward table
+// (negation is exact, so the values equal those of an inverse table).
+template <bool INV, class C>
+__device__ __forceinline__ C twid(const C* tw, int j) {
+  const C w = tw[j];
+  return INV ? C{w.x, -w.y} : w;
+}
+
 // One autosort stage of radix r over the nl lines of a block.  Output j of butterfly t
 // (k = t mod L) is  sum_q W_N^(q k m) W_r^(j q) src[t + q N/r]  (m = N / (L r)), written to
 // dst[(t - k) r + k + j L].  R > 0 fixes the radix at compile time (inputs held in
 // registers, W_r indices folded); R = 0 takes r at run time and advances (j q) mod r by j
-// per term instead of dividing.
-template <int R, class C>
+// per term instead of dividing.  INV conjugates the table's twiddles.
+template <int R, bool INV, class C>
 __device__ __forceinline__ void generic_stage(const C* src, C* dst, const C* tw, int N, int r, int L,
                                               int nl) {
   const int rr = R > 0 ? R : r;
@@ -78,7 +88,7 @@ __device__ __forceinline__ void generic_stage(const C* src, C* dst, const C* tw,
       for (int q = 0; q < R; ++q) {
         v[q] = sl[tl + q * nr];
         const int tq = q * k * m;  // stage twiddle W_{L r}^{q k}
-        if (tq) v[q] = gmul(v[q], tw[tq]);
+        if (tq) v[q] = gmul(v[q], twid<INV>(tw, tq));
       }
 #pragma unroll
       for (int j = 0; j < R; ++j) {
@@ -86,7 +96,7 @@ __device__ __forceinline__ void generic_stage(const C* src, C* dst, const C* tw,
 #pragma unroll
         for (int q = 0; q < R; ++q) {
           const int e = (j * q) % R;  // W_r^{j q} = W_N^{(j q mod r) N / r}
-          acc = gadd(acc, e ? gmul(v[q], tw[e * nr]) : v[q]);
+          acc = gadd(acc, e ? gmul(v[q], twid<INV>(tw, e * nr)) : v[q]);
         }
         dl[outb + j * L] = acc;
       }
@@ -97,14 +107,58 @@ __device__ __forceinline__ void generic_stage(const C* src, C* dst, const C* tw,
         for (int q = 0; q < rr; ++q) {
           C v = sl[tl + q * nr];
           const int tq = q * k * m;
-          if (tq) v = gmul(v, tw[tq]);
-          acc = gadd(acc, e ? gmul(v, tw[e * nr]) : v);
+          if (tq) v = gmul(v, twid<INV>(tw, tq));
+          acc = gadd(acc, e ? gmul(v, twid<INV>(tw, e * nr)) : v);
           e += j;
           if (e >= rr) e -= rr;
         }
         dl[outb + j * L] = acc;
       }
     }
+  }
+}
+
+// The unnormalised DFT of the nl lines in src (LDS, line l at l*N), by the plan's stages or,
+// for nfac == 0, one plain DFT per line; src and dst swap roles once per stage, so the
+// result is in src on return.  Ends with a barrier.
+template <bool INV, class C>
+__device__ __forceinline__ void generic_transform(C*& src, C*& dst, const C* tw, const GenericPlan& plan, int nl) {
+  const int N = plan.n;
+  if (plan.nfac == 0) {
+    for (int t = threadIdx.x; t < nl * N; t += blockDim.x) {
+      const int l = t / N;
+      const int j = t - l * N;
+      const C* sl = src + l * N;
+      C acc = {0, 0};
+      int idx = 0;  // (j q) mod N
+      for (int q = 0; q < N; ++q) {
+        acc = gadd(acc, gmul(sl[q], twid<INV>(tw, idx)));
+        idx += j;
+        if (idx >= N) idx -= N;
+      }
+      dst[t] = acc;
+    }
+    __syncthreads();
+    C* t = src;
+    src = dst;
+    dst = t;
+    return;
+  }
+  int L = 1;
+  for (int s = 0; s < plan.nfac; ++s) {
+    const int r = plan.fac[s];
+    switch (r) {
+      case 2: generic_stage<2, INV>(src, dst, tw, N, r, L, nl); break;
+      case 3: generic_stage<3, INV>(src, dst, tw, N, r, L, nl); break;
+      case 4: generic_stage<4, INV>(src, dst, tw, N, r, L, nl); break;
+      case 5: generic_stage<5, INV>(src, dst, tw, N, r, L, nl); break;
+      default: generic_stage<0, INV>(src, dst, tw, N, r, L, nl); break;
+    }
+    __syncthreads();
+    C* tt = src;
+    src = dst;
+    dst = tt;
+    L *= r;
   }
 }
 
@@ -150,47 +204,133 @@ __global__ void __launch_bounds__(kGenericThreads)
   __syncthreads();
   C* src = bufa;
   C* dst = bufb;
-  if (plan.nfac == 0) {
-    for (int t = threadIdx.x; t < nl * N; t += blockDim.x) {
-      const int l = t / N;
-      const int j = t - l * N;
-      const C* sl = src + l * N;
-      C acc = {0, 0};
-      int idx = 0;  // (j q) mod N
-      for (int q = 0; q < N; ++q) {
-        acc = gadd(acc, gmul(sl[q], tw[idx]));
-        idx += j;
-        if (idx >= N) idx -= N;
-      }
-      dst[t] = acc;
-    }
-    __syncthreads();
-    C* t = src;
-    src = dst;
-    dst = t;
-  } else {
-    int L = 1;
-    for (int s = 0; s < plan.nfac; ++s) {
-      const int r = plan.fac[s];
-      switch (r) {
-        case 2: generic_stage<2>(src, dst, tw, N, r, L, nl); break;
-        case 3: generic_stage<3>(src, dst, tw, N, r, L, nl); break;
-        case 4: generic_stage<4>(src, dst, tw, N, r, L, nl); break;
-        case 5: generic_stage<5>(src, dst, tw, N, r, L, nl); break;
-        default: generic_stage<0>(src, dst, tw, N, r, L, nl); break;
-      }
-      __syncthreads();
-      C* tt = src;
-      src = dst;
-      dst = tt;
-      L *= r;
-    }
-  }
+  generic_transform<false>(src, dst, tw, plan, nl);  // the table carries the direction
   for (int e = threadIdx.x; e < nl * N; e += blockDim.x) {
     const int l = strided ? e % nl : e / N;
     const int k = strided ? e / nl : e % N;
     const C v = src[l * N + k];
     data[lbase[l] + static_cast<long long>(k) * inner] = {v.x * scale, v.y * scale};
+  }
+}
+
+// Dynamic LDS of generic_pass: the forward W_N table, two line buffers per line, the line
+// bases and one FP64 slot per wave.
+__host__ __device__ constexpr size_t generic_pass_lds_bytes(int N) {
+  return generic_lds_bytes<float2>(N) + sizeof(double) * (kGenericThreads / 64);
+}
+
+// The fused passes of line_pass (passes.hpp) for an axis of any length up to
+// kGenericMaxN, fp32 storage: the same modes, arguments and scale conventions, with
+// generic_lines_per_block(N) lines of one start per work-group (blockIdx.y = start of the
+// launch).  The modulus pass leaves one FP64 error partial per work-group (its waves'
+// sums added in order); the update pass's work-group 0 of each start sums the start's
+// partial_stride partials in order, as line_pass does.
+template <Mode M>
+__global__ void __launch_bounds__(kGenericThreads)
+    generic_pass(PassArgs p, GenericPlan plan, long long lines) {
+  extern __shared__ unsigned char sm_raw[];
+  float2* sm = reinterpret_cast<float2*>(sm_raw);
+  const int N = plan.n;
+  const int lpb = generic_lines_per_block(N);
+  float2* tw = sm;
+  float2* bufa = sm + N;
+  float2* bufb = bufa + lpb * N;
+  long long* lbase = reinterpret_cast<long long*>(bufb + lpb * N);
+  double* red = reinterpret_cast<double*>(lbase + lpb);
+  for (int j = threadIdx.x; j < N; j += blockDim.x) {
+    double sn, cs;
+    sincospi(2.0 * static_cast<double>(j) / static_cast<double>(N), &sn, &cs);
+    tw[j] = {static_cast<float>(cs), static_cast<float>(-sn)};  // forward W_N^j
+  }
+  const int start = blockIdx.y;
+  const long long soff = static_cast<long long>(start) * p.n;
+  const long long line0 = static_cast<long long>(blockIdx.x) * lpb;
+  const int nl = static_cast<int>(lines - line0 < lpb ? lines - line0 : lpb);
+  const int inner = p.inner;
+  if (static_cast<int>(threadIdx.x) < nl) {
+    const long long L = line0 + threadIdx.x;
+    lbase[threadIdx.x] = (L / inner) * N * inner + L % inner;  // element 0 of the line, within the start
+  }
+  __syncthreads();
+  // element e of the block's work: line l, position k (adjacent lines first on a strided axis)
+  const bool strided = inner > 1;
+  auto at = [&](int e, int& l, int& k) {
+    l = strided ? e % nl : e / N;
+    k = strided ? e / nl : e % N;
+  };
+  float2* w = static_cast<float2*>(p.w) + soff;
+  const float2* in = (M == Mode::kFwd || M == Mode::kInv) && p.src ? static_cast<const float2*>(p.src) + soff : w;
+  for (int e = threadIdx.x; e < nl * N; e += blockDim.x) {
+    int l, k;
+    at(e, l, k);
+    bufa[l * N + k] = in[lbase[l] + static_cast<long long>(k) * inner];
+  }
+  __syncthreads();
+  float2* src = bufa;
+  float2* dst = bufb;
+  float out_scale = p.out_scale;
+  if constexpr (M == Mode::kFwd) {
+    generic_transform<false>(src, dst, tw, plan, nl);
+  } else if constexpr (M == Mode::kInv) {
+    generic_transform<true>(src, dst, tw, plan, nl);
+  } else if constexpr (M == Mode::kModulus) {
+    generic_transform<false>(src, dst, tw, plan, nl);
+    const long long arow = p.amp_stride * ((p.err_offset + start) / (p.amp_group > 1 ? p.amp_group : 1));
+    const float io = p.in_scale * p.out_scale;
+    double acc = 0.0;
+    for (int e = threadIdx.x; e < nl * N; e += blockDim.x) {
+      int l, k;
+      at(e, l, k);
+      const float a = p.amp[arow + lbase[l] + static_cast<long long>(k) * inner];
+      src[l * N + k] = project_scaled(src[l * N + k], a, p.in_scale, io, acc);
+    }
+    const double ws = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ws;
+    if (threadIdx.x == 0 && blockIdx.x == 0 && start == 0 && p.counter) atomicAdd(p.counter, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int v = 0; v < static_cast<int>(blockDim.x) / 64; ++v) t += red[v];
+      p.partials[static_cast<long long>(start) * p.partial_stride + blockIdx.x] = t;
+    }
+    generic_transform<true>(src, dst, tw, plan, nl);
+    out_scale = 1.0f;  // folded into the projection
+  } else {  // kUpdate
+    generic_transform<true>(src, dst, tw, plan, nl);
+    float2* x = static_cast<float2*>(p.x) + soff;
+    const std::uint8_t* sup = p.support + soff;
+    for (int e = threadIdx.x; e < nl * N; e += blockDim.x) {
+      int l, k;
+      at(e, l, k);
+      const long long i = lbase[l] + static_cast<long long>(k) * inner;
+      const float2 v = src[l * N + k];
+      const float2 pm = {v.x * p.in_scale, v.y * p.in_scale};
+      float2 nx = pm;
+      if (!sup[i]) {
+        const float2 xo = x[i];
+        nx = {p.a * xo.x + p.b * pm.x, p.a * xo.y + p.b * pm.y};
+      }
+      x[i] = nx;
+      src[l * N + k] = nx;
+    }
+    __syncthreads();
+    generic_transform<false>(src, dst, tw, plan, nl);
+    if (blockIdx.x == 0 && threadIdx.x < 64 && p.errs) {
+      const double t = partial_sum(p.partials + static_cast<long long>(start) * p.partial_stride, p.partial_stride,
+                                   threadIdx.x);
+      if (threadIdx.x == 0) {
+        const int slot = *p.counter - 1;
+        if (slot >= 0 && slot < p.err_cap) {
+          p.errs[static_cast<long long>(slot) * p.err_stride + p.err_offset + start] = t;
+        }
+      }
+    }
+  }
+  for (int e = threadIdx.x; e < nl * N; e += blockDim.x) {
+    int l, k;
+    at(e, l, k);
+    const float2 v = src[l * N + k];
+    w[lbase[l] + static_cast<long long>(k) * inner] = {v.x * out_scale, v.y * out_scale};
   }
 }
 

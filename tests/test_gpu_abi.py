@@ -87,11 +87,14 @@ def _problem(shape, m, masked=False):
 
 
 @pytest.mark.parametrize("shape", [(64, 64), (48, 40), (32, 32, 32), (256, 256), (16, 8, 32), (8, 128, 128),
-                                   (16, 64, 32)])
+                                   (16, 64, 32), (64, 48), (12, 20, 18), (16, 24, 32), (24, 64, 64), (67, 30)])
 @pytest.mark.parametrize("schedule,radix", [("graph", 0), ("fused", 0), ("unfused", 0), ("graph", 8)])
 def test_teacher_forced_iteration(qpch, shape, schedule, radix):
     """One iteration from the same start: error within 1e-5 relative, object within 1e-5
-    (radix 8: the radix-8 line passes on the axes of 64 or more)."""
+    (radix 8: the radix-8 line passes on the axes of 64 or more).  Axes that are not powers
+    of two run the generic passes: 48 x 40 and 12 x 20 x 18 entirely, 64 x 48 and 16 x 24 x
+    32 beside power-of-two line passes, 24 x 64 x 64 a generic z pass before the plane pass,
+    67 x 30 a prime axis (one plain DFT per line)."""
     obj, amp, mask, sup = _problem(shape, max(4, shape[-1] // 4), masked=True)
     start = np.stack([synthetic.random_field(obj.size, 11).reshape(shape),
                       synthetic.random_field(obj.size, 12).reshape(shape)])

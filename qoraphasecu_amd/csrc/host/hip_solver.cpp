@@ -22,9 +22,12 @@ namespace {
 
 constexpr double kEps = 1e-12;  // src/core/pointwise.hpp:16
 
+// Shapes the engine runs with fused passes (power-of-two axes up to 4096, other axes up to
+// 2048 in the generic passes), whose launch configuration the autotuner can choose.
 bool is_fusable(const Shape& shape) {
   for (std::size_t a : shape) {
-    if (a < 2 || a > 4096 || (a & (a - 1)) != 0) return false;
+    const bool pow2 = a >= 2 && a <= 4096 && (a & (a - 1)) == 0;
+    if (!pow2 && (a < 2 || a > 2048)) return false;
   }
   return true;
 }

@@ -56,6 +56,16 @@ __host__ __device__ constexpr size_t generic_lds_bytes(int N) {
   return sizeof(C) * static_cast<size_t>(N) * (1 + 2 * lpb) + sizeof(long long) * lpb;
 }
 
+// a / b for 0 <= a < 2^20 and 1 <= b: the quotient from the float reciprocal is off by at
+// most one at these sizes, and one step corrects it (exact integer result, a few
+// instructions instead of an integer division sequence).
+__device__ __forceinline__ int small_div(int a, int b) {
+  int q = static_cast<int>(static_cast<float>(a) * __builtin_amdgcn_rcpf(static_cast<float>(b)));
+  const int r = a - q * b;
+  q += (r >= b) - (r < 0);
+  return q;
+}
+
 // Twiddle j of a table, conjugated for an inverse transform read from a forward table
 // (negation is exact, so the values equal those of an inverse table).
 template <bool INV, class C>
@@ -76,9 +86,9 @@ __device__ __forceinline__ void generic_stage(const C* src, C* dst, const C* tw,
   const int nr = N / rr;
   const int m = N / (L * rr);
   for (int t = threadIdx.x; t < nl * nr; t += blockDim.x) {
-    const int l = t / nr;
+    const int l = small_div(t, nr);
     const int tl = t - l * nr;
-    const int k = tl % L;
+    const int k = tl - small_div(tl, L) * L;
     const int outb = (tl - k) * rr + k;
     const C* sl = src + l * N;
     C* dl = dst + l * N;
@@ -126,7 +136,7 @@ __device__ __forceinline__ void generic_transform(C*& src, C*& dst, const C* tw,
   const int N = plan.n;
   if (plan.nfac == 0) {
     for (int t = threadIdx.x; t < nl * N; t += blockDim.x) {
-      const int l = t / N;
+      const int l = small_div(t, N);
       const int j = t - l * N;
       const C* sl = src + l * N;
       C acc = {0, 0};
@@ -197,8 +207,8 @@ __global__ void __launch_bounds__(kGenericThreads)
   __syncthreads();
   const bool strided = inner > 1;
   for (int e = threadIdx.x; e < nl * N; e += blockDim.x) {
-    const int l = strided ? e % nl : e / N;
-    const int k = strided ? e / nl : e % N;
+    const int l = strided ? e - small_div(e, nl) * nl : small_div(e, N);
+    const int k = strided ? small_div(e, nl) : e - small_div(e, N) * N;
     bufa[l * N + k] = data[lbase[l] + static_cast<long long>(k) * inner];
   }
   __syncthreads();
@@ -206,8 +216,8 @@ __global__ void __launch_bounds__(kGenericThreads)
   C* dst = bufb;
   generic_transform<false>(src, dst, tw, plan, nl);  // the table carries the direction
   for (int e = threadIdx.x; e < nl * N; e += blockDim.x) {
-    const int l = strided ? e % nl : e / N;
-    const int k = strided ? e / nl : e % N;
+    const int l = strided ? e - small_div(e, nl) * nl : small_div(e, N);
+    const int k = strided ? small_div(e, nl) : e - small_div(e, N) * N;
     const C v = src[l * N + k];
     data[lbase[l] + static_cast<long long>(k) * inner] = {v.x * scale, v.y * scale};
   }
@@ -255,8 +265,13 @@ __global__ void __launch_bounds__(kGenericThreads)
   // element e of the block's work: line l, position k (adjacent lines first on a strided axis)
   const bool strided = inner > 1;
   auto at = [&](int e, int& l, int& k) {
-    l = strided ? e % nl : e / N;
-    k = strided ? e / nl : e % N;
+    if (strided) {
+      k = small_div(e, nl);
+      l = e - k * nl;
+    } else {
+      l = small_div(e, N);
+      k = e - l * N;
+    }
   };
   float2* w = static_cast<float2*>(p.w) + soff;
   const float2* in = (M == Mode::kFwd || M == Mode::kInv) && p.src ? static_cast<const float2*>(p.src) + soff : w;

@@ -15,7 +15,7 @@ import importlib.util as _ilu
 # torch would start a second one and find no GPU.  Loading torch first (when it is
 # installed) lets the engine bind to the same runtime, so both can share the device.
 if _ilu.find_spec("torch") is not None:
-    import torch  # noqa: F401
+    import torch as _torch  # noqa: F401
 
 from .api import (
     Ensemble,

@@ -16,6 +16,9 @@ namespace qpch::dev {
 
 constexpr int kGenericMaxN = 2048;
 constexpr int kGenericThreads = 256;
+// generic_pass keeps a block's elements (at most 2048: one line up to kGenericMaxN, or up
+// to 16 lines of fewer than 1024) in 2048 / kGenericThreads registers per thread.
+static_assert(kGenericMaxN <= 2048 && 2048 % kGenericThreads == 0, "generic_pass register slots");
 
 struct GenericPlan {
   int n;
@@ -247,11 +250,6 @@ __global__ void __launch_bounds__(kGenericThreads)
   float2* bufb = bufa + lpb * N;
   long long* lbase = reinterpret_cast<long long*>(bufb + lpb * N);
   double* red = reinterpret_cast<double*>(lbase + lpb);
-  for (int j = threadIdx.x; j < N; j += blockDim.x) {
-    double sn, cs;
-    sincospi(2.0 * static_cast<double>(j) / static_cast<double>(N), &sn, &cs);
-    tw[j] = {static_cast<float>(cs), static_cast<float>(-sn)};  // forward W_N^j
-  }
   const int start = blockIdx.y;
   const long long soff = static_cast<long long>(start) * p.n;
   const long long line0 = static_cast<long long>(blockIdx.x) * lpb;
@@ -262,9 +260,31 @@ __global__ void __launch_bounds__(kGenericThreads)
     lbase[threadIdx.x] = (L / inner) * N * inner + L % inner;  // element 0 of the line, within the start
   }
   __syncthreads();
-  // element e of the block's work: line l, position k (adjacent lines first on a strided axis)
+  // Element e = threadIdx.x + u * kGenericThreads of the block's work is line l, position k
+  // (adjacent lines first on a strided axis).  A block holds at most 2048 elements, so a
+  // thread has at most kPer.  Every slot loads (slots past the end re-read the last
+  // element), so that all of a thread's global loads are in flight before the first use;
+  // only slots below the end are written back.
+  constexpr int kPer = 2048 / kGenericThreads;
+  const int total = nl * N;
   const bool strided = inner > 1;
-  auto at = [&](int e, int& l, int& k) {
+  float2* w = static_cast<float2*>(p.w) + soff;
+  const float2* in = (M == Mode::kFwd || M == Mode::kInv) && p.src ? static_cast<const float2*>(p.src) + soff : w;
+  float2* x = static_cast<float2*>(p.x) + soff;
+  const std::uint8_t* sup = p.support + soff;
+  const long long arow =
+      M == Mode::kModulus ? p.amp_stride * ((p.err_offset + start) / (p.amp_group > 1 ? p.amp_group : 1)) : 0;
+  long long gi[kPer];  // element index within the start
+  int li[kPer];        // LDS index l * N + k
+  float2 v0[kPer];
+  float amp[kPer];
+  float2 xo[kPer];
+  bool inside[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int e0 = threadIdx.x + u * kGenericThreads;
+    const int e = e0 < total ? e0 : total - 1;
+    int l, k;
     if (strided) {
       k = small_div(e, nl);
       l = e - k * nl;
@@ -272,13 +292,24 @@ __global__ void __launch_bounds__(kGenericThreads)
       l = small_div(e, N);
       k = e - l * N;
     }
-  };
-  float2* w = static_cast<float2*>(p.w) + soff;
-  const float2* in = (M == Mode::kFwd || M == Mode::kInv) && p.src ? static_cast<const float2*>(p.src) + soff : w;
-  for (int e = threadIdx.x; e < nl * N; e += blockDim.x) {
-    int l, k;
-    at(e, l, k);
-    bufa[l * N + k] = in[lbase[l] + static_cast<long long>(k) * inner];
+    li[u] = l * N + k;
+    gi[u] = lbase[l] + static_cast<long long>(k) * inner;
+    v0[u] = in[gi[u]];
+    if constexpr (M == Mode::kModulus) amp[u] = p.amp[arow + gi[u]];
+    if constexpr (M == Mode::kUpdate) {
+      xo[u] = x[gi[u]];
+      inside[u] = sup[gi[u]] != 0;
+    }
+  }
+  // the twiddle table while the loads are in flight
+  for (int j = threadIdx.x; j < N; j += blockDim.x) {
+    double sn, cs;
+    sincospi(2.0 * static_cast<double>(j) / static_cast<double>(N), &sn, &cs);
+    tw[j] = {static_cast<float>(cs), static_cast<float>(-sn)};  // forward W_N^j
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    if (static_cast<int>(threadIdx.x) + u * kGenericThreads < total) bufa[li[u]] = v0[u];
   }
   __syncthreads();
   float2* src = bufa;
@@ -290,14 +321,13 @@ __global__ void __launch_bounds__(kGenericThreads)
     generic_transform<true>(src, dst, tw, plan, nl);
   } else if constexpr (M == Mode::kModulus) {
     generic_transform<false>(src, dst, tw, plan, nl);
-    const long long arow = p.amp_stride * ((p.err_offset + start) / (p.amp_group > 1 ? p.amp_group : 1));
     const float io = p.in_scale * p.out_scale;
     double acc = 0.0;
-    for (int e = threadIdx.x; e < nl * N; e += blockDim.x) {
-      int l, k;
-      at(e, l, k);
-      const float a = p.amp[arow + lbase[l] + static_cast<long long>(k) * inner];
-      src[l * N + k] = project_scaled(src[l * N + k], a, p.in_scale, io, acc);
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      if (static_cast<int>(threadIdx.x) + u * kGenericThreads < total) {
+        src[li[u]] = project_scaled(src[li[u]], amp[u], p.in_scale, io, acc);
+      }
     }
     const double ws = wave_sum(acc);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ws;
@@ -312,21 +342,16 @@ __global__ void __launch_bounds__(kGenericThreads)
     out_scale = 1.0f;  // folded into the projection
   } else {  // kUpdate
     generic_transform<true>(src, dst, tw, plan, nl);
-    float2* x = static_cast<float2*>(p.x) + soff;
-    const std::uint8_t* sup = p.support + soff;
-    for (int e = threadIdx.x; e < nl * N; e += blockDim.x) {
-      int l, k;
-      at(e, l, k);
-      const long long i = lbase[l] + static_cast<long long>(k) * inner;
-      const float2 v = src[l * N + k];
-      const float2 pm = {v.x * p.in_scale, v.y * p.in_scale};
-      float2 nx = pm;
-      if (!sup[i]) {
-        const float2 xo = x[i];
-        nx = {p.a * xo.x + p.b * pm.x, p.a * xo.y + p.b * pm.y};
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      if (static_cast<int>(threadIdx.x) + u * kGenericThreads < total) {
+        const float2 v = src[li[u]];
+        const float2 pm = {v.x * p.in_scale, v.y * p.in_scale};
+        float2 nx = pm;
+        if (!inside[u]) nx = {p.a * xo[u].x + p.b * pm.x, p.a * xo[u].y + p.b * pm.y};
+        x[gi[u]] = nx;
+        src[li[u]] = nx;
       }
-      x[i] = nx;
-      src[l * N + k] = nx;
     }
     __syncthreads();
     generic_transform<false>(src, dst, tw, plan, nl);
@@ -341,11 +366,12 @@ __global__ void __launch_bounds__(kGenericThreads)
       }
     }
   }
-  for (int e = threadIdx.x; e < nl * N; e += blockDim.x) {
-    int l, k;
-    at(e, l, k);
-    const float2 v = src[l * N + k];
-    w[lbase[l] + static_cast<long long>(k) * inner] = {v.x * out_scale, v.y * out_scale};
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    if (static_cast<int>(threadIdx.x) + u * kGenericThreads < total) {
+      const float2 v = src[li[u]];
+      w[gi[u]] = {v.x * out_scale, v.y * out_scale};
+    }
   }
 }
 

@@ -52,7 +52,9 @@ extern "C" {
 
 /* execution schedules, the reference's Variant ladder (include/qoraphasecu/solver.hpp:21-29) */
 #define QPCH_UNFUSED 0 /* one kernel per operation: transforms per axis, modulus, update */
-#define QPCH_FUSED 1   /* two fused passes per iteration (three or four in 3D), eager launches */
+#define QPCH_FUSED 1   /* two fused passes per iteration (three or four in 3D), eager launches;
+                          axes that are not powers of two (fp32, up to 2048) run the same
+                          passes in a generic mixed-radix kernel */
 #define QPCH_GRAPH 2   /* fused passes replayed from hipGraphs */
 
 typedef struct qpch_solver qpch_solver;

@@ -559,7 +559,7 @@ struct SolverImpl {
   bool mixed = false;
   int schedule = QPCH_GRAPH;
   bool fused = true;  // fused passes (generic ones on axes that are not powers of two)
-  bool pow2 = true;   // every axis a power of two <= 4096: FFT blur, stacked problems, fp16 / bf16
+  bool pow2 = true;   // every axis a power of two <= 4096: the FFT blur, fp16 / bf16 storage
   hipStream_t stream = nullptr;
   hipStream_t on = nullptr;   // where run_pass and chunk launch: the solver's stream or a lane
   int lanes = 1;              // launch groups in flight at once (each on its own stream)
@@ -1090,7 +1090,7 @@ struct SolverImpl {
       throw InvalidArgument("fp16/bf16 storage needs power-of-two extents between 2 and 4096");
     }
     if (storage != QPCH_F32 && schedule == QPCH_UNFUSED) throw InvalidArgument("fp16/bf16 storage needs a fused schedule");
-    if (problems > 1 && !pow2) throw InvalidArgument("several problems in one solver need power-of-two extents");
+    if (problems > 1 && !fused) throw InvalidArgument("several problems in one solver need the fused schedule");
     for (int a = 0; a < dims.nd; ++a) {
       if (!(is_pow2(dims.d[a]) && dims.d[a] <= kMaxFused) && dims.d[a] > dev::kGenericMaxN) {
         throw InvalidArgument("axis length " + std::to_string(dims.d[a]) + " is not supported");

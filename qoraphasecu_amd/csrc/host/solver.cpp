@@ -144,7 +144,11 @@ std::unique_ptr<MultiSolver> make_multi_solver(std::span<const Problem> problems
     if (p.amplitude.size() != n || p.mask.size() != n || p.support.size() != n) {
       throw std::invalid_argument("problem arrays do not match its shape; build it with make_problem");
     }
-    for (std::size_t a : p.shape) ensemble = ensemble && a >= 2 && (a & (a - 1)) == 0 && a <= 4096;
+    // shapes the fused passes cover: power-of-two axes up to 4096, other axes up to 2048
+    for (std::size_t a : p.shape) {
+      const bool pow2 = a >= 2 && (a & (a - 1)) == 0 && a <= 4096;
+      ensemble = ensemble && (pow2 || (a >= 2 && a <= 2048));
+    }
   }
   if (!ensemble) {
     if (!options.start.empty()) throw std::invalid_argument("start is not supported for several problems here");

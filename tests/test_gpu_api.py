@@ -186,6 +186,25 @@ def test_ensemble_equals_separate_runs(qpc, variant):
             assert np.array_equal(a.support, b.support)
 
 
+def test_ensemble_of_other_sizes_equals_separate_runs(qpc):
+    """Shapes whose axes are not all powers of two (48 x 40 twice, 64 x 48, 12 x 20 x 18)
+    beside a power-of-two one: stacked per shape and advanced together by the generic
+    passes, they give the same runs as each problem alone (fp32)."""
+    cases = [synthetic.triangle((48, 40), 12), synthetic.triangle((48, 40), 10), synthetic.triangle((64, 48), 12),
+             synthetic.triangle((12, 20, 18), 4), synthetic.triangle((64, 64), 16)]
+    recipe = qpc.Recipe(qpc.HIO(10), qpc.Shrinkwrap(), qpc.RAAR(10), qpc.detwin(qpc.HIO(5)), qpc.ER(10))
+    many = qpc.reconstruct_many([c[2] for c in cases], supports=[c[1].astype(bool) for c in cases], recipe=recipe,
+                                starts=2, seed=7, backend="hip", variant="batched")
+    assert len(many) == len(cases)
+    for (obj, sup, inten), ens in zip(cases, many):
+        single = qpc.reconstruct(inten, support=sup.astype(bool), recipe=recipe, starts=2, seed=7, backend="hip",
+                                 variant="batched")
+        for a, b in zip(ens, single):
+            np.testing.assert_allclose(a.errors, b.errors, rtol=1e-6)
+            assert rel(a.object, b.object) < 1e-6
+            assert np.array_equal(a.support, b.support)
+
+
 def test_launch_configuration_does_not_change_results(qpc, qpch):
     """Starts per launch, XCD-affine vs plain numbering and launch groups run side by side
     on their own streams only change which work-group does what: the iterates, errors and

@@ -165,6 +165,31 @@ int main() {
     });
   }
 
+  // axes that are not powers of two run the generic fused passes: a run still continues
+  // from the current state bit for bit, and ER still descends once the iterate is in the
+  // support set (from the second ER iteration on)
+  const Triangle t24 = triangle(24, 8);
+  const Problem q24 = make_problem(t24.intensity, t24.shape, t24.support);
+  for (Variant v : {Variant::graph, Variant::batched}) {
+    const std::string name = "other_sizes_continue_and_descend[" + std::string(to_string(v)) + "]";
+    run_case(name.c_str(), [&] {
+      const Options o = hip_options(v, 2);
+      auto solver = make_solver(q24, o);
+      solver->run(recipe(HIO{10}));
+      solver->run(recipe(ER{30}));
+      const auto split = solver->results();
+      const auto whole = reconstruct(q24, recipe(HIO{10}, ER{30}), o);
+      for (std::size_t k = 0; k < 2; ++k) {
+        CHECK(split[k].errors == whole[k].errors);
+        CHECK(split[k].object == whole[k].object);
+        CHECK(whole[k].errors.size() == 40u);
+        for (std::size_t t = 12; t < whole[k].errors.size(); ++t) {
+          CHECK(whole[k].errors[t] <= whole[k].errors[t - 1] * (1.0f + 1e-4f));
+        }
+      }
+    });
+  }
+
   // tests/test_recover.cpp:53-62: ER never raises the error (from the support-cut random
   // start every iterate lies in the support set, where ER is a descent)
   run_case("error_reduction_never_raises_the_error", [&] {

@@ -285,3 +285,31 @@ def test_largest_shapes(qpch, shape):
     got = R.run_abi(s, ops)[:, 0]
     s.close()
     assert np.all(np.abs(got - want) <= 3e-3 * want), (got, want)
+
+
+# Drawn once from numpy.random.default_rng(2024) over axis lengths {2, ..., 128} (small
+# primes, smooth composites and powers of two), 64 to 400 000 elements.
+RANDOM_SHAPES = [(9, 48), (12, 12), (100, 67), (100, 128), (80, 4), (100, 16), (10, 7, 24), (36, 67, 45),
+                 (128, 5, 27), (32, 49, 2)]
+
+
+@pytest.mark.parametrize("shape", RANDOM_SHAPES)
+def test_random_shapes_against_oracle(qpch, shape):
+    """Mixed axis lengths (generic passes beside power-of-two line passes, a prime axis of
+    67 as one plain DFT per line, axes of 2 and 4): one RAAR and one ER iteration from the
+    same start against the oracle within 1e-5 (plus the absolute rounding floor of
+    test_line_lengths_2_to_4096 for the small ones), on the graph and fused schedules."""
+    obj, amp, mask, sup = _problem(shape, max(1, min(shape) // 4), masked=True)
+    start = np.stack([synthetic.random_field(obj.size, 51).reshape(shape)])
+    f = oracle.to_fft_order
+    ops = [R.raar(1), R.er(1)]
+    o = oracle.Solver(amp, mask, sup, starts=1, start=start)
+    o.run(ops)
+    ores = o.results()[0]
+    for schedule in ("graph", "fused"):
+        s = qpch.Solver(f(amp), f(mask), f(sup), starts=1, start=np.stack([f(start[0])]), schedule=schedule)
+        errs = R.run_abi(s, ops)[:, 0]
+        x = s.read_state()[0]
+        s.close()
+        assert np.all(np.abs(errs - ores[2]) <= 1e-5 * ores[2] + 3e-7), (schedule, errs, ores[2])
+        assert rel(x, f(ores[0])) < 1e-5, (schedule, rel(x, f(ores[0])))

@@ -73,7 +73,9 @@ typedef struct {
   int mixed;           /* with F16/BF16: run refine (ER) iterations in F32 */
   int schedule;        /* QPCH_UNFUSED / QPCH_FUSED / QPCH_GRAPH */
   int problems;        /* independent problems of this shape stacked in one solver (0 = 1) */
-  int plane;           /* 3D: 0 = automatic, 1 = xy-plane update pass, 2 = line passes only */
+  int plane;           /* 0 = automatic, 1 = plane kernels (3D: xy-plane update pass; 2D fp32 images
+                          up to 8192 elements: whole iterations in one work-group per start),
+                          2 = line passes only */
   int xcd;             /* 0 = XCD-affine block numbering when starts per launch are a multiple of 8
                           (a start's blocks share one XCD's L2), 2 = plain numbering (a tile of
                           every start on one XCD), 3 = plain for the frequency-axis pass only

@@ -322,6 +322,33 @@ bool plane_supported(int ny, int nx) {
   return false;
 }
 
+// --- whole 2D images in one work-group (image_iterations): the plane shapes up to 8192 ----
+
+#define QPCH_FOR_EACH_IMAGE(X) \
+  X(16, 64) X(16, 128) X(32, 32) X(32, 64) X(32, 128) X(64, 16) X(64, 32) X(64, 64) X(64, 128) \
+  X(128, 16) X(128, 32) X(128, 64)
+
+bool image_supported(int ny, int nx) {
+#define QPCH_CASE(A, B) \
+  if (ny == A && nx == B) return true;
+  QPCH_FOR_EACH_IMAGE(QPCH_CASE)
+#undef QPCH_CASE
+  return false;
+}
+
+void launch_image(int ny, int nx, const PassArgs& p, int starts, int len, hipStream_t s) {
+#define QPCH_CASE(A, B)                                                                          \
+  if (ny == A && nx == B) {                                                                      \
+    hipLaunchKernelGGL((dev::image_iterations<A, B>), dim3(starts), dim3(dev::PlaneGeom<A, B>::kThreads), 0, s, p, \
+                       len);                                                                     \
+    QPCH_HIP(hipGetLastError());                                                                 \
+    return;                                                                                      \
+  }
+  QPCH_FOR_EACH_IMAGE(QPCH_CASE)
+#undef QPCH_CASE
+  throw InvalidArgument("no whole-image kernel for " + std::to_string(ny) + " x " + std::to_string(nx));
+}
+
 template <int NY, int NX, class St>
 void launch_plane_t(PassArgs a, int starts, hipStream_t s) {
   using G = dev::PlaneGeom<NY, NX>;
@@ -623,6 +650,7 @@ struct SolverImpl {
     bool plane = false;  // 3D: the update on whole xy-planes (axes 1 and 2 at once)
   };
   bool plane3d = false;  // 3D iterations use the plane pass
+  bool resident = false;  // 2D fp32 iterations run whole chunks in image_iterations
   int xcd_mode = 0;  // 0 XCD-affine by start, 2 plain numbering, 3 plain for the modulus pass only
   int radix = 16;        // FFT radix of the line passes (8 or 16)
 
@@ -699,11 +727,40 @@ struct SolverImpl {
 
   template <class St>
   void fused_prologue(int g) {
+    if (std::is_same_v<St, float2> && resident) return;  // image_iterations keeps no wave
     for (const auto& ps : prologue_passes(!std::is_same_v<St, float2>)) run_pass<St>(ps, g, batch, 0, 0);
+  }
+
+  // A chunk of fp32 iterations of every start of group g in image_iterations, then the
+  // error-slot counter advanced past it.
+  void resident_iterations(int g, int len, float a, float b) {
+    const int first = g * batch;
+    PassArgs p{};
+    p.n = n;
+    p.x = x32.get() + static_cast<long long>(first) * n;
+    p.support = supp.get() + static_cast<long long>(first) * n;
+    p.amp = amp_signed.get();
+    p.amp_stride = problems > 1 ? n : 0;
+    p.amp_group = per_problem;
+    p.errs = errs.get();
+    p.err_stride = starts;
+    p.err_offset = first;
+    p.err_cap = err_cap;
+    p.counter = counters.get() + g;
+    p.a = a;
+    p.b = b;
+    p.in_scale = p.out_scale = static_cast<float>(1.0 / std::sqrt(static_cast<double>(n)));
+    const hipStream_t s = on ? on : stream;
+    launch_image(dims.d[0], dims.d[1], p, batch, len, s);
+    hipLaunchKernelGGL(dev::advance_counter, dim3(1), dim3(1), 0, s, p.counter, len);
+    QPCH_HIP(hipGetLastError());
   }
 
   template <class St>
   void fused_iterations(int g, int len, float a, float b) {
+    if constexpr (std::is_same_v<St, float2>) {
+      if (resident) return resident_iterations(g, len, a, b);
+    }
     const auto passes = iteration_passes(!std::is_same_v<St, float2>);
     for (int it = 0; it < len; ++it) {
       for (const auto& ps : passes) run_pass<St>(ps, g, batch, a, b);
@@ -1082,6 +1139,7 @@ struct SolverImpl {
     pow2 = fusable(dims);
     fused = (pow2 || (storage == QPCH_F32 && generic_fusable(dims))) && schedule != QPCH_UNFUSED;
     plane3d = fused && dims.nd == 3 && plane_supported(dims.d[1], dims.d[2]) && p.plane != 2;
+    resident = fused && dims.nd == 2 && image_supported(dims.d[0], dims.d[1]) && p.plane != 2;
     if (p.xcd != 0 && p.xcd != 2 && p.xcd != 3) throw InvalidArgument("xcd must be 0, 2 or 3");
     xcd_mode = p.xcd;
     if (p.radix != 0 && p.radix != 8 && p.radix != 16) throw InvalidArgument("radix must be 0, 8 or 16");

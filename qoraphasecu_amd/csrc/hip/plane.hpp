@@ -151,4 +151,81 @@ __global__ void __launch_bounds__((PlaneGeom<NY, NX>::kThreads))
   }
 }
 
+
+// Whole iterations of a 2D image that one work-group holds (up to 8192 elements: the
+// iterate, support and amplitude of a start stay in registers, 16 elements each per
+// thread, which 128 x 128 would spill).  Per iteration: x and y forward transforms,
+// projection with the error numerator summed in a fixed order (waves' shuffle trees, then
+// the waves in index order), y and x inverse transforms, update; the transposes and line
+// exchanges are those of plane_update.  The iterate is read and written once per chunk of
+// `len` iterations and no pass boundary separates them; the amplitude is read once.
+// fp32 storage; in_scale = out_scale = 1/sqrt(n), as in the modulus pass, so the inverse
+// lands on P_M x.  Iteration it writes error slot *counter + it; the launcher then
+// advances the counter by len.  One work-group per start of the launch (blockIdx.x).
+template <int NY, int NX>
+__global__ void __launch_bounds__((PlaneGeom<NY, NX>::kThreads))
+    image_iterations(PassArgs p, int len) {
+  using G = PlaneGeom<NY, NX>;
+  static_assert(NY * NX <= 8192, "the image must fit the registers of one work-group");
+  __shared__ float2 lds[G::kLds];
+  __shared__ double red[G::kThreads / 64];
+  float2* twx = lds + G::kScratch;
+  float2* twy = twx + NX;
+  const int start = blockIdx.x;
+  const int t = threadIdx.x;
+  const int y = t / G::kTX, i = t % G::kTX;  // row layout
+  const int x = t % NX, j = t / NX;          // column layout
+  constexpr int kN = NY * NX;
+  const long long soff = static_cast<long long>(start) * p.n;
+  const View<float2, true> xrow(static_cast<float2*>(p.x) + soff, kN, y * NX + i, G::kTX);
+  const View<const std::uint8_t, true> srow(p.support + soff, kN, y * NX + i, G::kTX);
+  const long long arow = p.amp_stride * ((p.err_offset + start) / (p.amp_group > 1 ? p.amp_group : 1));
+  const View<const float, true> acol(p.amp + arow, kN, j * NX + x, G::kTY * NX);
+  float2 xr[16];
+  bool inside[16];
+  float amp[16];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    xr[m] = vld(xrow, m);
+    inside[m] = vld_flag(srow, m);
+    amp[m] = vld_f32(acol, m);
+  }
+  load_twiddles<NX>(twx);
+  load_twiddles<NY>(twy);
+  const int slot0 = p.counter ? *p.counter : 0;
+  __syncthreads();
+  const float io = p.in_scale * p.out_scale;
+  for (int it = 0; it < len; ++it) {
+    float2 v[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) v[m] = xr[m];
+    fft_line<NX, 16, true>(v, i, lds + y * G::SX::kLine, twx);
+    rows_to_cols<NY, NX>(v, y, i, x, j, lds);
+    fft_line<NY, 16, true>(v, j, lds + x * G::kColStride, twy);
+    double acc = 0.0;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) v[m] = project_scaled(v[m], amp[m], p.in_scale, io, acc);
+    const double ws = wave_sum(acc);
+    if ((t & 63) == 0) red[t >> 6] = ws;
+    __syncthreads();
+    if (t == 0 && p.errs) {
+      double e = 0.0;
+      for (int w = 0; w < G::kThreads / 64; ++w) e += red[w];
+      const int slot = slot0 + it;
+      if (slot >= 0 && slot < p.err_cap) {
+        p.errs[static_cast<long long>(slot) * p.err_stride + p.err_offset + start] = e;
+      }
+    }
+    fft_line<NY, 16, false>(v, j, lds + x * G::kColStride, twy);
+    cols_to_rows<NY, NX>(v, y, i, x, j, lds);  // (its barriers also order red's next writes)
+    fft_line<NX, 16, false>(v, i, lds + y * G::SX::kLine, twx);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      xr[m] = inside[m] ? v[m] : float2{p.a * xr[m].x + p.b * v[m].x, p.a * xr[m].y + p.b * v[m].y};
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 16; ++m) vst(xrow, m, xr[m]);
+}
+
 }  // namespace qpch::dev

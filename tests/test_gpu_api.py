@@ -236,6 +236,36 @@ def test_launch_configuration_does_not_change_results(qpc, qpch):
     assert np.array_equal(s2, s1)
 
 
+def test_whole_image_kernel_agrees_with_two_passes(qpc, qpch):
+    """A 64 x 64 fp32 solver runs its iterations in the whole-image kernel (plane 0 or 1) or
+    in the two-pass schedule (plane 2).  Starts per launch and launch groups only change
+    which work-group does what (bit-identical); the two schedules do the same arithmetic in
+    another order and agree to FP32 rounding (1e-5 after ten iterations, supports equal)."""
+    import recipes as R
+
+    obj, sup, inten = synthetic.triangle((64, 64), 16)
+    amp, mask, sup = oracle.make_problem(inten, sup)
+    f = oracle.to_fft_order
+    ops = R.flat(R.raar(6), R.shrinkwrap(method="rms"), R.er(4))
+    runs = {}
+    for plane, batch, lanes in [(0, 0, 1), (0, 4, 1), (0, 2, 4), (1, 0, 1), (2, 0, 1), (2, 4, 2)]:
+        s = qpch.Solver(f(amp), f(mask), f(sup), starts=8, seed=2, plane=plane, batch=batch, lanes=lanes)
+        errs = R.run_abi(s, ops, sync_every=4)
+        runs.setdefault(plane == 2, []).append(((plane, batch, lanes), errs, s.read_state(), s.get_support()))
+        s.close()
+    for two_pass, group in runs.items():
+        _, e0, x0, s0 = group[0]
+        for cfg, errs, x, su in group[1:]:
+            assert np.array_equal(errs, e0), cfg
+            assert np.array_equal(x, x0), cfg
+            assert np.array_equal(su, s0), cfg
+    _, e1, x1, s1 = runs[False][0]
+    _, e2, x2, s2 = runs[True][0]
+    np.testing.assert_allclose(e2, e1, rtol=1e-5)
+    assert rel(x2, x1) < 1e-5
+    assert np.array_equal(s2, s1)
+
+
 def test_autotuner_reports_its_choice(qpc, qpch):
     obj, sup, inten = synthetic.triangle((64, 64), 16)
     qpc.reconstruct(inten, support=sup.astype(bool), recipe=qpc.Recipe(qpc.ER(2)), starts=16, backend="hip",

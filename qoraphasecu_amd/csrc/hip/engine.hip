@@ -731,8 +731,8 @@ struct SolverImpl {
     for (const auto& ps : prologue_passes(!std::is_same_v<St, float2>)) run_pass<St>(ps, g, batch, 0, 0);
   }
 
-  // A chunk of fp32 iterations of every start of group g in image_iterations, then the
-  // error-slot counter advanced past it.
+  // A chunk of fp32 iterations of every start of group g in image_iterations (which also
+  // advances the group's error-slot counter past the chunk).
   void resident_iterations(int g, int len, float a, float b) {
     const int first = g * batch;
     PassArgs p{};
@@ -747,13 +747,11 @@ struct SolverImpl {
     p.err_offset = first;
     p.err_cap = err_cap;
     p.counter = counters.get() + g;
+    p.done = counters.get() + groups + g;  // tickets: the second half of counters
     p.a = a;
     p.b = b;
     p.in_scale = p.out_scale = static_cast<float>(1.0 / std::sqrt(static_cast<double>(n)));
-    const hipStream_t s = on ? on : stream;
-    launch_image(dims.d[0], dims.d[1], p, batch, len, s);
-    hipLaunchKernelGGL(dev::advance_counter, dim3(1), dim3(1), 0, s, p.counter, len);
-    QPCH_HIP(hipGetLastError());
+    launch_image(dims.d[0], dims.d[1], p, batch, len, on ? on : stream);
   }
 
   template <class St>
@@ -979,7 +977,7 @@ struct SolverImpl {
     err_cap = capacity;
     hipLaunchKernelGGL(dev::fill_double, dim3(grid_for(static_cast<long long>(errs.size()))), dim3(256), 0,
                        stream, errs.get(), static_cast<long long>(errs.size()), 0.0);
-    hipLaunchKernelGGL(dev::fill_int, dim3(1), dim3(256), 0, stream, counters.get(), groups, 0);
+    hipLaunchKernelGGL(dev::fill_int, dim3(1), dim3(256), 0, stream, counters.get(), 2 * groups, 0);
     QPCH_HIP(hipGetLastError());
   }
 
@@ -1210,7 +1208,7 @@ struct SolverImpl {
       x16.resize(static_cast<size_t>(n) * starts);
       w16.resize(static_cast<size_t>(n) * starts);
     }
-    counters.resize(groups);
+    counters.resize(2 * static_cast<size_t>(groups));  // slot counters, then image_iterations' tickets
     // partial sums of the error: one per block of the modulus pass of each start
     int pb = grid_for(n, 256, 1024);
     if (fused) pb = std::max({pb, axis0_partials<8>(dims, radix), axis0_partials<4>(dims, radix)});

@@ -247,9 +247,4 @@ __global__ void __launch_bounds__(256)
 }
 
 
-// The error-slot counter advanced past a chunk written by image_iterations.
-__global__ void advance_counter(int* counter, int n) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *counter += n;
-}
-
 }  // namespace qpch::dev

@@ -53,6 +53,7 @@ struct PassArgs {
   int err_offset;               // global index of this launch's first start
   int err_cap;                  // slots available
   int* counter;                 // error slot counter (null: no history, as in the loss)
+  int* done;                    // image_iterations: work-groups finished in this launch (0 between launches)
   int starts;                   // starts in this launch
   int slots;                    // > 0: XCD-affine grid, block = tile * slots + start
   int no_xcd;                   // launcher input: 1 keeps the plain block numbering

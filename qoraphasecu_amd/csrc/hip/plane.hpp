@@ -160,8 +160,9 @@ __global__ void __launch_bounds__((PlaneGeom<NY, NX>::kThreads))
 // exchanges are those of plane_update.  The iterate is read and written once per chunk of
 // `len` iterations and no pass boundary separates them; the amplitude is read once.
 // fp32 storage; in_scale = out_scale = 1/sqrt(n), as in the modulus pass, so the inverse
-// lands on P_M x.  Iteration it writes error slot *counter + it; the launcher then
-// advances the counter by len.  One work-group per start of the launch (blockIdx.x).
+// lands on P_M x.  Iteration it writes error slot *counter + it, and the last work-group
+// of the launch to finish advances the counter by len.  One work-group per start of the
+// launch (blockIdx.x).
 template <int NY, int NX>
 __global__ void __launch_bounds__((PlaneGeom<NY, NX>::kThreads))
     image_iterations(PassArgs p, int len) {
@@ -226,6 +227,15 @@ __global__ void __launch_bounds__((PlaneGeom<NY, NX>::kThreads))
   }
 #pragma unroll
   for (int m = 0; m < 16; ++m) vst(xrow, m, xr[m]);
+  // Every work-group read the counter at its start and used the slot (thread 0 wrote the
+  // errors with it) before thread 0 takes a ticket here, so the last ticket's holder can
+  // advance the counter without any work-group of this launch reading the new value.
+  if (p.counter && t == 0) {
+    if (atomicAdd(p.done, 1) == static_cast<int>(gridDim.x) - 1) {
+      *p.done = 0;
+      *p.counter = slot0 + len;
+    }
+  }
 }
 
 }  // namespace qpch::dev

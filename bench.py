@@ -10,7 +10,10 @@ bench/bench.cpp:33-45 protocol: construct, warm up with a run of the same length
 hipGraphs exist, then time).  The headline value is the 256^2 workload (32 starts,
 repeat(10, RAAR(50, beta 0.9), Shrinkwrap) then ER(200), Poisson data with a beamstop)
 on the batched fp16-storage path; the 128^3 workload (8 starts, repeat(8, RAAR(50),
-Shrinkwrap rms) then ER(200)) is reported under "details".  Prints one JSON line.
+Shrinkwrap rms) then ER(200)) is reported beside it under "c4".  Prints one compact JSON
+line on stdout (the headline, the roofline of the dominant kernel and the CPU baseline);
+the full per-workload record (every run, pass timing, per-start errors and the autotune
+log) goes to the file named by --details, and a readable summary goes to stderr.
 """
 
 from __future__ import annotations
@@ -382,6 +385,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-passes", action="store_true", help="skip the per-pass timing")
     ap.add_argument("--no-floor", action="store_true", help="skip the rocFFT transform floor")
+    ap.add_argument("--details", default=None, metavar="PATH",
+                    help="write the full per-workload record as JSON to PATH")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB 10 times (the reference point of the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -519,20 +524,29 @@ def main():
     from qoraphasecu_amd import _qpch
 
     details["autotune_log"] = _qpch.autotune_log().splitlines()
+    if args.details:
+        os.makedirs(os.path.dirname(os.path.abspath(args.details)), exist_ok=True)
+        with open(args.details, "w") as fh:
+            json.dump(details, fh, default=float)
     if headline is None:
-        print(json.dumps({"details": details}))
+        print(json.dumps({"metric": "RAAR iters/s (all starts) at 256² & 128³, batched/fp16 vs plain fp32; %HBM",
+                          "value": None, "note": "the 256^2 workload (c2) was not run"}))
         return
     best = "mixed" if "mixed" in headline else [k for k in ("mixed_bf16", "batched", "graph") if k in headline][0]
     h = headline[best]
     cpu = headline.get("cpu_oracle")
+
+    def rnd(v, k=4):
+        return None if v is None else float(f"{v:.{k}g}")
+
     out = {
         "metric": "RAAR iters/s (all starts) at 256² & 128³, batched/fp16 vs plain fp32; %HBM",
-        "value": h["start_iterations_per_s"],
+        "value": rnd(h["start_iterations_per_s"], 6),
         "unit": "start-iterations/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": h["ms_per_run"],
+        "ms_per_step": rnd(h["ms_per_run"], 6),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -540,25 +554,48 @@ def main():
         "storage": "f16 (ER in f32)" if best == "mixed" else best,
         "data": "synthetic (seeded ellipse objects, Poisson counts)",
         "config": {
-            "workload": headline["workload"] + f"; variant {best}, all starts in one launch",
+            "workload": "256x256, 32 starts, repeat(10, RAAR(50, 0.9), Shrinkwrap) + ER(200), Poisson 1e7 photons, "
+                        f"6 px beamstop; {best}, all starts in one launch",
             "starts": headline["starts"],
             "iterations": headline["iterations"],
-            "shape": [256, 256],
         },
-        "speedup_vs_plain_fp32": headline.get(f"speedup_{best}_vs_plain"),
-        "model_hbm_frac": h.get("model_hbm_frac"),
-        "roofline": roof,
+        "plain_fp32": rnd(headline["graph"]["start_iterations_per_s"]) if "graph" in headline else None,
+        "speedup_vs_plain_fp32": rnd(headline.get(f"speedup_{best}_vs_plain"), 3),
+        "model_hbm_frac": rnd(h.get("model_hbm_frac"), 3),
+        "roofline": None if roof is None else {
+            "bound": roof["bound"],
+            "achieved": rnd(roof["achieved"]),
+            "peak": roof["peak"],
+            "unit": roof["unit"],
+            "frac": rnd(roof["frac"], 3),
+            "traffic": roof["traffic"],
+            "kernel": roof["kernel"],
+            "bytes_per_launch": roof["algorithmic_bytes_per_launch"],
+            "avg_launch_us": rnd(roof["avg_launch_ms"] * 1e3),
+        },
         "cpu_baseline": None if cpu is None else {
-            "value": cpu["value"],
+            "value": rnd(cpu["value"]),
             "unit": "start-iterations/s",
             "cores": cpu["threads"],
             "kind": "port",
             "sample": f"oracle (C, float, OpenMP) RAAR x{cpu['iterations']} on all {cpu['starts']} starts "
                       f"of the 256^2 workload ({cpu['seconds']:.1f} s)",
         },
-        "details": details,
     }
-    print(json.dumps(out, default=float))
+    c4 = details.get("c4")
+    if c4 is not None:
+        best4 = "mixed" if "mixed" in c4 else None
+        out["c4"] = {
+            "workload": "128^3, 8 starts, repeat(8, RAAR(50), Shrinkwrap rms) + ER(200)",
+            "mixed": rnd(c4["mixed"]["start_iterations_per_s"]) if best4 else None,
+            "plain_fp32": rnd(c4["graph"]["start_iterations_per_s"]) if "graph" in c4 else None,
+            "speedup": rnd(c4.get("speedup_mixed_vs_plain"), 3),
+            "model_hbm_frac": rnd(c4["mixed"].get("model_hbm_frac"), 3) if best4 else None,
+        }
+    line = json.dumps(out, ensure_ascii=False)
+    if len(line) > 2048:
+        raise RuntimeError(f"bench line is {len(line)} bytes; keep it within 2 KB")
+    print(line, flush=True)
 
 
 if __name__ == "__main__":

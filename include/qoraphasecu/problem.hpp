@@ -32,6 +32,20 @@ Problem make_problem(std::span<const float> intensity, Shape shape,
                      std::span<const std::uint8_t> support = {},
                      std::span<const std::uint8_t> mask = {}, float support_threshold = 0.1f);
 
+// Inputs of one problem for make_problems (spans that must outlive the call).
+struct ProblemInput {
+  std::span<const float> intensity;
+  Shape shape;
+  std::span<const std::uint8_t> support = {};  // empty: the autocorrelation support
+  std::span<const std::uint8_t> mask = {};     // empty: every pixel usable
+};
+
+// make_problem for several problems at once (no reference counterpart: the entry point of
+// the multi-problem reconstruct).  Each problem is validated and built as make_problem
+// builds it; the autocorrelation supports of all problems of one shape come from one
+// device pass.
+std::vector<Problem> make_problems(std::span<const ProblemInput> inputs, float support_threshold = 0.1f);
+
 // |unitary IFFT(intensity)| above `threshold` times its maximum (computed on the GPU).
 std::vector<std::uint8_t> autocorrelation_support(std::span<const float> intensity,
                                                   const Shape& shape, float threshold = 0.1f);

@@ -202,6 +202,19 @@ int qpch_support_project(float* host_x, const uint8_t* support, size_t n);
 int qpch_gaussian_blur(const float* host_in, float* host_out, int ndim, const size_t* shape,
                        float sigma);
 
+/* make_problem's autocorrelation support for `count` problems of one shape at once
+ * (src/core/problem.cpp:72-88, which does one problem on the host around one pocketfft
+ * call): each centre-origin intensity (count * n floats, back to back) is moved to FFT
+ * order, inverse transformed (unitary), |.| taken, moved back to centre-origin and cut at
+ * threshold * its maximum (strict >).  support: count * n bytes, centre-origin, 0 / 1.
+ * The whole chain runs on the device; one copy in, one copy out. */
+int qpch_autocorrelation_support(const float* intensity, int count, int ndim, const size_t* shape,
+                                 float threshold, uint8_t* support);
+/* simulate_intensity for `count` objects of one shape (src/core/problem.cpp:90-99):
+ * |centred unitary F(object)|^2.  object: count * n complex (interleaved floats),
+ * centre-origin; intensity: count * n floats, centre-origin. */
+int qpch_simulate_intensity(const float* object, int count, int ndim, const size_t* shape, float* intensity);
+
 /* --- device-pointer entry points (caller's stream) --------------------------------------- */
 
 /* Amplitude loss and its gradient for `batch` objects x (device pointers, complex):

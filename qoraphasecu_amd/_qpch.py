@@ -116,6 +116,8 @@ def lib():
         L.qpch_support_project.argtypes = [vp, vp, sz]
         L.qpch_gaussian_blur.argtypes = [vp, vp, i, szp, f]
         L.qpch_amplitude_loss.argtypes = [vp, vp, ctypes.c_longlong, vp, vp, i, szp, i, i, vp]
+        L.qpch_autocorrelation_support.argtypes = [vp, i, i, szp, f, vp]
+        L.qpch_simulate_intensity.argtypes = [vp, i, i, szp, vp]
         _lib = L
     return _lib
 
@@ -194,6 +196,28 @@ def gaussian_blur(image, sigma):
     out = np.empty_like(im)
     nd, sh = _shape(im.shape)
     check(lib().qpch_gaussian_blur(_ptr(im), _ptr(out), nd, sh, float(sigma)))
+    return out
+
+
+def autocorrelation_support(intensity, threshold=0.1, count=None):
+    """Autocorrelation supports on the device of centre-origin intensities: one array, or
+    `count` of one shape stacked along a leading axis (one device pass for all)."""
+    i = np.ascontiguousarray(intensity, dtype=np.float32)
+    shape = i.shape if count is None else i.shape[1:]
+    out = np.empty(i.shape, np.uint8)
+    nd, sh = _shape(shape)
+    check(lib().qpch_autocorrelation_support(_ptr(i), 1 if count is None else count, nd, sh, float(threshold),
+                                             _ptr(out)))
+    return out
+
+
+def simulate_intensity(obj, count=None):
+    """|centred unitary F(obj)|^2 on the device, of one object or `count` stacked ones."""
+    o = np.ascontiguousarray(obj, dtype=np.complex64)
+    shape = o.shape if count is None else o.shape[1:]
+    out = np.empty(o.shape, np.float32)
+    nd, sh = _shape(shape)
+    check(lib().qpch_simulate_intensity(_ptr(o), 1 if count is None else count, nd, sh, _ptr(out)))
     return out
 
 

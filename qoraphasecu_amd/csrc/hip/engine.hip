@@ -165,6 +165,10 @@ Dims make_dims(int ndim, const size_t* shape) {
 
 constexpr int kMaxFused = 4096;
 
+dev::Grid3 grid3(const Dims& d) {
+  return d.nd == 2 ? dev::Grid3{1, d.d[0], d.d[1]} : dev::Grid3{d.d[0], d.d[1], d.d[2]};
+}
+
 bool eager_launches() {
   const char* v = std::getenv("QPCH_EAGER");
   return v && v[0] == '1';
@@ -1652,6 +1656,57 @@ int qpch_gaussian_blur(const float* host_in, float* host_out, int ndim, const si
       std::swap(cur, other);
     }
     qpch::copy_sync(host_out, cur, sizeof(float) * d.n, hipMemcpyDeviceToHost, st.s);
+  });
+}
+
+int qpch_autocorrelation_support(const float* intensity, int count, int ndim, const size_t* shape,
+                                 float threshold, uint8_t* support) {
+  return guarded([&] {
+    if (count < 1 || count > 65535) throw qpch::InvalidArgument("count must be between 1 and 65535");
+    if (!intensity || !support) throw qpch::InvalidArgument("intensity and support are required");
+    const qpch::Dims d = qpch::make_dims(ndim, shape);
+    const qpch::dev::Grid3 g = qpch::grid3(d);
+    const long long total = d.n * count;
+    const qpch::OwnStream st;
+    qpch::DevBuf<float> buf = qpch::upload(intensity, static_cast<size_t>(total), st.s);
+    qpch::DevBuf<float2> w(static_cast<size_t>(total));
+    const int blocks = qpch::grid_for(d.n, 256, 1024);
+    qpch::DevBuf<float> block_max(static_cast<size_t>(blocks) * count);
+    qpch::DevBuf<std::uint8_t> out(static_cast<size_t>(total));
+    hipLaunchKernelGGL(qpch::dev::centred_to_wave<false>, dim3(blocks, count), dim3(256), 0, st.s, buf.get(),
+                       w.get(), d.n, g);
+    QPCH_HIP(hipGetLastError());
+    qpch::device_fft(w.get(), d, count, false, static_cast<float>(1.0 / std::sqrt(static_cast<double>(d.n))), st.s);
+    // the magnitudes overwrite the intensities (read by the first kernel only)
+    hipLaunchKernelGGL(qpch::dev::autocorrelation_magnitude, dim3(blocks, count), dim3(256), 0, st.s, w.get(),
+                       buf.get(), block_max.get(), d.n, g);
+    hipLaunchKernelGGL(qpch::dev::autocorrelation_cut, dim3(blocks, count), dim3(256), 0, st.s, buf.get(),
+                       block_max.get(), blocks, d.n, threshold, out.get());
+    QPCH_HIP(hipGetLastError());
+    qpch::copy_sync(support, out.get(), static_cast<size_t>(total), hipMemcpyDeviceToHost, st.s);
+  });
+}
+
+int qpch_simulate_intensity(const float* object, int count, int ndim, const size_t* shape, float* intensity) {
+  return guarded([&] {
+    if (count < 1 || count > 65535) throw qpch::InvalidArgument("count must be between 1 and 65535");
+    if (!object || !intensity) throw qpch::InvalidArgument("object and intensity are required");
+    const qpch::Dims d = qpch::make_dims(ndim, shape);
+    const qpch::dev::Grid3 g = qpch::grid3(d);
+    const long long total = d.n * count;
+    const qpch::OwnStream st;
+    qpch::DevBuf<float> obj = qpch::upload(object, static_cast<size_t>(2 * total), st.s);
+    qpch::DevBuf<float2> w(static_cast<size_t>(total));
+    qpch::DevBuf<float> out(static_cast<size_t>(total));
+    const int blocks = qpch::grid_for(d.n, 256, 1024);
+    hipLaunchKernelGGL(qpch::dev::centred_to_wave<true>, dim3(blocks, count), dim3(256), 0, st.s, obj.get(), w.get(),
+                       d.n, g);
+    QPCH_HIP(hipGetLastError());
+    qpch::device_fft(w.get(), d, count, true, static_cast<float>(1.0 / std::sqrt(static_cast<double>(d.n))), st.s);
+    hipLaunchKernelGGL(qpch::dev::wave_to_intensity, dim3(blocks, count), dim3(256), 0, st.s, w.get(), out.get(), d.n,
+                       g);
+    QPCH_HIP(hipGetLastError());
+    qpch::copy_sync(intensity, out.get(), sizeof(float) * static_cast<size_t>(total), hipMemcpyDeviceToHost, st.s);
   });
 }
 

@@ -247,4 +247,114 @@ __global__ void __launch_bounds__(256)
 }
 
 
+// --- make_problem on the device (src/core/problem.cpp:72-99) ----------------------------
+// Arrays of `count` problems of one shape, back to back (blockIdx.y = problem).  Shape
+// (d0, d1, d2) with d0 = 1 in 2D.  The centre-origin index c of an axis of length N sits at
+// FFT-order index (c + N - N / 2) % N (numpy's ifftshift), and FFT-order index p at
+// centre-origin index (p + N / 2) % N (fftshift).
+
+struct Grid3 {
+  int d0, d1, d2;
+};
+
+__device__ __forceinline__ long long centred_to_fft(long long c, Grid3 g) {
+  const int x = static_cast<int>(c % g.d2);
+  const int y = static_cast<int>((c / g.d2) % g.d1);
+  const int z = static_cast<int>(c / (static_cast<long long>(g.d2) * g.d1));
+  const int fx = (x + g.d2 - g.d2 / 2) % g.d2;
+  const int fy = (y + g.d1 - g.d1 / 2) % g.d1;
+  const int fz = (z + g.d0 - g.d0 / 2) % g.d0;
+  return (static_cast<long long>(fz) * g.d1 + fy) * g.d2 + fx;
+}
+
+__device__ __forceinline__ long long fft_to_centred(long long p, Grid3 g) {
+  const int x = static_cast<int>(p % g.d2);
+  const int y = static_cast<int>((p / g.d2) % g.d1);
+  const int z = static_cast<int>(p / (static_cast<long long>(g.d2) * g.d1));
+  const int cx = (x + g.d2 / 2) % g.d2;
+  const int cy = (y + g.d1 / 2) % g.d1;
+  const int cz = (z + g.d0 / 2) % g.d0;
+  return (static_cast<long long>(cz) * g.d1 + cy) * g.d2 + cx;
+}
+
+// w[p] = in[c]: a centre-origin real (kComplex = false) or complex array moved to FFT order
+// as complex values.
+template <bool kComplex>
+__global__ void __launch_bounds__(256)
+    centred_to_wave(const float* in, float2* w, long long n, Grid3 g) {
+  const long long soff = static_cast<long long>(blockIdx.y) * n;
+  for (long long c = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; c < n;
+       c += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const long long p = centred_to_fft(c, g);
+    if constexpr (kComplex) {
+      const float2 v = reinterpret_cast<const float2*>(in)[soff + c];
+      w[soff + p] = v;
+    } else {
+      w[soff + p] = {in[soff + c], 0.0f};
+    }
+  }
+}
+
+// |w| of the autocorrelation (std::abs, i.e. hypot: the root of the sum of squares taken
+// in double and rounded once to float), stored at centre-origin positions, and the
+// largest value of each block (a maximum is exact in any order).
+__global__ void __launch_bounds__(256)
+    autocorrelation_magnitude(const float2* w, float* mag, float* block_max, long long n, Grid3 g) {
+  __shared__ float red[256];
+  const long long soff = static_cast<long long>(blockIdx.y) * n;
+  float best = 0.0f;
+  for (long long p = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; p < n;
+       p += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const float2 v = w[soff + p];
+    const double re = v.x, im = v.y;
+    const float m = static_cast<float>(sqrt(re * re + im * im));
+    mag[soff + fft_to_centred(p, g)] = m;
+    best = fmaxf(best, m);
+  }
+  red[threadIdx.x] = best;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (static_cast<int>(threadIdx.x) < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) block_max[static_cast<long long>(blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+}
+
+// support = mag > threshold * max (strict; the level is the float product, as the
+// reference's `threshold * max`).  Every block first takes the maximum of the `blocks`
+// block maxima of its problem.
+__global__ void __launch_bounds__(256)
+    autocorrelation_cut(const float* mag, const float* block_max, int blocks, long long n, float threshold,
+                        std::uint8_t* support) {
+  __shared__ float red[256];
+  const long long soff = static_cast<long long>(blockIdx.y) * n;
+  float best = 0.0f;
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
+    best = fmaxf(best, block_max[static_cast<long long>(blockIdx.y) * blocks + b]);
+  }
+  red[threadIdx.x] = best;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (static_cast<int>(threadIdx.x) < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  const float level = __fmul_rn(threshold, red[0]);
+  for (long long c = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; c < n;
+       c += static_cast<long long>(gridDim.x) * blockDim.x) {
+    support[soff + c] = mag[soff + c] > level ? 1 : 0;
+  }
+}
+
+// intensity[c] = |w[p]|^2 = re * re + im * im in float, each product rounded (std::norm),
+// at centre-origin positions.
+__global__ void __launch_bounds__(256)
+    wave_to_intensity(const float2* w, float* intensity, long long n, Grid3 g) {
+  const long long soff = static_cast<long long>(blockIdx.y) * n;
+  for (long long p = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; p < n;
+       p += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const float2 v = w[soff + p];
+    intensity[soff + fft_to_centred(p, g)] = __fadd_rn(__fmul_rn(v.x, v.x), __fmul_rn(v.y, v.y));
+  }
+}
+
 }  // namespace qpch::dev

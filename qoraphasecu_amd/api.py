@@ -1,11 +1,14 @@
-"""``reconstruct``: array in, reconstruction out (reference: python/qoraphasecu/api.py:1-185).
+"""Array in, reconstruction out: `reconstruct` and its result types, with the call
+signature and result shape of the reference's python/qoraphasecu/api.py, plus
+`reconstruct_many` for problems of several sizes at once.
 
-The arrays go to the C++ engine (``_core``), whose ``Backend::hip`` runs every iteration
-on the GPU.  ``backend="auto"`` picks ``"hip"`` when a device is present.
+Every call goes to the C++ engine (`_core`); its Backend::hip runs the iterations on the
+GPU.  backend="auto" means "hip" when a device is visible.
 """
 
 from __future__ import annotations
 
+import math
 import warnings
 from dataclasses import dataclass
 from typing import Literal
@@ -13,14 +16,17 @@ from typing import Literal
 import numpy as np
 
 from . import _core, interop
-from .recipe import default_recipe
+from .recipe import Recipe, default_recipe
 
+# Above this fraction of negative intensity pixels the caller is warned that they will be
+# clipped to zero (the reference's behaviour and threshold).
 NEGATIVE_WARN_FRACTION = 0.01
 
 
 @dataclass(frozen=True)
 class Reconstruction:
-    """One reconstructed object, defined up to a global phase, a translation and a twin."""
+    """One run's object (centre-origin, known up to a global phase, a shift and the
+    twin), its final support and its Fourier error per iteration."""
 
     object: np.ndarray
     support: np.ndarray
@@ -36,17 +42,17 @@ class Reconstruction:
 
     @property
     def error(self) -> float:
-        """Final Fourier error; the ranking key."""
-        return float(self.errors[-1]) if self.errors.size else float("nan")
+        """The last Fourier error (NaN for an empty history): what runs are ranked by."""
+        return float(self.errors[-1]) if len(self.errors) else math.nan
 
     def to_qoraphase(self):
-        """The same run as a ``qoraphase.Reconstruction``; ``errors`` becomes ``metrics``."""
+        """This run as a qoraphase.Reconstruction (the error history becomes its metrics)."""
         return interop.reconstruction_to_qoraphase(self)
 
 
 @dataclass(frozen=True)
 class Ensemble:
-    """The runs of a multi-start reconstruction, not mutually aligned."""
+    """All runs of a multi-start reconstruction, each in its own frame."""
 
     runs: tuple
 
@@ -61,16 +67,21 @@ class Ensemble:
 
     @property
     def ranking(self) -> tuple:
-        """Run indices, best (lowest error) first."""
-        return tuple(sorted(range(len(self.runs)), key=lambda i: self.runs[i].error))
+        """Run indices by increasing final error; runs whose error is NaN come last (the
+        rule of the C++ best_index)."""
+        def key(i):
+            e = self.runs[i].error
+            return (math.isnan(e), e if not math.isnan(e) else 0.0, i)
+
+        return tuple(sorted(range(len(self.runs)), key=key))
 
     @property
     def best(self) -> Reconstruction:
         return self.runs[self.ranking[0]]
 
     def to_qoraphase(self, problem=None):
-        """A ``qoraphase.Ensemble`` of the runs, for its ``align`` and ``average``; pass the
-        measurement's ``qoraphase.CDI`` as *problem* to have ``average`` scored against it."""
+        """The runs as a qoraphase.Ensemble, for its align and average; give the
+        measurement's qoraphase.CDI as *problem* to have the average scored against it."""
         return interop.ensemble_to_qoraphase(self.runs, problem)
 
 
@@ -87,18 +98,19 @@ def hip_device_info() -> dict:
 
 
 def autocorrelation_support(intensity, threshold: float = 0.1) -> np.ndarray:
-    """Loose support from the thresholded autocorrelation; Shrinkwrap tightens it."""
-    intensity = np.ascontiguousarray(intensity, dtype=np.float32)
-    return _core.autocorrelation_support(intensity, threshold).astype(bool)
+    """The loose support a reconstruction starts from: where the autocorrelation's
+    magnitude exceeds threshold times its peak."""
+    return _core.autocorrelation_support(np.ascontiguousarray(intensity, dtype=np.float32), threshold).astype(bool)
 
 
 def simulate_intensity(obj) -> np.ndarray:
-    """|centred unitary FFT(obj)|^2."""
+    """The noiseless intensity |centred unitary FFT(obj)|^2."""
     return _core.simulate_intensity(np.ascontiguousarray(obj, dtype=np.complex64))
 
 
 def aligned_error(a, reference) -> float:
-    """``||a - reference|| / ||reference||`` after removing shift, global phase and twin."""
+    """||a - reference|| / ||reference|| once the shift, the global phase and the twin of
+    `a` that fit `reference` best are taken out."""
     a = np.ascontiguousarray(a, dtype=np.complex64)
     reference = np.ascontiguousarray(reference, dtype=np.complex64)
     if a.shape != reference.shape:
@@ -106,24 +118,62 @@ def aligned_error(a, reference) -> float:
     return _core.aligned_error(a, reference)
 
 
-def _flag(name, array, shape):
-    if array is None:
+# --- argument normalisation -------------------------------------------------------------
+
+
+def _intensity(intensity) -> np.ndarray:
+    arr = np.ascontiguousarray(intensity, dtype=np.float32)
+    if arr.ndim not in (2, 3):
+        raise ValueError(f"intensity must be 2D or 3D, got ndim={arr.ndim}")
+    return arr
+
+
+def _warn_negative(intensity: np.ndarray) -> None:
+    share = float(np.count_nonzero(intensity < 0)) / max(intensity.size, 1)
+    if share > NEGATIVE_WARN_FRACTION:
+        warnings.warn(f"{share:.1%} of intensity pixels are negative and will be clipped to 0; "
+                      "consider masking them instead", stacklevel=3)
+
+
+def _bool_plane(name: str, arr, shape):
+    """A boolean array of the intensity's shape as uint8 (None stays None)."""
+    if arr is None:
         return None
-    array = np.ascontiguousarray(array, dtype=bool)
-    if array.shape != shape:
-        raise ValueError(f"{name}.shape {array.shape} != intensity.shape {shape}")
-    return array.view(np.uint8)
+    flags = np.ascontiguousarray(arr, dtype=bool)
+    if flags.shape != shape:
+        raise ValueError(f"{name}.shape {flags.shape} != intensity.shape {shape}")
+    return flags.view(np.uint8)
 
 
-def _start(init, shape, starts):
+def _initial(init, shape, starts):
+    """Starting objects as a (starts, *shape) complex64 array: one object is used for
+    every start."""
     if init is None:
         return None
-    init = np.asarray(init, dtype=np.complex64)
-    if init.shape == shape:
-        init = np.broadcast_to(init, (starts, *shape))
-    if init.shape != (starts, *shape):
-        raise ValueError(f"init.shape {init.shape} must be the intensity's {shape} or (starts, *shape)")
-    return np.ascontiguousarray(init)
+    arr = np.asarray(init, dtype=np.complex64)
+    if arr.shape == shape:
+        arr = np.broadcast_to(arr, (starts, *shape))
+    if arr.shape != (starts, *shape):
+        raise ValueError(f"init.shape {arr.shape} must be the intensity's {shape} or (starts, *shape)")
+    return np.ascontiguousarray(arr)
+
+
+def _resolve(recipe, backend: str):
+    if backend == "auto":
+        backend = "hip" if hip_available() else "cpu"
+    if recipe is None:
+        recipe = default_recipe()
+    elif interop.is_qoraphase(recipe):
+        recipe = interop.from_qoraphase(recipe)
+    return recipe, backend
+
+
+def _package(runs, starts):
+    results = tuple(Reconstruction(obj, sup.astype(bool), err) for obj, sup, err in runs)
+    return results[0] if starts == 1 else Ensemble(results)
+
+
+# --- entry points -----------------------------------------------------------------------
 
 
 def reconstruct(
@@ -131,7 +181,7 @@ def reconstruct(
     *,
     support=None,
     mask=None,
-    recipe=None,
+    recipe: Recipe | None = None,
     starts: int = 1,
     batch: int | None = None,
     seed: int | None = None,
@@ -142,56 +192,25 @@ def reconstruct(
                      "mixed_bf16"] = "batched",
     sync_every: int = 10,
 ) -> Reconstruction | Ensemble:
-    """Array in, reconstruction out.
+    """Reconstruct an object from one measured intensity.
 
-    Args:
-        intensity: measured intensity, centre-origin, 2D or 3D.
-        support: boolean, ``True`` inside the object; ``None`` derives an
-            autocorrelation support.
-        mask: boolean, ``True`` marks *unusable* pixels.
-        recipe: a :class:`Recipe` or a QoraPhase recipe (converted by
-            :func:`from_qoraphase`); ``None`` uses :func:`default_recipe`.
-        starts: number of random starts; more than one returns an :class:`Ensemble`.
-        batch: starts per launch in the batched and reduced-precision variants;
-            ``None`` puts all of them in one.
-        seed: seed of the random starting phases; ``None`` means 0.
-        init: starting object(s), centre-origin, instead of random phases: one array of
-            the intensity's shape for every start, or ``(starts, *shape)``.
-        backend: ``"auto"`` picks ``"hip"`` when a device is present.
-        variant: execution variant (see ``qpcu::Variant``).
+    intensity: centre-origin, 2D or 3D.  support: True inside the object (None: the
+    autocorrelation support at `support_threshold`).  mask: True marks an unusable pixel.
+    recipe: a Recipe, or a QoraPhase recipe or stage (None: `default_recipe()`).
+    starts: random starts; more than one returns an Ensemble.  batch: starts per launch in
+    the batched and reduced-precision variants (None: all in one).  seed: of the random
+    starting phases (None: 0).  init: a starting object for every start, or one per start
+    as (starts, *shape), instead of random phases; it is cut to the support first.
+    backend: "auto" takes "hip" when a device is visible.  variant: the execution variant
+    (qpcu::Variant).  sync_every: iterations between error read-backs.
     """
-    intensity = np.ascontiguousarray(intensity, dtype=np.float32)
-    if intensity.ndim not in (2, 3):
-        raise ValueError(f"intensity must be 2D or 3D, got ndim={intensity.ndim}")
-    negative = float(np.mean(intensity < 0))
-    if negative > NEGATIVE_WARN_FRACTION:
-        warnings.warn(
-            f"{negative:.1%} of intensity pixels are negative and will be clipped to 0; "
-            "consider masking them instead",
-            stacklevel=2,
-        )
-    if backend == "auto":
-        backend = "hip" if hip_available() else "cpu"
-    if recipe is None:
-        recipe = default_recipe()
-    elif interop.is_qoraphase(recipe):
-        recipe = interop.from_qoraphase(recipe)
-    runs = _core.reconstruct(
-        intensity,
-        _flag("support", support, intensity.shape),
-        _flag("mask", mask, intensity.shape),
-        recipe.ops(),
-        starts,
-        0 if batch is None else batch,
-        0 if seed is None else seed,
-        _start(init, intensity.shape, starts),
-        support_threshold,
-        backend,
-        variant,
-        sync_every,
-    )
-    results = tuple(Reconstruction(obj, sup.astype(bool), err) for obj, sup, err in runs)
-    return results[0] if starts == 1 else Ensemble(results)
+    arr = _intensity(intensity)
+    _warn_negative(arr)
+    recipe, backend = _resolve(recipe, backend)
+    runs = _core.reconstruct(arr, _bool_plane("support", support, arr.shape), _bool_plane("mask", mask, arr.shape),
+                             recipe.ops(), starts, batch or 0, seed or 0, _initial(init, arr.shape, starts),
+                             support_threshold, backend, variant, sync_every)
+    return _package(runs, starts)
 
 
 def reconstruct_many(
@@ -199,7 +218,7 @@ def reconstruct_many(
     *,
     supports=None,
     masks=None,
-    recipe=None,
+    recipe: Recipe | None = None,
     starts: int = 1,
     batch: int | None = None,
     seed: int | None = None,
@@ -208,36 +227,24 @@ def reconstruct_many(
     variant: str = "batched",
     sync_every: int = 10,
 ) -> list:
-    """Several independent problems (any mix of 2D/3D shapes) with one recipe.
+    """Reconstruct several independent problems (any mix of 2D and 3D shapes) with one
+    recipe; one Reconstruction or Ensemble per problem, in order.
 
-    On the batched and reduced-precision variants, problems of one shape share one
-    stacked device solver and all shapes advance together, one graph launch per chunk of
-    iterations.  Returns one :class:`Reconstruction` or :class:`Ensemble` per problem."""
-    intensities = [np.ascontiguousarray(i, dtype=np.float32) for i in intensities]
-    if not intensities:
+    On the batched and reduced-precision variants the problems of one shape share one
+    stacked device solver, and every shape advances in the same graph launch."""
+    arrays = [_intensity(i) for i in intensities]
+    if not arrays:
         raise ValueError("no problems")
-    for i in intensities:
-        if i.ndim not in (2, 3):
-            raise ValueError(f"intensity must be 2D or 3D, got ndim={i.ndim}")
-    if backend == "auto":
-        backend = "hip" if hip_available() else "cpu"
-    if recipe is None:
-        recipe = default_recipe()
-    elif interop.is_qoraphase(recipe):
-        recipe = interop.from_qoraphase(recipe)
+    recipe, backend = _resolve(recipe, backend)
 
-    def flags(name, arrays):
-        if arrays is None:
+    def per_problem(name, given):
+        if given is None:
             return None
-        if len(arrays) != len(intensities):
+        if len(given) != len(arrays):
             raise ValueError(f"{name} needs one entry per intensity")
-        return [None if a is None else _flag(name, a, i.shape) for a, i in zip(arrays, intensities)]
+        return [_bool_plane(name, g, a.shape) for g, a in zip(given, arrays)]
 
-    runs = _core.reconstruct_many(intensities, flags("support", supports), flags("mask", masks), recipe.ops(),
-                                  starts, 0 if batch is None else batch, 0 if seed is None else seed,
-                                  support_threshold, backend, variant, sync_every)
-    out = []
-    for per in runs:
-        results = tuple(Reconstruction(obj, sup.astype(bool), err) for obj, sup, err in per)
-        out.append(results[0] if starts == 1 else Ensemble(results))
-    return out
+    runs = _core.reconstruct_many(arrays, per_problem("support", supports), per_problem("mask", masks),
+                                  recipe.ops(), starts, batch or 0, seed or 0, support_threshold, backend, variant,
+                                  sync_every)
+    return [_package(per, starts) for per in runs]

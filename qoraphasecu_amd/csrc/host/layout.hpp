@@ -1,5 +1,7 @@
-// Centre-origin <-> FFT-order reordering and the 3-axis view of a shape.
-// Reference semantics: src/core/layout.hpp:12-47 (numpy ifftshift / fftshift).
+// Array orderings of the host API.  Public arrays are centre-origin (index n / 2 of each
+// axis is the origin); the engine works in FFT order (index 0 is the origin).  Moving
+// between them is numpy's ifftshift / fftshift, the convention of the reference's
+// src/core/layout.hpp.
 #pragma once
 
 #include <array>
@@ -11,25 +13,34 @@
 
 namespace qpcu::detail {
 
-// {1, ny, nx} for 2D, {nz, ny, nx} for 3D.
+// Extents as (planes, rows, columns): a 2D shape has one plane.
 inline std::array<std::size_t, 3> dims3(const Shape& shape) {
-  if (shape.size() == 2) return {1, shape[0], shape[1]};
-  return {shape[0], shape[1], shape[2]};
+  return shape.size() == 2 ? std::array<std::size_t, 3>{1, shape[0], shape[1]}
+                           : std::array<std::size_t, 3>{shape[0], shape[1], shape[2]};
 }
 
-// out[i] = in[(i + s) mod n] along each axis, s = n / 2 into FFT order and n - n / 2 back.
+// For one axis of length `len`: the input coordinate each output coordinate reads.
+// Into FFT order output i reads input (i + len / 2) mod len; back to centre-origin it
+// reads (i + len - len / 2) mod len (the two differ on odd lengths).
+inline std::vector<std::size_t> source_coordinates(std::size_t len, bool into_fft_order) {
+  const std::size_t lead = into_fft_order ? len / 2 : len - len / 2;
+  std::vector<std::size_t> src(len);
+  for (std::size_t i = 0; i < len; ++i) src[i] = i + lead < len ? i + lead : i + lead - len;
+  return src;
+}
+
 template <class T>
-std::vector<T> shifted(std::span<const T> in, const Shape& shape, bool into_fft_order) {
-  const auto d = dims3(shape);
-  std::array<std::size_t, 3> s{};
-  for (int a = 0; a < 3; ++a) s[a] = into_fft_order ? d[a] / 2 : d[a] - d[a] / 2;
-  std::vector<T> out(in.size());
-  std::size_t o = 0;
-  for (std::size_t z = 0; z < d[0]; ++z) {
-    const std::size_t iz = (z + s[0]) % d[0];
-    for (std::size_t y = 0; y < d[1]; ++y) {
-      const std::size_t row = (iz * d[1] + (y + s[1]) % d[1]) * d[2];
-      for (std::size_t x = 0; x < d[2]; ++x) out[o++] = in[row + (x + s[2]) % d[2]];
+std::vector<T> reorder(std::span<const T> in, const Shape& shape, bool into_fft_order) {
+  const auto ext = dims3(shape);
+  const auto planes = source_coordinates(ext[0], into_fft_order);
+  const auto rows = source_coordinates(ext[1], into_fft_order);
+  const auto cols = source_coordinates(ext[2], into_fft_order);
+  std::vector<T> out;
+  out.reserve(in.size());
+  for (std::size_t p : planes) {
+    for (std::size_t r : rows) {
+      const T* line = in.data() + (p * ext[1] + r) * ext[2];
+      for (std::size_t c : cols) out.push_back(line[c]);
     }
   }
   return out;
@@ -37,12 +48,12 @@ std::vector<T> shifted(std::span<const T> in, const Shape& shape, bool into_fft_
 
 template <class T>
 std::vector<T> to_fft_order(std::span<const T> in, const Shape& shape) {
-  return shifted(in, shape, true);
+  return reorder(in, shape, true);
 }
 
 template <class T>
 std::vector<T> to_centered(std::span<const T> in, const Shape& shape) {
-  return shifted(in, shape, false);
+  return reorder(in, shape, false);
 }
 
 }  // namespace qpcu::detail

@@ -1,108 +1,152 @@
-"""Recipe stages, with the names and defaults of the reference's Python API
-(python/qoraphasecu/recipe.py:1-112).  A recipe flattens to the operation 7-tuples
-(kind, n, beta, threshold, sigma, method, axis) that the C++ engine walks.
+"""Recipes: the stage objects a caller composes, and their flattening into the records the
+engine walks.
+
+Public names, fields and defaults are those of the reference's Python API
+(python/qoraphasecu/recipe.py) so that a recipe written for it runs unchanged here.  The
+flattened form is a list of 7-field records (kind, n, beta, threshold, sigma, method,
+axis), the format the C++ binding reads (python/bindings.cpp:22-23); `Op` is a named tuple,
+so a record is also a plain 7-tuple.
 """
 
+from __future__ import annotations
+
 from dataclasses import dataclass
-from typing import Literal
+from typing import Iterator, Literal, NamedTuple
 
 
-def _op(kind, n=0, beta=0.0, threshold=0.0, sigma=0.0, method="max", axis=0):
-    return (kind, n, beta, threshold, sigma, method, axis)
+class Op(NamedTuple):
+    """One engine operation.  Iteration kinds use n and beta, "shrinkwrap" uses threshold,
+    sigma and method, "detwin_begin" uses axis."""
+
+    kind: str
+    n: int = 0
+    beta: float = 0.0
+    threshold: float = 0.0
+    sigma: float = 0.0
+    method: str = "max"
+    axis: int = 0
+
+
+ITERATION_KINDS = frozenset({"er", "hio", "raar"})
+_LEVELS = ("max", "rms")
+
+
+class Stage:
+    """Something that expands to engine operations, in execution order."""
+
+    def records(self) -> Iterator[Op]:
+        raise NotImplementedError
+
+    def ops(self) -> list:
+        return list(self.records())
 
 
 @dataclass(frozen=True)
-class ER:
-    """``x <- P_S P_M x``."""
+class ER(Stage):
+    """Error reduction: inside the support the iterate takes its Fourier projection,
+    outside it becomes zero."""
 
     n: int
 
-    def ops(self):
-        return [_op("er", self.n)]
+    def records(self):
+        yield Op("er", self.n)
 
 
 @dataclass(frozen=True)
-class HIO:
-    """``x <- P_M x`` inside the support, ``x - beta P_M x`` outside."""
+class HIO(Stage):
+    """Hybrid input-output: the projection inside the support, x - beta * projection
+    outside."""
 
     n: int
     beta: float = 0.9
 
-    def ops(self):
-        return [_op("hio", self.n, self.beta)]
+    def records(self):
+        yield Op("hio", self.n, self.beta)
 
 
 @dataclass(frozen=True)
-class RAAR:
-    """``x <- (beta/2)(R_S R_M x + x) + (1 - beta) P_M x``."""
+class RAAR(Stage):
+    """Relaxed averaged alternating reflections: the projection inside the support,
+    beta * x + (1 - 2 beta) * projection outside."""
 
     n: int
     beta: float = 0.9
 
-    def ops(self):
-        return [_op("raar", self.n, self.beta)]
+    def records(self):
+        yield Op("raar", self.n, self.beta)
 
 
 @dataclass(frozen=True)
-class Shrinkwrap:
-    """New support from the blurred magnitude; ``method`` is ``"max"`` or ``"rms"``."""
+class Shrinkwrap(Stage):
+    """Replace the support by the pixels whose Gaussian-blurred (sigma) magnitude exceeds
+    threshold times a level: the blurred maximum ("max") or the root mean square over the
+    current support ("rms")."""
 
     threshold: float = 0.3
     sigma: float = 1.0
     method: Literal["max", "rms"] = "max"
 
     def __post_init__(self):
-        if self.method not in ("max", "rms"):
-            raise ValueError(f"method must be 'max' or 'rms', got {self.method!r}")
+        if self.method not in _LEVELS:
+            raise ValueError(f"Shrinkwrap level {self.method!r} is unknown; use one of {_LEVELS}")
 
-    def ops(self):
-        return [_op("shrinkwrap", threshold=self.threshold, sigma=self.sigma, method=self.method)]
+    def records(self):
+        yield Op("shrinkwrap", threshold=self.threshold, sigma=self.sigma, method=self.method)
 
 
-class Recipe:
-    """Stages in execution order."""
+class Recipe(Stage):
+    """Stages run in the order given."""
 
-    def __init__(self, *stages):
+    def __init__(self, *stages: Stage):
         self.stages = tuple(stages)
 
-    def __len__(self):
+    def __len__(self) -> int:
         return len(self.stages)
 
-    def ops(self):
-        return [op for stage in self.stages for op in stage.ops()]
+    def records(self):
+        for stage in self.stages:
+            yield from stage.records()
 
     def iterations(self) -> int:
-        return sum(op[1] for op in self.ops() if op[0] in ("er", "hio", "raar"))
+        """Total iterations of the flattened recipe (the length of each error history)."""
+        return sum(op.n for op in self.records() if op.kind in ITERATION_KINDS)
 
 
 @dataclass(frozen=True)
-class _Repeat:
-    n: int
+class Repeat(Stage):
+    """`count` copies of a body, one after another."""
+
+    count: int
     body: Recipe
 
-    def ops(self):
-        return self.body.ops() * self.n
+    def records(self):
+        for _ in range(self.count):
+            yield from self.body.records()
 
 
 @dataclass(frozen=True)
-class _Detwin:
+class Detwin(Stage):
+    """The body run on the support cut at its centre of mass along `axis`; the full support
+    is put back afterwards."""
+
     body: Recipe
-    axis: int
+    axis: int = 0
 
-    def ops(self):
-        return [_op("detwin_begin", axis=self.axis), *self.body.ops(), _op("detwin_end")]
-
-
-def repeat(n: int, *stages) -> _Repeat:
-    """``repeat(10, RAAR(50), Shrinkwrap())``"""
-    return _Repeat(n, Recipe(*stages))
+    def records(self):
+        yield Op("detwin_begin", axis=self.axis)
+        yield from self.body.records()
+        yield Op("detwin_end")
 
 
-def detwin(*stages, axis: int = 0) -> _Detwin:
-    """Run *stages* against the support cut at its centre of mass, then restore it."""
-    return _Detwin(Recipe(*stages), axis)
+def repeat(n: int, *stages: Stage) -> Repeat:
+    """repeat(10, RAAR(50), Shrinkwrap()) runs the pair ten times."""
+    return Repeat(n, Recipe(*stages))
+
+
+def detwin(*stages: Stage, axis: int = 0) -> Detwin:
+    return Detwin(Recipe(*stages), axis)
 
 
 def default_recipe() -> Recipe:
-    """RAAR, a detwin pass, then shrinkwrapped RAAR and ER."""
+    """The default of the reference (src/core/recipe.cpp:26-28): 720 iterations."""
     return Recipe(RAAR(50), detwin(HIO(20)), RAAR(50), repeat(8, RAAR(50), Shrinkwrap()), ER(200))

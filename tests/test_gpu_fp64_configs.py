@@ -10,10 +10,10 @@ yardstick is the oracle's own float run against its double run: the device fp32 
 must follow the FP64 error history (to 1e-3 relative) for at least 80% as many iterations
 as the CPU float reference does, the fp16 / bf16-storage paths must agree with it over the first
 iterations to their storage precision, and where the dynamics converge (the noiseless
-128^3 volume) the final error must be as close to the FP64 run's as the CPU float run's is:
-within twice that distance, and never required closer than a floor (1e-4 for fp32 storage,
-1e-3 for fp16 / bf16).  At 128^3 the CPU float run itself ends 4.6e-4 from the FP64 run
-(final errors 0.02919 and 0.02873)."""
+128^3 volume) the final error must be at least as close to the FP64 run's as the CPU float
+run's is, and is never required closer than a floor (1e-4 for fp32 storage, 1e-3 for fp16 /
+bf16).  At 128^3 the CPU float run itself ends 4.6e-4 from the FP64 run (final errors
+0.02919 and 0.02873)."""
 
 import json
 import os
@@ -102,7 +102,7 @@ def test_against_fp64_oracle(name, starts, variants, final_bar):
             # 80% of the CPU float reference's count allows a few times its rounding error
             assert min(track) >= 0.8 * min(yard_track), (variant, track, yard_track)
         if final_bar is not None:
-            bar = max(final_bar[variant], 2.0 * yard_final)
+            bar = max(final_bar[variant], yard_final)
             assert abs(best_gpu - best_ref) <= bar, (variant, best_gpu, best_ref, yard_final)
     _REPORT[name] = rep
 

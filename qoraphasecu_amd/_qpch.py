@@ -110,6 +110,7 @@ def lib():
         L.qpch_group_destroy.argtypes = [vp]
         L.qpch_group_destroy.restype = None
         L.qpch_solver_time_passes.argtypes = [vp, i, i, ctypes.POINTER(f), i, ctypes.POINTER(i)]
+        L.qpch_solver_time_iterations.argtypes = [vp, i, i, i, ctypes.POINTER(f)]
         L.qpch_solver_pass_info.argtypes = [vp, i, ctypes.POINTER(i), ctypes.POINTER(i), i, ctypes.POINTER(i)]
         L.qpch_fft.argtypes = [vp, i, szp, i]
         L.qpch_modulus_project.argtypes = [vp, vp, vp, i, szp, ctypes.POINTER(d)]
@@ -333,6 +334,12 @@ class Solver:
         check(lib().qpch_solver_time_passes(self._h, int(low), reps, ms, 8, ctypes.byref(cnt)))
         names = {0: "fwd", 1: "inv", 2: "modulus", 3: "update"}
         return [(axes[k], names[modes[k]], float(ms[k])) for k in range(cnt.value)]
+
+    def time_iterations(self, low=False, length=10, reps=20):
+        """Microseconds per iteration of all starts, graph-replayed chunks of `length`."""
+        us = ctypes.c_float(0.0)
+        check(lib().qpch_solver_time_iterations(self._h, int(low), length, reps, ctypes.byref(us)))
+        return us.value
 
     def synchronize(self):
         check(lib().qpch_solver_synchronize(self._h))

@@ -620,6 +620,9 @@ struct SolverImpl {
   std::map<std::tuple<int, int, float, float, bool>, hipGraphExec_t> graphs;
 
   ~SolverImpl() {
+    // Drain first: graph execs and buffers must not go while a launch may still use them.
+    for (hipStream_t l : lane_streams) (void)hipStreamSynchronize(l);
+    if (stream) (void)hipStreamSynchronize(stream);
     clear_graphs();
     for (hipStream_t l : lane_streams) (void)hipStreamDestroy(l);
     for (hipEvent_t e : lane_done) (void)hipEventDestroy(e);
@@ -1280,6 +1283,32 @@ struct SolverImpl {
     return k;
   }
 
+  // Average time of one iteration of every launch group on the production path: chunks
+  // of `len` iterations replayed from their hipGraphs (graph schedule) or launched eagerly
+  // (fused schedule), `reps` chunks after one warm-up chunk, between two events on the
+  // solver's stream.  Restarts the run (its error history) and changes the iterate.
+  float time_iterations(bool lowp, int len, int reps) {
+    if (!fused) throw InvalidArgument("iteration timing needs a fused schedule");
+    if (lowp && !low()) throw InvalidArgument("this solver has no reduced-precision storage");
+    // the precision iterate() picks is low() && !(mixed && refine)
+    const bool refine = !lowp;
+    if (!lowp && low() && !mixed) throw InvalidArgument("a reduced-precision solver without mixed runs no fp32 iterations");
+    begin_run(len * (reps + 1));
+    iterate(len, 0.9f, -0.8f, refine);
+    hipEvent_t e0{}, e1{};
+    QPCH_HIP(hipEventCreate(&e0));
+    QPCH_HIP(hipEventCreate(&e1));
+    QPCH_HIP(hipEventRecord(e0, stream));
+    for (int r = 0; r < reps; ++r) iterate(len, 0.9f, -0.8f, refine);
+    QPCH_HIP(hipEventRecord(e1, stream));
+    QPCH_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    QPCH_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms * 1e3f / static_cast<float>(reps * len);
+  }
+
   void read_state(float* host) {
     if (low_active) convert_x(false);
     QPCH_HIP(hipStreamSynchronize(stream));
@@ -1316,6 +1345,7 @@ struct qpch_group {
   std::map<std::tuple<int, float, float, int>, Entry> graphs;
 
   ~qpch_group() {
+    if (main) (void)hipStreamSynchronize(main);
     for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second.exec);
     for (hipEvent_t e : done) (void)hipEventDestroy(e);
     if (fork) (void)hipEventDestroy(fork);
@@ -1422,7 +1452,16 @@ int qpch_device_info(qpch_device_info_t* info) {
     info->l2_bytes = static_cast<size_t>(prop.l2CacheSize);
     int clock_khz = 0;
     QPCH_HIP(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeMemoryClockRate, dev));
-    info->peak_bandwidth_gbs = 2.0 * clock_khz * 1e3 * (prop.memoryBusWidth / 8.0) / 1e9;
+    // 2 x memory clock x bus width (the reference's formula, src/cuda/operators.cu:18-36)
+    // undercounts HBM3E: on gfx950 the runtime's clock and width give 4096 GB/s, half of
+    // the 8 TB/s AMD publishes for MI355X (and that the fused passes' measured traffic
+    // already exceeds).  Known devices report their published peak.
+    const std::string arch(prop.gcnArchName);
+    if (arch.rfind("gfx950", 0) == 0) {
+      info->peak_bandwidth_gbs = 8000.0;
+    } else {
+      info->peak_bandwidth_gbs = 2.0 * clock_khz * 1e3 * (prop.memoryBusWidth / 8.0) / 1e9;
+    }
   });
 }
 
@@ -1562,6 +1601,14 @@ int qpch_solver_time_passes(qpch_solver* s, int low, int reps, float* ms, int ca
   return guarded([&] {
     if (reps < 1) throw qpch::InvalidArgument("reps must be >= 1");
     *count = s->impl.time_passes(low != 0, reps, ms, cap);
+  });
+}
+
+int qpch_solver_time_iterations(qpch_solver* s, int low, int len, int reps, float* us_per_iteration) {
+  return guarded([&] {
+    if (len < 1 || reps < 1) throw qpch::InvalidArgument("len and reps must be >= 1");
+    if (!us_per_iteration) throw qpch::InvalidArgument("null output");
+    *us_per_iteration = s->impl.time_iterations(low != 0, len, reps);
   });
 }
 
@@ -1717,7 +1764,6 @@ int qpch_simulate_intensity(const float* object, int count, int ndim, const size
 namespace {
 std::mutex g_tune_mu;
 std::map<std::string, qpch_tuning> g_tune_cache;
-std::map<std::string, int> g_radix_cache;  // (shape, storage, mixed) -> radix
 std::string g_tune_log;
 
 std::string tune_key(const qpch_problem& p) {
@@ -1766,32 +1812,12 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
     // caller asks for line passes.
     const bool plane_ok = d.nd == 3 && qpch::plane_supported(d.d[1], d.d[2]);
     const std::vector<int> planes{problem->plane != 0 ? problem->plane : (plane_ok ? 1 : 0)};
-    // The radix changes the arithmetic (rounding), and a start's result must not depend on
-    // how the starts are split into launches (batched == looped).  So it is tuned once per
-    // (shape, storage, mixed) at one canonical configuration (all of the first caller's
-    // starts in one launch, one lane, XCD-affine numbering) and then kept for every split
-    // of that shape; radix 8 must be 2% faster to replace 16.  A caller may fix it in
-    // problem->radix.
-    bool r8_possible = qpch::fusable(d);
-    if (r8_possible) {
-      bool any = false;
-      for (int a = 0; a < d.nd; ++a) any = any || qpch::has_radix8(d.d[a]);
-      r8_possible = any;
-    }
-    std::string rkey = std::to_string(d.nd);
-    for (int a = 0; a < d.nd; ++a) rkey += "x" + std::to_string(d.d[a]);
-    rkey += "/st" + std::to_string(problem->storage) + "/m" + std::to_string(problem->mixed);
-    int tuned_radix = 16;
-    bool radix_known = problem->radix != 0 || !r8_possible;
-    if (problem->radix != 0) tuned_radix = problem->radix;
-    if (!radix_known) {
-      std::lock_guard<std::mutex> lock(g_tune_mu);
-      auto it = g_radix_cache.find(rkey);
-      if (it != g_radix_cache.end()) {
-        tuned_radix = it->second;
-        radix_known = true;
-      }
-    }
+    // The radix changes the arithmetic (rounding), so it is not tuned: results must not
+    // depend on timings, on the split of the starts into launches (batched == looped) or
+    // on what the process ran before.  Radix 16 everywhere unless the caller fixes it in
+    // problem->radix (radix 8 was never 2% faster on the workloads with all their starts
+    // in one launch: 128^2 to 1024^2 and 128^3, fp32 and fp16).
+    const int tuned_radix = problem->radix != 0 ? problem->radix : 16;
     std::vector<int> radices{tuned_radix};
     std::string line = "qpch autotune " + key + ":";
     qpch_tuning best{};
@@ -1848,21 +1874,6 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
       // lane, XCD-affine numbering), which then stays unless another is clearly faster.
       if (r.us_per_iteration < 0.98 * best.us_per_iteration) best = r;
     };
-    if (!radix_known) {
-      const qpch_tuning canon16{total, planes[0], 0, 0.0, 1, 16};
-      qpch_tuning canon8 = canon16;
-      canon8.radix = 8;
-      const double t16 = measure(canon16), t8 = measure(canon8);
-      tuned_radix = t8 < 0.98 * t16 ? 8 : 16;
-      line += " [radix for " + rkey + ", all " + std::to_string(total) + " starts in one launch: 16: " +
-              std::to_string(t16) + " us/it, 8: " + std::to_string(t8) + " us/it -> " + std::to_string(tuned_radix) +
-              "]";
-      std::lock_guard<std::mutex> lock(g_tune_mu);
-      auto [it, fresh] = g_radix_cache.emplace(rkey, tuned_radix);
-      tuned_radix = it->second;  // a concurrent tuner's choice stands
-      (void)fresh;
-      radices[0] = tuned_radix;
-    }
     // first the split of the starts into launches (default kernels), then the kernel
     // choices for the best split
     for (int b : batches) {

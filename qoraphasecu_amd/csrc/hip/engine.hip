@@ -1417,6 +1417,117 @@ struct qpch_group {
   }
 };
 
+namespace qpch {
+
+// --- the amplitude loss and its gradient ---------------------------------------------------
+// Scratch of one call: the wave (the batch's transforms) and the FP64 loss partials, carved
+// from one block at 256-byte aligned offsets.
+struct LossPlan {
+  bool fused = false;  // fp32 with power-of-two extents: the fused line passes
+  int parts = 0;       // loss partials per object
+  size_t wave_bytes = 0, part_offset = 0, bytes = 0;
+};
+
+struct LossCall {
+  const void* x;
+  const void* amplitude;
+  long long amplitude_stride;
+  void* grad;
+  double* loss;
+  Dims d;
+  int batch, precision;
+  hipStream_t st;
+  LossPlan plan;
+};
+
+size_t round256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
+LossCall loss_call(const void* x, const void* amplitude, long long amplitude_stride, void* grad, double* loss,
+                   int ndim, const size_t* shape, int batch, int precision, void* stream) {
+  LossCall c{x, amplitude, amplitude_stride, grad, loss, make_dims(ndim, shape), batch, precision,
+             static_cast<hipStream_t>(stream), {}};
+  const Dims& d = c.d;
+  if (batch < 1) throw InvalidArgument("batch must be >= 1");
+  if (precision != 0 && precision != 1) throw InvalidArgument("precision must be 0 (fp32) or 1 (fp64)");
+  if (!x || !amplitude || !grad || !loss) throw InvalidArgument("null pointer");
+  if (amplitude_stride != 0 && amplitude_stride != d.n) {
+    throw InvalidArgument("amplitude_stride must be 0 (shared) or the element count (one per object)");
+  }
+  LossPlan& pl = c.plan;
+  pl.fused = precision == 0 && fusable(d);
+  if (pl.fused) {
+    pl.parts = pass_partials<8>(d.d[0], d.inner(0) > 1, static_cast<int>(d.outer(0)), static_cast<int>(d.inner(0)));
+  } else {
+    pl.parts = grid_for(d.n, 256, 512);
+  }
+  pl.wave_bytes = (precision == 0 ? sizeof(float2) : sizeof(double2)) * static_cast<size_t>(d.n) * batch;
+  pl.part_offset = round256(pl.wave_bytes);
+  pl.bytes = pl.part_offset + round256(sizeof(double) * static_cast<size_t>(pl.parts) * batch);
+  return c;
+}
+
+// Runs the loss and gradient of a validated call with its scratch at `base` (plan.bytes,
+// 256-byte aligned).  Every scratch element is written before it is read.
+void amplitude_loss_run(const LossCall& c, void* base) {
+  const Dims& d = c.d;
+  const int batch = c.batch;
+  const hipStream_t st = c.st;
+  const long long total = d.n * batch;
+  const double s = 1.0 / std::sqrt(static_cast<double>(d.n));
+  auto* partials = reinterpret_cast<double*>(static_cast<unsigned char*>(base) + c.plan.part_offset);
+  const int parts = c.plan.parts;
+  if (c.plan.fused) {
+    auto* w = static_cast<float2*>(base);
+    auto pass = [&](int axis, dev::Mode mode, const void* src, float in_scale, float out_scale) {
+      dev::PassArgs p{};
+      p.outer = static_cast<int>(d.outer(axis));
+      p.inner = static_cast<int>(d.inner(axis));
+      p.n = d.n;
+      p.w = w;
+      p.src = src;
+      p.x = const_cast<void*>(c.x);
+      p.out = c.grad;
+      p.amp = static_cast<const float*>(c.amplitude);
+      p.amp_stride = c.amplitude_stride;
+      p.partials = partials;
+      p.partial_stride = parts;
+      p.in_scale = in_scale;
+      p.out_scale = out_scale;
+      launch_pass<float2>(d.d[axis], p.inner > 1, mode, p, batch, st);
+    };
+    const int last = d.nd - 1;
+    pass(last, dev::Mode::kFwd, c.x, 1.0f, 1.0f);
+    if (d.nd == 3) pass(1, dev::Mode::kFwd, nullptr, 1.0f, 1.0f);
+    pass(0, dev::Mode::kModulus, nullptr, static_cast<float>(s), static_cast<float>(s));
+    if (d.nd == 3) pass(1, dev::Mode::kInv, nullptr, 1.0f, 1.0f);
+    pass(last, dev::Mode::kGrad, nullptr, 1.0f, 1.0f);
+  } else if (c.precision == 0) {
+    auto* w = static_cast<float2*>(base);
+    QPCH_HIP(hipMemcpyAsync(w, c.x, sizeof(float2) * total, hipMemcpyDeviceToDevice, st));
+    device_fft(w, d, batch, true, 1.0f, st);
+    hipLaunchKernelGGL(dev::loss_project<float>, dim3(parts, batch), dim3(256), 0, st, w,
+                       static_cast<const float*>(c.amplitude), c.amplitude_stride, d.n, static_cast<float>(s),
+                       partials);
+    device_fft(w, d, batch, false, 1.0f, st);
+    hipLaunchKernelGGL(dev::grad_from<float>, dim3(grid_for(total)), dim3(256), 0, st,
+                       static_cast<const float2*>(c.x), w, static_cast<float2*>(c.grad), total, 1.0f);
+  } else {
+    auto* w = static_cast<double2*>(base);
+    QPCH_HIP(hipMemcpyAsync(w, c.x, sizeof(double2) * total, hipMemcpyDeviceToDevice, st));
+    device_fft_f64(w, d, batch, true, st);
+    hipLaunchKernelGGL(dev::loss_project<double>, dim3(parts, batch), dim3(256), 0, st, w,
+                       static_cast<const double*>(c.amplitude), c.amplitude_stride, d.n, s, partials);
+    device_fft_f64(w, d, batch, false, st);
+    hipLaunchKernelGGL(dev::grad_from<double>, dim3(grid_for(total)), dim3(256), 0, st,
+                       static_cast<const double2*>(c.x), w, static_cast<double2*>(c.grad), total, 1.0);
+  }
+  QPCH_HIP(hipGetLastError());
+  hipLaunchKernelGGL(dev::sum_partials, dim3((batch + 63) / 64), dim3(64), 0, st, partials, parts, c.loss, batch);
+  QPCH_HIP(hipGetLastError());
+}
+
+}  // namespace qpch
+
 using qpch::guarded;
 
 extern "C" {
@@ -1905,87 +2016,18 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
 int qpch_amplitude_loss(const void* x, const void* amplitude, long long amplitude_stride, void* grad,
                         double* loss, int ndim, const size_t* shape, int batch, int precision, void* stream) {
   return guarded([&] {
-    const qpch::Dims d = qpch::make_dims(ndim, shape);
-    if (batch < 1) throw qpch::InvalidArgument("batch must be >= 1");
-    if (precision != 0 && precision != 1) throw qpch::InvalidArgument("precision must be 0 (fp32) or 1 (fp64)");
-    if (!x || !amplitude || !grad || !loss) throw qpch::InvalidArgument("null pointer");
-    if (amplitude_stride != 0 && amplitude_stride != d.n) {
-      throw qpch::InvalidArgument("amplitude_stride must be 0 (shared) or the element count (one per object)");
-    }
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    const long long total = d.n * batch;
-    const double s = 1.0 / std::sqrt(static_cast<double>(d.n));
+    const qpch::LossCall call =
+        qpch::loss_call(x, amplitude, amplitude_stride, grad, loss, ndim, shape, batch, precision, stream);
     // stream-ordered scratch: safe for callers that interleave several streams
-    struct Scratch {
-      hipStream_t st;
-      std::vector<void*> ptrs;
-      void* get(size_t bytes) {
-        void* p = nullptr;
-        QPCH_HIP(hipMallocAsync(&p, bytes, st));
-        ptrs.push_back(p);
-        return p;
-      }
-      ~Scratch() {
-        for (void* p : ptrs) (void)hipFreeAsync(p, st);
-      }
-    } scratch{st, {}};
-    if (precision == 0 && qpch::fusable(d)) {
-      auto* w = static_cast<float2*>(scratch.get(sizeof(float2) * total));
-      const int o0 = static_cast<int>(d.outer(0)), in0 = static_cast<int>(d.inner(0));
-      const int parts = qpch::pass_partials<8>(d.d[0], in0 > 1, o0, in0);
-      auto* partials = static_cast<double*>(scratch.get(sizeof(double) * parts * batch));
-      auto pass = [&](int axis, qpch::dev::Mode mode, const void* src, float in_scale, float out_scale) {
-        qpch::dev::PassArgs p{};
-        p.outer = static_cast<int>(d.outer(axis));
-        p.inner = static_cast<int>(d.inner(axis));
-        p.n = d.n;
-        p.w = w;
-        p.src = src;
-        p.x = const_cast<void*>(x);
-        p.out = grad;
-        p.amp = static_cast<const float*>(amplitude);
-        p.amp_stride = amplitude_stride;
-        p.partials = partials;
-        p.partial_stride = parts;
-        p.in_scale = in_scale;
-        p.out_scale = out_scale;
-        qpch::launch_pass<float2>(d.d[axis], p.inner > 1, mode, p, batch, st);
-      };
-      const int last = d.nd - 1;
-      pass(last, qpch::dev::Mode::kFwd, x, 1.0f, 1.0f);
-      if (d.nd == 3) pass(1, qpch::dev::Mode::kFwd, nullptr, 1.0f, 1.0f);
-      pass(0, qpch::dev::Mode::kModulus, nullptr, static_cast<float>(s), static_cast<float>(s));
-      if (d.nd == 3) pass(1, qpch::dev::Mode::kInv, nullptr, 1.0f, 1.0f);
-      pass(last, qpch::dev::Mode::kGrad, nullptr, 1.0f, 1.0f);
-      hipLaunchKernelGGL(qpch::dev::sum_partials, dim3((batch + 63) / 64), dim3(64), 0, st, partials, parts, loss,
-                         batch);
-      QPCH_HIP(hipGetLastError());
-      return;
+    void* base = nullptr;
+    QPCH_HIP(hipMallocAsync(&base, call.plan.bytes, call.st));
+    try {
+      qpch::amplitude_loss_run(call, base);
+    } catch (...) {
+      (void)hipFreeAsync(base, call.st);
+      throw;
     }
-    const int g = qpch::grid_for(d.n, 256, 512);
-    auto* partials = static_cast<double*>(scratch.get(sizeof(double) * g * batch));
-    if (precision == 0) {
-      auto* w = static_cast<float2*>(scratch.get(sizeof(float2) * total));
-      QPCH_HIP(hipMemcpyAsync(w, x, sizeof(float2) * total, hipMemcpyDeviceToDevice, st));
-      qpch::device_fft(w, d, batch, true, 1.0f, st);
-      hipLaunchKernelGGL(qpch::dev::loss_project<float>, dim3(g, batch), dim3(256), 0, st, w,
-                         static_cast<const float*>(amplitude), amplitude_stride, d.n, static_cast<float>(s), partials);
-      qpch::device_fft(w, d, batch, false, 1.0f, st);
-      hipLaunchKernelGGL(qpch::dev::grad_from<float>, dim3(qpch::grid_for(total)), dim3(256), 0, st,
-                         static_cast<const float2*>(x), w, static_cast<float2*>(grad), total, 1.0f);
-    } else {
-      auto* w = static_cast<double2*>(scratch.get(sizeof(double2) * total));
-      QPCH_HIP(hipMemcpyAsync(w, x, sizeof(double2) * total, hipMemcpyDeviceToDevice, st));
-      qpch::device_fft_f64(w, d, batch, true, st);
-      hipLaunchKernelGGL(qpch::dev::loss_project<double>, dim3(g, batch), dim3(256), 0, st, w,
-                         static_cast<const double*>(amplitude), amplitude_stride, d.n, s, partials);
-      qpch::device_fft_f64(w, d, batch, false, st);
-      hipLaunchKernelGGL(qpch::dev::grad_from<double>, dim3(qpch::grid_for(total)), dim3(256), 0, st,
-                         static_cast<const double2*>(x), w, static_cast<double2*>(grad), total, 1.0);
-    }
-    QPCH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(qpch::dev::sum_partials, dim3((batch + 63) / 64), dim3(64), 0, st, partials, g, loss, batch);
-    QPCH_HIP(hipGetLastError());
+    QPCH_HIP(hipFreeAsync(base, call.st));
   });
 }
 

@@ -6,10 +6,14 @@ the slow side).  With QPCH_REPORT=<file> the numbers are written as JSON.
 
 HIO / RAAR with Shrinkwrap and detwin is chaotic: from the same start, two runs that differ
 only in rounding separate after some iterations and end in different local minima.  The
-yardstick is the oracle's own float run against its double run: the device fp32 paths
-must follow the FP64 error history (to 1e-3 relative) for at least 80% as many iterations
-as the CPU float reference does, the fp16 / bf16-storage paths must agree with it over the first
-iterations to their storage precision, and where the dynamics converge (the noiseless
+yardstick is float rounding itself, measured on the same starts by two independent float
+runs followed against the oracle's double run: the oracle's own float run and, on the 2D
+workloads, a numpy float32 run (pocketfft, the reference's CPU FFT library;
+tests/numpy_float32.py).  The two differ by up to 8 iterations from one problem to the
+next (256^2 ensemble problem: 39-42 against 34).  The device fp32 paths must follow the
+FP64 error history (to 1e-3 relative) for at least 80% as many iterations as the shorter
+of the two float references does, the fp16 / bf16-storage paths must agree with it over the
+first iterations to their storage precision, and where the dynamics converge (the noiseless
 128^3 volume) the final error must be at least as close to the FP64 run's as the CPU float
 run's is, and is never required closer than a floor (1e-4 for fp32 storage, 1e-3 for fp16 /
 bf16).  At 128^3 the CPU float run itself ends 4.6e-4 from the FP64 run (final errors
@@ -21,6 +25,7 @@ import os
 import numpy as np
 import pytest
 
+import numpy_float32
 import oracle
 
 pytestmark = pytest.mark.gpu
@@ -68,12 +73,18 @@ def test_against_fp64_oracle(name, starts, variants, final_bar):
     yard.run(w["recipe"].ops())
     yard_res = yard.results()
     yard_track = [_tracking(r[2], h) for r, h in zip(yard_res, ref_hist)]
+    if len(w["shape"]) == 2:  # the second float reference (at 128^3 it would take minutes)
+        np_runs = numpy_float32.run(amp, mask, sup, numpy_float32.random_starts(amp, sup, starts), w["recipe"].ops())
+        np_track = [_tracking(h2, h) for h2, h in zip(np_runs, ref_hist)]
+    else:
+        np_track = yard_track
     yard_final = abs(min(float(r[2][-1]) for r in yard_res) - best_ref)
     rep = {"starts": starts, "iterations": w["recipe"].iterations(),
            "fp64_final_errors": [float(h[-1]) for h in ref_hist], "fp64_best_final_error": best_ref,
-           "oracle_float_iterations_within_1e-3": yard_track, "oracle_float_best_final_abs_diff": yard_final}
+           "oracle_float_iterations_within_1e-3": yard_track, "numpy_float32_iterations_within_1e-3": np_track,
+           "oracle_float_best_final_abs_diff": yard_final}
     print(f"{name}: the oracle's float run tracks its double run to 1e-3 for {min(yard_track)}-{max(yard_track)} "
-          f"iterations")
+          f"iterations, numpy float32 for {min(np_track)}-{max(np_track)}")
     for variant in variants:
         got = qpc.reconstruct(w["intensity"], support=sup.astype(bool),
                               mask=None if w["mask"] is None else w["mask"].astype(bool), recipe=w["recipe"],
@@ -100,7 +111,7 @@ def test_against_fp64_oracle(name, starts, variants, final_bar):
         else:
             # the iterations to divergence grow with the log of the per-iteration rounding:
             # 80% of the CPU float reference's count allows a few times its rounding error
-            assert min(track) >= 0.8 * min(yard_track), (variant, track, yard_track)
+            assert min(track) >= 0.8 * min(min(yard_track), min(np_track)), (variant, track, yard_track, np_track)
         if final_bar is not None:
             bar = max(final_bar[variant], yard_final)
             assert abs(best_gpu - best_ref) <= bar, (variant, best_gpu, best_ref, yard_final)
@@ -133,11 +144,14 @@ def test_ensemble_against_fp64_oracle():
         yard = oracle.Solver(amp, m, sup, starts=ENSEMBLE_STARTS, seed=0, prec="f")
         yard.run(ops)
         yard_track = [_tracking(r[2], h) for r, h in zip(yard.results(), hist)]
-        refs.append(dict(sup=sup.astype(bool), want=want, hist=hist, yard_track=yard_track))
+        np_runs = numpy_float32.run(amp, m, sup, numpy_float32.random_starts(amp, sup, ENSEMBLE_STARTS), ops)
+        np_track = [_tracking(h2, h) for h2, h in zip(np_runs, hist)]
+        refs.append(dict(sup=sup.astype(bool), want=want, hist=hist, yard_track=yard_track, np_track=np_track))
     rep = {"problems": [f"{p[1].shape[0]}^2 (problem {k})" for k, p in zip(ENSEMBLE_SAMPLE, picked)],
            "starts": ENSEMBLE_STARTS, "iterations": w["recipe"].iterations(),
            "fp64_best_final_errors": [min(float(h[-1]) for h in r["hist"]) for r in refs],
-           "oracle_float_iterations_within_1e-3": [r["yard_track"] for r in refs]}
+           "oracle_float_iterations_within_1e-3": [r["yard_track"] for r in refs],
+           "numpy_float32_iterations_within_1e-3": [r["np_track"] for r in refs]}
     for variant in ("batched", "mixed"):
         runs = qpc.reconstruct_many([p[1] for p in picked], supports=[r["sup"] for r in refs],
                                     masks=[p[2].astype(bool) for p in picked], recipe=w["recipe"],
@@ -158,9 +172,11 @@ def test_ensemble_against_fp64_oracle():
             print(f"c3 problem {k} ({p[1].shape[0]}^2) {variant}: best final error {min(errs):.4e} (fp64 "
                   f"{best_ref:.4e}); first {LOW_ITERS} iterations within {early:.1e}; tracks the fp64 history to "
                   f"1e-3 for {min(track)}-{max(track)} iterations (oracle float run: {min(r['yard_track'])}-"
-                  f"{max(r['yard_track'])}); object error vs fp64 median {float(np.median(obj)):.2e}")
+                  f"{max(r['yard_track'])}, numpy float32: {min(r['np_track'])}-{max(r['np_track'])}); object error "
+                  f"vs fp64 median {float(np.median(obj)):.2e}")
             if variant == "mixed":
                 assert early <= LOW_TOL, (variant, k, early)
             else:
-                assert min(track) >= 0.8 * min(r["yard_track"]), (variant, k, track, r["yard_track"])
+                floor = min(min(r["yard_track"]), min(r["np_track"]))
+                assert min(track) >= 0.8 * floor, (variant, k, track, r["yard_track"], r["np_track"])
     _REPORT["c3"] = rep

@@ -22,6 +22,7 @@ work-group per start (the whole-image kernel: 16 x 64 ... 128 x 64)."""
 import numpy as np
 import pytest
 
+import numpy_float32
 import oracle
 import recipes as R
 import synthetic
@@ -38,26 +39,6 @@ def _worst(errs, ref):
     """Per iteration, the larger relative error difference over the starts (errs [iters,
     starts] against a list of per-start histories)."""
     return np.max(np.stack([np.abs(errs[:, k] - ref[k]) / ref[k] for k in range(len(ref))]), axis=0)
-
-
-def _numpy_float32_run(amp, mask, sup, x0, kind, iters, beta=0.9):
-    """The iteration in numpy float32 on FFT-order arrays (numpy.fft is pocketfft; the
-    element-wise steps are those of src/cpu/kernels.cpp:25-50).  Error history as the
-    oracle's: measured on the iterate entering each iteration."""
-    a, b = (np.float32(c) for c in R.coefficients(kind, beta))
-    usable = mask == 0
-    energy = float(np.sum(amp.astype(np.float64)[usable] ** 2))
-    x = np.where(sup, x0, 0).astype(np.complex64)
-    errs = np.empty(iters)
-    for t in range(iters):
-        w = np.fft.fftn(x, norm="ortho").astype(np.complex64)
-        mag = np.sqrt(w.real * w.real + w.imag * w.imag)
-        d = (mag - amp)[usable].astype(np.float64)
-        errs[t] = np.sqrt(np.sum(d * d) / (energy + 1e-12))
-        target = np.where(usable, amp, mag)
-        p = np.fft.ifftn(w * (target / (mag + np.float32(1e-12))), norm="ortho").astype(np.complex64)
-        x = np.where(sup, p, a * x + b * p).astype(np.complex64)
-    return errs
 
 
 @pytest.mark.parametrize("shape", SHAPES)
@@ -77,8 +58,7 @@ def test_per_iteration_error(qpch, shape, kind, iters):
     s = qpch.Solver(f(amp), f(mask), f(sup), starts=2, start=np.stack([f(x) for x in start]), schedule="graph")
     errs = R.run_abi(s, ops).astype(np.float64)
     s.close()
-    numpy32 = np.stack([_numpy_float32_run(f(amp), f(mask), f(sup).astype(bool), f(start[k]), kind, iters)
-                        for k in range(2)], axis=1)
+    numpy32 = np.stack(numpy_float32.run(amp, mask, sup, [f(x) for x in start], ops), axis=1)
     d = _worst(errs, hist["f"])
     drift = _worst(np.stack(hist["f"], axis=1), hist["d"])
     spread = _worst(numpy32, hist["f"])

@@ -237,6 +237,21 @@ int qpch_simulate_intensity(const float* object, int count, int ndim, const size
 int qpch_amplitude_loss(const void* x, const void* amplitude, long long amplitude_stride, void* grad,
                         double* loss, int ndim, const size_t* shape, int batch, int precision, void* stream);
 
+/* qpch_amplitude_loss without any allocation, for callers that cannot allocate device
+ * memory (code under hipGraph / torch CUDA-graph capture, tight refinement loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Same arguments, results and
+ * kernels as qpch_amplitude_loss (identical bits); its scratch (the batch's transforms and
+ * the FP64 loss partials: 8 or 16 bytes per element of x plus 8 bytes per partial, each
+ * part rounded up to 256 bytes) is a 32 MiB device array of the library, one per GPU, held
+ * from load until the process ends.  A call needing more scratch fails with QPCH_EINVAL
+ * before any work is queued (fp32: x of just under 4.2 million elements in all, e.g.
+ * 32 x 256^2 or 1 x 128^3; fp64: half that).  Every call on one GPU uses that GPU's array: calls that overlap on
+ * one GPU (two streams, two threads) race and must be ordered by the caller.  Nothing in
+ * the array needs clearing between calls. */
+int qpch_amplitude_loss_noalloc(const void* x, const void* amplitude, long long amplitude_stride, void* grad,
+                                double* loss, int ndim, const size_t* shape, int batch, int precision,
+                                void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -117,6 +117,7 @@ def lib():
         L.qpch_support_project.argtypes = [vp, vp, sz]
         L.qpch_gaussian_blur.argtypes = [vp, vp, i, szp, f]
         L.qpch_amplitude_loss.argtypes = [vp, vp, ctypes.c_longlong, vp, vp, i, szp, i, i, vp]
+        L.qpch_amplitude_loss_noalloc.argtypes = [vp, vp, ctypes.c_longlong, vp, vp, i, szp, i, i, vp]
         L.qpch_autocorrelation_support.argtypes = [vp, i, i, szp, f, vp]
         L.qpch_simulate_intensity.argtypes = [vp, i, i, szp, vp]
         _lib = L
@@ -228,6 +229,15 @@ def amplitude_loss(x_ptr, amp_ptr, amp_stride, grad_ptr, loss_ptr, shape, batch,
     check(lib().qpch_amplitude_loss(ctypes.c_void_p(x_ptr), ctypes.c_void_p(amp_ptr), amp_stride,
                                     ctypes.c_void_p(grad_ptr), ctypes.c_void_p(loss_ptr), nd, sh, batch,
                                     precision, ctypes.c_void_p(stream)))
+
+
+def amplitude_loss_noalloc(x_ptr, amp_ptr, amp_stride, grad_ptr, loss_ptr, shape, batch, precision, stream):
+    """Device-pointer entry that allocates nothing: see qpch_amplitude_loss_noalloc in
+    include/qpch.h."""
+    nd, sh = _shape(shape)
+    check(lib().qpch_amplitude_loss_noalloc(ctypes.c_void_p(x_ptr), ctypes.c_void_p(amp_ptr), amp_stride,
+                                            ctypes.c_void_p(grad_ptr), ctypes.c_void_p(loss_ptr), nd, sh, batch,
+                                            precision, ctypes.c_void_p(stream)))
 
 
 # --- solver ------------------------------------------------------------------------------

@@ -1417,6 +1417,13 @@ struct qpch_group {
   }
 };
 
+// Device scratch of qpch_amplitude_loss_noalloc: the wave and loss partials of one call.
+// Every GPU the library runs on holds its own copy from module load until the process ends;
+// all calls on one GPU share it.  Host code never passes this name to a kernel: it is a
+// host-side stand-in, and the device address comes from hipGetSymbolAddress.
+constexpr size_t kNoAllocScratchBytes = size_t{32} << 20;  // 32 MiB
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 namespace qpch {
 
 // --- the amplitude loss and its gradient ---------------------------------------------------
@@ -1524,6 +1531,25 @@ void amplitude_loss_run(const LossCall& c, void* base) {
   QPCH_HIP(hipGetLastError());
   hipLaunchKernelGGL(dev::sum_partials, dim3((batch + 63) / 64), dim3(64), 0, st, partials, parts, c.loss, batch);
   QPCH_HIP(hipGetLastError());
+}
+
+// The current device's copy of `scratch`, looked up once per device with that device
+// current (each GPU holds the array at its own address).
+void* noalloc_scratch() {
+  static std::mutex mu;
+  static std::map<int, void*> address;  // device -> its copy
+  int dev = 0;
+  QPCH_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = address.find(dev);
+  if (it != address.end()) return it->second;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || ptr == nullptr) {
+    throw DeviceError(std::string("the no-allocation scratch has no device address: ") + hipGetErrorString(e));
+  }
+  address.emplace(dev, ptr);
+  return ptr;
 }
 
 }  // namespace qpch
@@ -2028,6 +2054,21 @@ int qpch_amplitude_loss(const void* x, const void* amplitude, long long amplitud
       throw;
     }
     QPCH_HIP(hipFreeAsync(base, call.st));
+  });
+}
+
+int qpch_amplitude_loss_noalloc(const void* x, const void* amplitude, long long amplitude_stride, void* grad,
+                                double* loss, int ndim, const size_t* shape, int batch, int precision, void* stream) {
+  return guarded([&] {
+    const qpch::LossCall call =
+        qpch::loss_call(x, amplitude, amplitude_stride, grad, loss, ndim, shape, batch, precision, stream);
+    if (call.plan.bytes > kNoAllocScratchBytes) {
+      throw qpch::InvalidArgument("this call needs " + std::to_string(call.plan.bytes) +
+                                  " bytes of scratch; qpch_amplitude_loss_noalloc holds " +
+                                  std::to_string(kNoAllocScratchBytes) +
+                                  " (use qpch_amplitude_loss, which allocates)");
+    }
+    qpch::amplitude_loss_run(call, qpch::noalloc_scratch());
   });
 }
 

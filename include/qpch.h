@@ -196,6 +196,34 @@ int qpch_group_iterate(qpch_group* group, int iterations, float a, float b, int 
 int qpch_group_synchronize(qpch_group* group);
 void qpch_group_destroy(qpch_group* group);
 
+/* One step of a recipe for qpch_group_run: the flat recipe of the reference
+ * (include/qoraphasecu/recipe.hpp:44-80) with each iteration stage as its coefficients. */
+#define QPCH_OP_ITERATE 0       /* `iterations` of x <- S ? P_M x : a x + b P_M x */
+#define QPCH_OP_SHRINKWRAP 1    /* threshold, sigma, method (0 max, 1 rms) */
+#define QPCH_OP_DETWIN_BEGIN 2  /* save the support, cut it at its centre of mass along `axis` */
+#define QPCH_OP_DETWIN_END 3    /* restore the saved support */
+typedef struct {
+  int kind;
+  int iterations;
+  float a, b;
+  int refine;  /* ITERATE: an ER stage (run in fp32 by a mixed solver) */
+  float threshold, sigma;
+  int method;
+  int axis;
+} qpch_op;
+
+/* Every solver of the group runs the whole recipe `ops` in ONE graph launch: each launch
+ * group of each solver (its `batch` starts) is an independent branch that carries its
+ * starts through every iteration, Shrinkwrap, detwin and precision switch, on the
+ * solver's lane streams, from one fork to one join.  The graph is captured on the first
+ * run of a recipe from a given precision state and replayed after that.  The kernels and
+ * results are those of the step-by-step calls (qpch_group_iterate / qpch_solver_iterate,
+ * qpch_solver_shrinkwrap, save/half/restore_support), bit for bit; what goes is the launch
+ * and host round trip of every chunk and step.  Each solver needs qpch_solver_begin_run
+ * with a capacity of at least the recipe's iterations.  Replaces the reference's graph per
+ * sync_every chunk with eager stage steps in between (src/cuda/cuda_solver.cu:300-421). */
+int qpch_group_run(qpch_group* group, const qpch_op* ops, int count);
+
 /* --- operators on host arrays (synchronous) --------------------------------------------- */
 
 /* unitary n-D transform in place (complex, FFT order) */

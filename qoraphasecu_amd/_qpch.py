@@ -56,6 +56,46 @@ class QpchTuning(ctypes.Structure):
     ]
 
 
+class QpchOp(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int),
+        ("iterations", ctypes.c_int),
+        ("a", ctypes.c_float),
+        ("b", ctypes.c_float),
+        ("refine", ctypes.c_int),
+        ("threshold", ctypes.c_float),
+        ("sigma", ctypes.c_float),
+        ("method", ctypes.c_int),
+        ("axis", ctypes.c_int),
+    ]
+
+
+OP_ITERATE, OP_SHRINKWRAP, OP_DETWIN_BEGIN, OP_DETWIN_END = 0, 1, 2, 3
+
+
+def engine_ops(records):
+    """The engine's operation array for a recipe of 7-field records (kind, n, beta,
+    threshold, sigma, method, axis); update coefficients as src/core/pointwise.hpp:41-45,
+    in float32."""
+    f = np.float32
+    out = (QpchOp * len(records))()
+    for k, (kind, n, beta, threshold, sigma, method, axis) in enumerate(records):
+        o = out[k]
+        if kind in ("er", "hio", "raar"):
+            o.kind, o.iterations, o.refine = OP_ITERATE, n, int(kind == "er")
+            o.a, o.b = {"er": (0.0, 0.0), "hio": (1.0, float(-f(beta))),
+                        "raar": (float(f(beta)), float(f(1.0) - f(2.0) * f(beta)))}[kind]
+        elif kind == "shrinkwrap":
+            o.kind, o.threshold, o.sigma, o.method = OP_SHRINKWRAP, threshold, sigma, 0 if method == "max" else 1
+        elif kind == "detwin_begin":
+            o.kind, o.axis = OP_DETWIN_BEGIN, axis
+        elif kind == "detwin_end":
+            o.kind = OP_DETWIN_END
+        else:
+            raise ValueError(f"unknown recipe operation {kind!r}")
+    return out
+
+
 class QpchDeviceInfo(ctypes.Structure):
     _fields_ = [
         ("name", ctypes.c_char * 256),
@@ -107,6 +147,7 @@ def lib():
         L.qpch_group_create.argtypes = [ctypes.POINTER(vp), i, ctypes.POINTER(vp)]
         L.qpch_group_iterate.argtypes = [vp, i, f, f, i]
         L.qpch_group_synchronize.argtypes = [vp]
+        L.qpch_group_run.argtypes = [vp, ctypes.POINTER(QpchOp), i]
         L.qpch_group_destroy.argtypes = [vp]
         L.qpch_group_destroy.restype = None
         L.qpch_solver_time_passes.argtypes = [vp, i, i, ctypes.POINTER(f), i, ctypes.POINTER(i)]
@@ -375,6 +416,12 @@ class Group:
 
     def iterate(self, n, a, b, refine=False):
         check(lib().qpch_group_iterate(self._h, n, a, b, int(refine)))
+
+    def run(self, records):
+        """A whole recipe of 7-field records for every solver in one graph launch
+        (qpch_group_run); each solver needs begin_run with room for its iterations."""
+        ops = engine_ops(records)
+        check(lib().qpch_group_run(self._h, ops, len(records)))
 
     def synchronize(self):
         check(lib().qpch_group_synchronize(self._h))

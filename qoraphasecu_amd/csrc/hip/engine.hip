@@ -794,32 +794,33 @@ struct SolverImpl {
   }
 
   // --- precision switching -----------------------------------------------------------------
-  void convert_x(bool to_low) {
-    const long long total = n * starts;
+  // x between fp32 and the reduced storage, for starts [first, first + count), on stream s.
+  void convert_range(bool to_low, int first, int count, hipStream_t s) {
+    const long long total = n * count;
+    const long long off = n * first;
+    float2* x32p = x32.get() + off;
+    std::uint32_t* x16p = x16.get() + off;
     if (to_low) {
       if (storage == QPCH_F16) {
-        hipLaunchKernelGGL((dev::convert<float2, __half2>), dim3(grid_for(total)), dim3(256), 0,
-                           stream, x32.get(), reinterpret_cast<__half2*>(x16.get()), total, 1.0f,
-                           nullptr);
+        hipLaunchKernelGGL((dev::convert<float2, __half2>), dim3(grid_for(total)), dim3(256), 0, s, x32p,
+                           reinterpret_cast<__half2*>(x16p), total, 1.0f, nullptr);
       } else {
-        hipLaunchKernelGGL((dev::convert<float2, __hip_bfloat162>), dim3(grid_for(total)),
-                           dim3(256), 0, stream, x32.get(),
-                           reinterpret_cast<__hip_bfloat162*>(x16.get()), total, 1.0f, nullptr);
+        hipLaunchKernelGGL((dev::convert<float2, __hip_bfloat162>), dim3(grid_for(total)), dim3(256), 0, s, x32p,
+                           reinterpret_cast<__hip_bfloat162*>(x16p), total, 1.0f, nullptr);
       }
     } else {
       if (storage == QPCH_F16) {
-        hipLaunchKernelGGL((dev::convert<__half2, float2>), dim3(grid_for(total)), dim3(256), 0,
-                           stream, reinterpret_cast<const __half2*>(x16.get()), x32.get(), total,
-                           1.0f, nullptr);
+        hipLaunchKernelGGL((dev::convert<__half2, float2>), dim3(grid_for(total)), dim3(256), 0, s,
+                           reinterpret_cast<const __half2*>(x16p), x32p, total, 1.0f, nullptr);
       } else {
-        hipLaunchKernelGGL((dev::convert<__hip_bfloat162, float2>), dim3(grid_for(total)),
-                           dim3(256), 0, stream,
-                           reinterpret_cast<const __hip_bfloat162*>(x16.get()), x32.get(), total,
-                           1.0f, nullptr);
+        hipLaunchKernelGGL((dev::convert<__hip_bfloat162, float2>), dim3(grid_for(total)), dim3(256), 0, s,
+                           reinterpret_cast<const __hip_bfloat162*>(x16p), x32p, total, 1.0f, nullptr);
       }
     }
     QPCH_HIP(hipGetLastError());
   }
+
+  void convert_x(bool to_low) { convert_range(to_low, 0, starts, stream); }
 
   void set_precision(bool lowp) {
     if (lowp == low_active) return;
@@ -1019,16 +1020,40 @@ struct SolverImpl {
     return it->second.second.get();
   }
 
+  // Buffers and per-axis tables of a Shrinkwrap with this sigma: made before any capture
+  // (the uploads synchronise), sized for every start.
+  void shrinkwrap_prepare(float sigma) {
+    if (!(sigma >= 0.0f)) throw InvalidArgument("Shrinkwrap needs threshold > 0, sigma >= 0");
+    const long long total = n * starts;
+    if (sw_a.size() < static_cast<size_t>(total)) sw_a.resize(total);
+    if (fused && pow2) {
+      if (sw_c.size() < static_cast<size_t>(total)) sw_c.resize(total);
+      const int last = dims.nd - 1;
+      const size_t need = static_cast<size_t>(pass_tiles<8>(dims.d[last], false, static_cast<int>(dims.outer(last)), 1)) *
+                          starts * 2;
+      if (sw_part.size() < need) sw_part.resize(need);
+      for (int a = 0; a < dims.nd; ++a) (void)transfer_for(dims.d[a], sigma);
+      return;
+    }
+    if (sw_b.size() < static_cast<size_t>(total)) sw_b.resize(total);
+    for (int a = 0; a < dims.nd; ++a) (void)taps_for(dims.d[a], sigma);
+    const int blocks = grid_for(n, 256, 512);
+    if (sw_part.size() < static_cast<size_t>(blocks) * starts * 2) sw_part.resize(static_cast<size_t>(blocks) * starts * 2);
+  }
+
+  // The blur through the FFT passes (power-of-two shapes) for starts [first, first + count).
   template <class St>
-  void shrinkwrap_fft(float threshold, float sigma, int method) {
+  void shrinkwrap_fft(float threshold, float sigma, int method, int first, int count, hipStream_t s) {
     const int nd = dims.nd;
     const int last = nd - 1;
+    const long long off = n * first;
+    const int tiles = pass_tiles<8>(dims.d[last], false, static_cast<int>(dims.outer(last)), 1);
     dev::BlurArgs a{};
     a.n = n;
-    a.x = xp<St>(0);
-    a.w = sw_c.get();
-    a.out = sw_a.get();
-    a.support = supp.get();
+    a.x = xp<St>(first);
+    a.w = sw_c.get() + off;
+    a.out = sw_a.get() + off;
+    a.support = supp.get() + off;
     a.g0 = transfer_for(dims.d[0], sigma);
     a.g1 = transfer_for(dims.d[1], sigma);
     a.g2 = nd == 3 ? transfer_for(dims.d[2], sigma) : nullptr;
@@ -1036,7 +1061,7 @@ struct SolverImpl {
     a.d2 = nd == 3 ? dims.d[2] : 1;
     a.method = method;
     a.scale = static_cast<float>(1.0 / static_cast<double>(n));
-    a.partials = sw_part.get();
+    a.partials = sw_part.get() + static_cast<long long>(first) * tiles * 2;
     auto geometry = [&](int axis) {
       a.outer = static_cast<int>(dims.outer(axis));
       a.inner = static_cast<int>(dims.inner(axis));
@@ -1046,80 +1071,159 @@ struct SolverImpl {
       p.outer = static_cast<int>(dims.outer(axis));
       p.inner = static_cast<int>(dims.inner(axis));
       p.n = n;
-      p.w = sw_c.get();
+      p.w = sw_c.get() + off;
       p.out_scale = 1.0f;
-      launch_pass<float2>(dims.d[axis], p.inner > 1, mode, p, starts, stream);
+      launch_pass<float2>(dims.d[axis], p.inner > 1, mode, p, count, s);
     };
     geometry(last);
-    launch_blur<St>(dims.d[last], false, dev::Blur::kMag, a, starts, stream);
+    launch_blur<St>(dims.d[last], false, dev::Blur::kMag, a, count, s);
     if (nd == 3) line(1, Mode::kFwd);
     geometry(0);
-    launch_blur<St>(dims.d[0], true, dev::Blur::kGauss, a, starts, stream);
+    launch_blur<St>(dims.d[0], true, dev::Blur::kGauss, a, count, s);
     if (nd == 3) line(1, Mode::kInv);
     geometry(last);
-    launch_blur<St>(dims.d[last], false, dev::Blur::kReal, a, starts, stream);
-    const int tiles = pass_tiles<8>(dims.d[last], false, static_cast<int>(dims.outer(last)), 1);
-    hipLaunchKernelGGL(dev::shrinkwrap_threshold, dim3(grid_for(n, 256, 2048), starts), dim3(256), 0, stream,
-                       sw_a.get(), sw_part.get(), tiles, n, method, threshold, supp.get());
+    launch_blur<St>(dims.d[last], false, dev::Blur::kReal, a, count, s);
+    hipLaunchKernelGGL(dev::shrinkwrap_threshold, dim3(grid_for(n, 256, 2048), count), dim3(256), 0, s,
+                       sw_a.get() + off, sw_part.get() + static_cast<long long>(first) * tiles * 2, tiles, n, method,
+                       threshold, supp.get() + off);
     QPCH_HIP(hipGetLastError());
   }
 
-  void shrinkwrap(float threshold, float sigma, int method) {
+  // Shrinkwrap of starts [first, first + count) on stream s, x read in the reduced storage
+  // when lowx; shrinkwrap_prepare(sigma) must have run.
+  void shrinkwrap_range(float threshold, float sigma, int method, int first, int count, bool lowx, hipStream_t s) {
     if (!(threshold > 0.0f) || !(sigma >= 0.0f)) throw InvalidArgument("Shrinkwrap needs threshold > 0, sigma >= 0");
     if (fused && pow2) {
-      const long long total = n * starts;
-      if (sw_a.size() < static_cast<size_t>(total)) sw_a.resize(total);
-      if (sw_c.size() < static_cast<size_t>(total)) sw_c.resize(total);
-      const int last = dims.nd - 1;
-      const size_t need = static_cast<size_t>(pass_tiles<8>(dims.d[last], false, static_cast<int>(dims.outer(last)), 1)) *
-                          starts * 2;
-      if (sw_part.size() < need) sw_part.resize(need);
-      if (!low_active) {
-        shrinkwrap_fft<float2>(threshold, sigma, method);
+      if (!lowx) {
+        shrinkwrap_fft<float2>(threshold, sigma, method, first, count, s);
       } else if (storage == QPCH_F16) {
-        shrinkwrap_fft<__half2>(threshold, sigma, method);
+        shrinkwrap_fft<__half2>(threshold, sigma, method, first, count, s);
       } else {
-        shrinkwrap_fft<__hip_bfloat162>(threshold, sigma, method);
+        shrinkwrap_fft<__hip_bfloat162>(threshold, sigma, method, first, count, s);
       }
       return;
     }
-    const long long total = n * starts;
-    if (sw_a.size() < static_cast<size_t>(total)) {
-      sw_a.resize(total);
-      sw_b.resize(total);
-    }
+    const long long off = n * first;
     const int nd = dims.nd;
     float* cur = nullptr;
-    bool first = true;
+    bool first_axis = true;
     for (int a = nd - 1; a >= 0; --a) {
       const int N = dims.d[a];
       const auto& tp = taps_for(N, sigma);
       dev::BlurAxis ax{N, static_cast<int>(dims.inner(a)), static_cast<int>(tp.second.size()), tp.first,
                        tp.second.get()};
-      float* out = (cur == sw_a.get()) ? sw_b.get() : sw_a.get();
-      const dim3 grid(grid_for(n, 256, 2048), starts);
-      if (first) {
-        if (!low_active) {
-          hipLaunchKernelGGL((dev::blur_axis<float2, true>), grid, dim3(256), 0, stream, x32.get(), out, n, ax);
+      float* out = (cur == sw_a.get() + off) ? sw_b.get() + off : sw_a.get() + off;
+      const dim3 grid(grid_for(n, 256, 2048), count);
+      if (first_axis) {
+        if (!lowx) {
+          hipLaunchKernelGGL((dev::blur_axis<float2, true>), grid, dim3(256), 0, s, x32.get() + off, out, n, ax);
         } else if (storage == QPCH_F16) {
-          hipLaunchKernelGGL((dev::blur_axis<__half2, true>), grid, dim3(256), 0, stream, x16.get(), out, n, ax);
+          hipLaunchKernelGGL((dev::blur_axis<__half2, true>), grid, dim3(256), 0, s, x16.get() + off, out, n, ax);
         } else {
-          hipLaunchKernelGGL((dev::blur_axis<__hip_bfloat162, true>), grid, dim3(256), 0, stream, x16.get(), out, n, ax);
+          hipLaunchKernelGGL((dev::blur_axis<__hip_bfloat162, true>), grid, dim3(256), 0, s, x16.get() + off, out, n,
+                             ax);
         }
       } else {
-        hipLaunchKernelGGL((dev::blur_axis<float2, false>), grid, dim3(256), 0, stream, cur, out, n, ax);
+        hipLaunchKernelGGL((dev::blur_axis<float2, false>), grid, dim3(256), 0, s, cur, out, n, ax);
       }
       QPCH_HIP(hipGetLastError());
       cur = out;
-      first = false;
+      first_axis = false;
     }
     const int blocks = grid_for(n, 256, 512);
-    if (sw_part.size() < static_cast<size_t>(blocks) * starts * 2) sw_part.resize(static_cast<size_t>(blocks) * starts * 2);
-    hipLaunchKernelGGL(dev::shrinkwrap_level, dim3(blocks, starts), dim3(256), 0, stream, cur, supp.get(), n,
-                       method, sw_part.get());
-    hipLaunchKernelGGL(dev::shrinkwrap_threshold, dim3(grid_for(n, 256, 2048), starts), dim3(256), 0, stream,
-                       cur, sw_part.get(), blocks, n, method, threshold, supp.get());
+    double* part = sw_part.get() + static_cast<long long>(first) * blocks * 2;
+    hipLaunchKernelGGL(dev::shrinkwrap_level, dim3(blocks, count), dim3(256), 0, s, cur, supp.get() + off, n, method,
+                       part);
+    hipLaunchKernelGGL(dev::shrinkwrap_threshold, dim3(grid_for(n, 256, 2048), count), dim3(256), 0, s, cur, part,
+                       blocks, n, method, threshold, supp.get() + off);
     QPCH_HIP(hipGetLastError());
+  }
+
+  void shrinkwrap(float threshold, float sigma, int method) {
+    if (!(threshold > 0.0f) || !(sigma >= 0.0f)) throw InvalidArgument("Shrinkwrap needs threshold > 0, sigma >= 0");
+    shrinkwrap_prepare(sigma);
+    shrinkwrap_range(threshold, sigma, method, 0, starts, low_active, stream);
+  }
+
+  // --- detwin over a range of starts (save, cut at the centre of mass; restore) -------------
+  void detwin_prepare() {
+    if (detwin_sums.size() < static_cast<size_t>(2 * starts)) detwin_sums.resize(2 * starts);
+  }
+
+  void half_support_range(int axis, int first, int count, hipStream_t s) {
+    if (axis < 0 || axis >= dims.nd) throw InvalidArgument("detwin axis out of range");
+    const long long inner = dims.inner(axis);
+    const int len = dims.d[axis];
+    const long long off = n * first;
+    const dim3 grid(static_cast<unsigned>(std::min<long long>((n + 255) / 256, 256)), count);
+    hipLaunchKernelGGL(dev::fill_u64, dim3(1), dim3(256), 0, s, detwin_sums.get() + 2 * first, 2 * count);
+    hipLaunchKernelGGL(dev::support_moments, grid, dim3(256), 0, s, supp.get() + off, n, inner, len,
+                       detwin_sums.get() + 2 * first);
+    hipLaunchKernelGGL(dev::support_cut, grid, dim3(256), 0, s, supp.get() + off, n, inner, len,
+                       detwin_sums.get() + 2 * first);
+    QPCH_HIP(hipGetLastError());
+  }
+
+  void copy_support_range(bool save, int first, int count, hipStream_t s) {
+    const long long off = n * first;
+    std::uint8_t* dst = save ? saved.get() + off : supp.get() + off;
+    const std::uint8_t* src = save ? supp.get() + off : saved.get() + off;
+    QPCH_HIP(hipMemcpyAsync(dst, src, static_cast<size_t>(n * count), hipMemcpyDeviceToDevice, s));
+  }
+
+  // --- a whole recipe for one launch group (qpch_group_run) ---------------------------------
+  // Runs `ops` for the starts of launch group g on stream s: precision switches, prologue,
+  // iterations, Shrinkwrap and detwin, exactly the kernels the step-by-step entry points
+  // launch for those starts.  `lowp` and `valid` are the group's storage precision and wave
+  // validity, updated as the recipe goes.  Launches only: safe inside a capture, once
+  // shrinkwrap_prepare / detwin_prepare have run.
+  void sequence(int g, const qpch_op* ops, int count, hipStream_t s, bool& lowp, bool& valid) {
+    const int first = g * batch;
+    hipStream_t saved_on = on;
+    on = s;
+    try {
+      for (int k = 0; k < count; ++k) {
+        const qpch_op& op = ops[k];
+        switch (op.kind) {
+          case QPCH_OP_ITERATE: {
+            const bool want = low() && !(mixed && op.refine);
+            if (want != lowp) {
+              convert_range(want, first, batch, s);
+              lowp = want;
+              valid = false;
+            }
+            if (!want) {
+              if (!valid) fused_prologue<float2>(g);
+              fused_iterations<float2>(g, op.iterations, op.a, op.b);
+            } else if (storage == QPCH_F16) {
+              if (!valid) fused_prologue<__half2>(g);
+              fused_iterations<__half2>(g, op.iterations, op.a, op.b);
+            } else {
+              if (!valid) fused_prologue<__hip_bfloat162>(g);
+              fused_iterations<__hip_bfloat162>(g, op.iterations, op.a, op.b);
+            }
+            valid = true;
+            break;
+          }
+          case QPCH_OP_SHRINKWRAP:
+            shrinkwrap_range(op.threshold, op.sigma, op.method, first, batch, lowp, s);
+            break;
+          case QPCH_OP_DETWIN_BEGIN:
+            copy_support_range(true, first, batch, s);
+            half_support_range(op.axis, first, batch, s);
+            break;
+          case QPCH_OP_DETWIN_END:
+            copy_support_range(false, first, batch, s);
+            break;
+          default:
+            throw InvalidArgument("unknown operation kind");
+        }
+      }
+    } catch (...) {
+      on = saved_on;
+      throw;
+    }
+    on = saved_on;
   }
 
   // --- set-up ------------------------------------------------------------------------------
@@ -1344,8 +1448,20 @@ struct qpch_group {
   };
   std::map<std::tuple<int, float, float, int>, Entry> graphs;
 
+  // whole-recipe graphs (qpch_group_run), keyed by the recipe and the solvers' starting
+  // precision state
+  struct RunEntry {
+    hipGraphExec_t exec;
+    std::vector<int> generations;
+    std::vector<std::pair<bool, bool>> final_state;  // per solver: (low_active, wave_valid)
+  };
+  std::map<std::string, RunEntry> runs;
+  std::vector<hipEvent_t> branch_done;  // one per branch stream of a run
+
   ~qpch_group() {
     if (main) (void)hipStreamSynchronize(main);
+    for (auto& kv : runs) (void)hipGraphExecDestroy(kv.second.exec);
+    for (hipEvent_t e : branch_done) (void)hipEventDestroy(e);
     for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second.exec);
     for (hipEvent_t e : done) (void)hipEventDestroy(e);
     if (fork) (void)hipEventDestroy(fork);
@@ -1357,6 +1473,124 @@ struct qpch_group {
   hipEvent_t fork_ready() {
     QPCH_HIP(hipEventRecord(fork, main));
     return fork;
+  }
+
+  // The streams the branches of solver k run on: its lanes when it has them, else its own
+  // stream (launch group g goes to lane g mod L).
+  static std::vector<hipStream_t> branch_streams(qpch::SolverImpl& m) {
+    if (m.lane_streams.size() > 1) return m.lane_streams;
+    return {m.stream};
+  }
+
+  void run(const qpch_op* ops, int count) {
+    if (count < 0 || (count > 0 && !ops)) throw qpch::InvalidArgument("bad recipe");
+    long long iterations = 0;
+    bool detwin = false;
+    for (int k = 0; k < count; ++k) {
+      const qpch_op& op = ops[k];
+      if (op.kind == QPCH_OP_ITERATE) {
+        if (op.iterations < 1) throw qpch::InvalidArgument("iterations must be >= 1");
+        iterations += op.iterations;
+      } else if (op.kind == QPCH_OP_SHRINKWRAP) {
+        if (!(op.threshold > 0.0f) || !(op.sigma >= 0.0f)) {
+          throw qpch::InvalidArgument("Shrinkwrap needs threshold > 0, sigma >= 0");
+        }
+        if (op.method != 0 && op.method != 1) throw qpch::InvalidArgument("Shrinkwrap method must be 0 or 1");
+      } else if (op.kind == QPCH_OP_DETWIN_BEGIN || op.kind == QPCH_OP_DETWIN_END) {
+        detwin = true;
+      } else {
+        throw qpch::InvalidArgument("unknown operation kind");
+      }
+    }
+    if (count == 0) return;
+    // buffers and tables first: nothing inside the capture may allocate or upload
+    std::string key(reinterpret_cast<const char*>(ops), sizeof(qpch_op) * static_cast<size_t>(count));
+    std::vector<int> gens;
+    for (auto* sv : solvers) {
+      auto& m = sv->impl;
+      if (m.err_cap < iterations) throw qpch::InvalidArgument("qpch_solver_begin_run capacity is below the recipe's iterations");
+      for (int k = 0; k < count; ++k) {
+        if (ops[k].kind == QPCH_OP_SHRINKWRAP) m.shrinkwrap_prepare(ops[k].sigma);
+        if (ops[k].kind == QPCH_OP_DETWIN_BEGIN && (ops[k].axis < 0 || ops[k].axis >= m.dims.nd)) {
+          throw qpch::InvalidArgument("detwin axis out of range");
+        }
+      }
+      if (detwin) m.detwin_prepare();
+      key += m.low_active ? 'L' : 'F';
+      key += m.wave_valid ? 'v' : 'i';
+      gens.push_back(m.generation);
+    }
+    // the run starts after every solver's queued work (begin_run's resets among it)
+    for (size_t k = 0; k < solvers.size(); ++k) {
+      QPCH_HIP(hipEventRecord(done[k], solvers[k]->impl.stream));
+      QPCH_HIP(hipStreamWaitEvent(main, done[k], 0));
+    }
+    auto it = runs.find(key);
+    if (it != runs.end() && it->second.generations != gens) {
+      (void)hipGraphExecDestroy(it->second.exec);
+      runs.erase(it);
+      it = runs.end();
+    }
+    const bool eager = qpch::eager_launches();
+    std::vector<std::pair<bool, bool>> final_state;
+    if (it == runs.end() || eager) {
+      size_t branches = 0;
+      for (auto* sv : solvers) branches += branch_streams(sv->impl).size();
+      while (branch_done.size() < branches) {
+        hipEvent_t e{};
+        QPCH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        branch_done.push_back(e);
+      }
+      if (!eager) QPCH_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
+      hipGraph_t graph{};
+      try {
+        QPCH_HIP(hipEventRecord(fork, main));
+        size_t b = 0;
+        for (auto* sv : solvers) {
+          auto& m = sv->impl;
+          const auto streams = branch_streams(m);
+          for (hipStream_t st : streams) QPCH_HIP(hipStreamWaitEvent(st, fork, 0));
+          bool lowp = m.low_active, valid = m.wave_valid;
+          for (int g = 0; g < m.groups; ++g) {
+            bool l = m.low_active, v = m.wave_valid;  // every group starts from the solver's state
+            m.sequence(g, ops, count, streams[static_cast<size_t>(g) % streams.size()], l, v);
+            lowp = l;
+            valid = v;
+          }
+          final_state.emplace_back(lowp, valid);
+          for (hipStream_t st : streams) {
+            QPCH_HIP(hipEventRecord(branch_done[b], st));
+            QPCH_HIP(hipStreamWaitEvent(main, branch_done[b], 0));
+            ++b;
+          }
+        }
+      } catch (...) {
+        if (!eager) {
+          (void)hipStreamEndCapture(main, &graph);
+          if (graph) (void)hipGraphDestroy(graph);
+        }
+        throw;
+      }
+      if (!eager) {
+        QPCH_HIP(hipStreamEndCapture(main, &graph));
+        hipGraphExec_t exec{};
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        QPCH_HIP(e);
+        it = runs.emplace(key, RunEntry{exec, gens, final_state}).first;
+      }
+    }
+    if (!eager) {
+      QPCH_HIP(hipGraphLaunch(it->second.exec, main));
+      final_state = it->second.final_state;
+    }
+    QPCH_HIP(hipEventRecord(joined, main));
+    for (size_t k = 0; k < solvers.size(); ++k) {
+      auto& m = solvers[k]->impl;
+      QPCH_HIP(hipStreamWaitEvent(m.stream, joined, 0));
+      m.low_active = final_state[k].first;
+      m.wave_valid = final_state[k].second;
+    }
   }
 
   void iterate(int len, float a, float b, bool refine) {
@@ -1647,31 +1881,22 @@ int qpch_solver_set_support(qpch_solver* s, const uint8_t* host) {
 int qpch_solver_save_support(qpch_solver* s) {
   return guarded([&] {
     auto& m = s->impl;
-    QPCH_HIP(hipMemcpyAsync(m.saved.get(), m.supp.get(), m.supp.size(), hipMemcpyDeviceToDevice, m.stream));
+    m.copy_support_range(true, 0, m.starts, m.stream);
   });
 }
 
 int qpch_solver_half_support(qpch_solver* s, int axis) {
   return guarded([&] {
     auto& m = s->impl;
-    if (axis < 0 || axis >= m.dims.nd) throw qpch::InvalidArgument("detwin axis out of range");
-    if (m.detwin_sums.size() < static_cast<size_t>(2 * m.starts)) m.detwin_sums.resize(2 * m.starts);
-    const long long inner = m.dims.inner(axis);
-    const int len = m.dims.d[axis];
-    const dim3 grid(static_cast<unsigned>(std::min<long long>((m.dims.n + 255) / 256, 256)), m.starts);
-    hipLaunchKernelGGL(qpch::dev::fill_u64, dim3(1), dim3(256), 0, m.stream, m.detwin_sums.get(), 2 * m.starts);
-    hipLaunchKernelGGL(qpch::dev::support_moments, grid, dim3(256), 0, m.stream, m.supp.get(), m.dims.n, inner, len,
-                       m.detwin_sums.get());
-    hipLaunchKernelGGL(qpch::dev::support_cut, grid, dim3(256), 0, m.stream, m.supp.get(), m.dims.n, inner, len,
-                       m.detwin_sums.get());
-    QPCH_HIP(hipGetLastError());
+    m.detwin_prepare();
+    m.half_support_range(axis, 0, m.starts, m.stream);
   });
 }
 
 int qpch_solver_restore_support(qpch_solver* s) {
   return guarded([&] {
     auto& m = s->impl;
-    QPCH_HIP(hipMemcpyAsync(m.supp.get(), m.saved.get(), m.supp.size(), hipMemcpyDeviceToDevice, m.stream));
+    m.copy_support_range(false, 0, m.starts, m.stream);
   });
 }
 
@@ -1719,6 +1944,10 @@ int qpch_group_create(qpch_solver* const* solvers, int count, qpch_group** out) 
 
 int qpch_group_iterate(qpch_group* g, int iterations, float a, float b, int refine) {
   return guarded([&] { g->iterate(iterations, a, b, refine != 0); });
+}
+
+int qpch_group_run(qpch_group* g, const qpch_op* ops, int count) {
+  return guarded([&] { g->run(ops, count); });
 }
 
 int qpch_group_synchronize(qpch_group* g) {

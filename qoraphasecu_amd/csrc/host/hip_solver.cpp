@@ -10,6 +10,7 @@
 #include <string>
 #include <type_traits>
 #include <variant>
+#include <vector>
 
 #include "abi.hpp"
 #include "backends.hpp"
@@ -79,6 +80,50 @@ struct Coefficients {
   float a, b;
 };
 
+// The recipe as the engine's operation list for a whole-recipe graph (qpch_group_run).
+std::vector<qpch_op> engine_ops(const Recipe& recipe) {
+  std::vector<qpch_op> out;
+  out.reserve(recipe.ops.size());
+  auto iterate = [&](int n, Coefficients c, bool refine) {
+    qpch_op o{};
+    o.kind = QPCH_OP_ITERATE;
+    o.iterations = n;
+    o.a = c.a;
+    o.b = c.b;
+    o.refine = refine ? 1 : 0;
+    out.push_back(o);
+  };
+  for (const Op& op : recipe.ops) {
+    std::visit(
+        [&](const auto& v) {
+          using T = std::decay_t<decltype(v)>;
+          qpch_op o{};
+          if constexpr (std::is_same_v<T, ER>) {
+            iterate(v.n, {0.0f, 0.0f}, true);
+          } else if constexpr (std::is_same_v<T, HIO>) {
+            iterate(v.n, {1.0f, -v.beta}, false);
+          } else if constexpr (std::is_same_v<T, RAAR>) {
+            iterate(v.n, {v.beta, 1.0f - 2.0f * v.beta}, false);
+          } else if constexpr (std::is_same_v<T, Shrinkwrap>) {
+            o.kind = QPCH_OP_SHRINKWRAP;
+            o.threshold = v.threshold;
+            o.sigma = v.sigma;
+            o.method = v.method == Threshold::max ? 0 : 1;
+            out.push_back(o);
+          } else if constexpr (std::is_same_v<T, DetwinBegin>) {
+            o.kind = QPCH_OP_DETWIN_BEGIN;
+            o.axis = v.axis;
+            out.push_back(o);
+          } else {
+            o.kind = QPCH_OP_DETWIN_END;
+            out.push_back(o);
+          }
+        },
+        op);
+  }
+  return out;
+}
+
 class HipSolver final : public Solver {
  public:
   HipSolver(const Problem& problem, const Options& options)
@@ -114,9 +159,17 @@ class HipSolver final : public Solver {
     if (plan.schedule == QPCH_GRAPH && is_fusable(shape_)) tune(p);
     qpch_check(qpch_solver_create(&p, &h_));
     energy_ = qpch_solver_energy(h_);
+    // the batched variants run a whole recipe in one graph launch when no progress
+    // callback asks for the errors chunk by chunk
+    if (plan.batched && plan.schedule == QPCH_GRAPH && is_fusable(shape_)) {
+      qpch_check(qpch_group_create(&h_, 1, &whole_));
+    }
   }
 
-  ~HipSolver() override { qpch_solver_destroy(h_); }
+  ~HipSolver() override {
+    if (whole_) qpch_group_destroy(whole_);
+    qpch_solver_destroy(h_);
+  }
 
   HipSolver(const HipSolver&) = delete;
   HipSolver& operator=(const HipSolver&) = delete;
@@ -127,8 +180,15 @@ class HipSolver final : public Solver {
     const int capacity = recipe.iterations();
     run_iterations_ = 0;
     if (capacity > 0) qpch_check(qpch_solver_begin_run(h_, capacity));
-    for (const Op& op : recipe.ops) {
-      std::visit([this](const auto& o) { apply(o); }, op);
+    if (whole_ && !options_.progress) {
+      const auto ops = engine_ops(recipe);
+      qpch_check(qpch_group_run(whole_, ops.data(), static_cast<int>(ops.size())));
+      iteration_ += capacity;
+      run_iterations_ = capacity;
+    } else {
+      for (const Op& op : recipe.ops) {
+        std::visit([this](const auto& o) { apply(o); }, op);
+      }
     }
     if (run_iterations_ > 0) collect_errors();
     qpch_check(qpch_solver_synchronize(h_));
@@ -210,6 +270,7 @@ class HipSolver final : public Solver {
   Options options_;
   int starts_;
   qpch_solver* h_ = nullptr;
+  qpch_group* whole_ = nullptr;  // this solver alone, for whole-recipe graphs
   double energy_ = 0.0;
   std::vector<std::vector<float>> errors_;
   int iteration_ = 0;
@@ -304,8 +365,16 @@ class HipEnsemble final : public MultiSolver {
     if (capacity > 0) {
       for (auto& c : classes_) qpch_check(qpch_solver_begin_run(c.h, capacity));
     }
-    for (const Op& op : recipe.ops) {
-      std::visit([this](const auto& o) { apply(o); }, op);
+    if (!options_.progress) {
+      // the whole recipe in one graph launch: every shape's launch groups are branches
+      const auto ops = engine_ops(recipe);
+      qpch_check(qpch_group_run(group_, ops.data(), static_cast<int>(ops.size())));
+      iteration_ += capacity;
+      run_iterations_ = capacity;
+    } else {
+      for (const Op& op : recipe.ops) {
+        std::visit([this](const auto& o) { apply(o); }, op);
+      }
     }
     if (run_iterations_ > 0) collect_errors();
     qpch_check(qpch_group_synchronize(group_));

@@ -275,7 +275,9 @@ int qpch_amplitude_loss(const void* x, const void* amplitude, long long amplitud
  * before any work is queued (fp32: x of just under 4.2 million elements in all, e.g.
  * 32 x 256^2 or 1 x 128^3; fp64: half that).  Every call on one GPU uses that GPU's array: calls that overlap on
  * one GPU (two streams, two threads) race and must be ordered by the caller.  Nothing in
- * the array needs clearing between calls. */
+ * the array needs clearing between calls.  An axis whose length has a prime factor above
+ * 64 (from 256 on) also needs the transform's Bluestein tables, which one call of
+ * qpch_amplitude_loss for that shape makes; without them the call fails with QPCH_EINVAL. */
 int qpch_amplitude_loss_noalloc(const void* x, const void* amplitude, long long amplitude_stride, void* grad,
                                 double* loss, int ndim, const size_t* shape, int batch, int precision,
                                 void* stream);

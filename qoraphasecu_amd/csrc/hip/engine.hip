@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <complex>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -415,7 +416,102 @@ void launch_blur(int N, bool col, dev::Blur m, const dev::BlurArgs& a, int start
 // ---------------------------------------------------------------------------------------
 // generic (any length) transforms
 
-dev::GenericPlan generic_plan(int n) {
+// --- Bluestein tables: the chirp c_j = exp(-i pi j^2 / n) (j < n) and the transformed chirp
+// filter bhat = DFT_m(b), b_j = conj(c_|j|) for |j| < n (indices mod m), zero elsewhere;
+// evaluated on the host in double (j^2 reduced mod 2n first), rounded once to the kernel's
+// precision, uploaded once per device and length, and kept until the process ends.
+struct BluesteinTables {
+  const void* chirp = nullptr;
+  const void* bhat = nullptr;
+};
+
+int bluestein_length(int n) {
+  int m = 1;
+  while (m < 2 * n - 1) m *= 2;
+  return m;
+}
+
+// In-place forward DFT of a power-of-two length in double (radix 2, twiddles evaluated
+// directly at every use).
+void host_fft_pow2(std::vector<std::complex<double>>& a) {
+  const size_t n = a.size();
+  for (size_t i = 1, j = 0; i < n; ++i) {
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  const double pi = 3.14159265358979323846264338327950288;
+  for (size_t len = 2; len <= n; len <<= 1) {
+    for (size_t k = 0; k < len / 2; ++k) {
+      const double ang = -2.0 * pi * static_cast<double>(k) / static_cast<double>(len);
+      const std::complex<double> w(std::cos(ang), std::sin(ang));
+      for (size_t i = 0; i < n; i += len) {
+        const std::complex<double> u = a[i + k], v = a[i + k + len / 2] * w;
+        a[i + k] = u + v;
+        a[i + k + len / 2] = u - v;
+      }
+    }
+  }
+}
+
+std::mutex g_bluestein_mu;
+std::map<std::tuple<int, int, bool>, BluesteinTables> g_bluestein;  // (device, n, fp64)
+
+bool needs_bluestein(int n);
+
+const BluesteinTables* bluestein_lookup(int n, bool fp64) {
+  int dev = 0;
+  QPCH_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_bluestein_mu);
+  auto it = g_bluestein.find(std::make_tuple(dev, n, fp64));
+  return it == g_bluestein.end() ? nullptr : &it->second;
+}
+
+const BluesteinTables& bluestein_tables(int n, bool fp64) {
+  int dev = 0;
+  QPCH_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_bluestein_mu);
+  const auto key = std::make_tuple(dev, n, fp64);
+  auto it = g_bluestein.find(key);
+  if (it != g_bluestein.end()) return it->second;
+  const int m = bluestein_length(n);
+  const double pi = 3.14159265358979323846264338327950288;
+  std::vector<std::complex<double>> chirp(n), filt(m, 0.0);
+  for (int j = 0; j < n; ++j) {
+    const long long q = (static_cast<long long>(j) * j) % (2LL * n);
+    const double ang = -pi * static_cast<double>(q) / static_cast<double>(n);
+    chirp[j] = std::complex<double>(std::cos(ang), std::sin(ang));
+  }
+  filt[0] = std::conj(chirp[0]);
+  for (int j = 1; j < n; ++j) filt[j] = filt[m - j] = std::conj(chirp[j]);
+  host_fft_pow2(filt);
+  BluesteinTables t;
+  const OwnStream st;
+  auto put = [&](const std::vector<std::complex<double>>& v) -> const void* {
+    void* d = nullptr;
+    if (fp64) {
+      QPCH_HIP(hipMalloc(&d, sizeof(double2) * v.size()));
+      std::vector<double2> h(v.size());
+      for (size_t k = 0; k < v.size(); ++k) h[k] = {v[k].real(), v[k].imag()};
+      copy_sync(d, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, st.s);
+    } else {
+      QPCH_HIP(hipMalloc(&d, sizeof(float2) * v.size()));
+      std::vector<float2> h(v.size());
+      for (size_t k = 0; k < v.size(); ++k) h[k] = {static_cast<float>(v[k].real()), static_cast<float>(v[k].imag())};
+      copy_sync(d, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice, st.s);
+    }
+    return d;
+  };
+  t.chirp = put(chirp);
+  t.bhat = put(filt);
+  return g_bluestein.emplace(key, t).first->second;
+}
+
+// The plan of a generic line transform of length n in the kernel's precision; lengths with a
+// prime factor above 64 from dev::kBluesteinMinN on take Bluestein's algorithm (their
+// tables are made on first use: outside any graph capture, which solver set-up ensures).
+dev::GenericPlan generic_plan(int n, bool fp64 = false) {
   dev::GenericPlan p{};
   p.n = n;
   int rest = n, nf = 0;
@@ -435,7 +531,32 @@ dev::GenericPlan generic_plan(int n) {
     }
   }
   p.nfac = ok ? nf : 0;
+  if (p.nfac == 0 && n >= dev::kBluesteinMinN) {
+    p.m = bluestein_length(n);
+    int rest2 = p.m, k = 0;
+    while (rest2 % 4 == 0) { p.mfac[k++] = 4; rest2 /= 4; }
+    if (rest2 == 2) p.mfac[k++] = 2;
+    p.nmfac = k;
+    const BluesteinTables& t = bluestein_tables(n, fp64);
+    p.chirp = t.chirp;
+    p.bhat = t.bhat;
+  }
   return p;
+}
+
+bool needs_bluestein(int n) {
+  if (n < dev::kBluesteinMinN || n > dev::kGenericMaxN || (n & (n - 1)) == 0) return false;
+  int rest = n;
+  for (int f = 2; f <= 64 && rest > 1; ++f) {
+    while (rest % f == 0) rest /= f;
+  }
+  return rest > 1;  // a prime factor above 64 remains
+}
+
+// Dynamic LDS above the default 64 KiB is allowed up to the device's 160 KiB; a plan that
+// needed more would fail its launch with an error.
+void check_lds(size_t bytes) {
+  if (bytes > 160 * 1024) throw InvalidArgument("generic transform needs " + std::to_string(bytes) + " bytes of LDS");
 }
 
 void launch_generic(float2* data, long long n_per_start, int batch, int N, long long outer,
@@ -447,8 +568,9 @@ void launch_generic(float2* data, long long n_per_start, int batch, int N, long 
   const dev::GenericPlan plan = generic_plan(N);
   const long long lines = outer * inner;
   const long long lpb = dev::generic_lines_per_block(N);
+  check_lds(dev::generic_lds_bytes<float2>(plan));
   hipLaunchKernelGGL(dev::generic_fft<float>, dim3(static_cast<unsigned>((lines + lpb - 1) / lpb), batch),
-                     dim3(dev::kGenericThreads), dev::generic_lds_bytes<float2>(N), s, data, n_per_start,
+                     dim3(dev::kGenericThreads), dev::generic_lds_bytes<float2>(plan), s, data, n_per_start,
                      static_cast<int>(inner), lines, plan, forward ? 1 : 0, scale);
   QPCH_HIP(hipGetLastError());
 }
@@ -460,7 +582,8 @@ void launch_generic_pass(int N, Mode m, const PassArgs& a, int starts, hipStream
   const long long lines = static_cast<long long>(a.outer) * a.inner;
   const long long lpb = dev::generic_lines_per_block(N);
   const dim3 grid(static_cast<unsigned>((lines + lpb - 1) / lpb), starts);
-  const size_t lds = dev::generic_pass_lds_bytes(N);
+  const size_t lds = dev::generic_pass_lds_bytes(plan);
+  check_lds(lds);
   switch (m) {
     case Mode::kFwd: hipLaunchKernelGGL(dev::generic_pass<Mode::kFwd>, grid, dim3(dev::kGenericThreads), lds, s, a, plan, lines); break;
     case Mode::kInv: hipLaunchKernelGGL(dev::generic_pass<Mode::kInv>, grid, dim3(dev::kGenericThreads), lds, s, a, plan, lines); break;
@@ -489,11 +612,12 @@ void device_fft_f64(double2* data, const Dims& d, int batch, bool forward, hipSt
     const int N = d.d[a];
     if (N == 1) continue;
     if (N > 1024) throw InvalidArgument("the FP64 transform supports axes up to 1024");
-    const dev::GenericPlan plan = generic_plan(N);
+    const dev::GenericPlan plan = generic_plan(N, true);
     const long long lines = d.outer(a) * d.inner(a);
     const long long lpb = dev::generic_lines_per_block(N);
+    check_lds(dev::generic_lds_bytes<double2>(plan));
     hipLaunchKernelGGL(dev::generic_fft<double>, dim3(static_cast<unsigned>((lines + lpb - 1) / lpb), batch),
-                       dim3(dev::kGenericThreads), dev::generic_lds_bytes<double2>(N), s, data, d.n,
+                       dim3(dev::kGenericThreads), dev::generic_lds_bytes<double2>(plan), s, data, d.n,
                        static_cast<int>(d.inner(a)), lines, plan, forward ? 1 : 0, 1.0);
     QPCH_HIP(hipGetLastError());
   }
@@ -1264,6 +1388,10 @@ struct SolverImpl {
       }
     }
     if (!p.amplitude || !p.mask || !p.support) throw InvalidArgument("amplitude, mask and support are required");
+    // Bluestein tables of long prime axes now, so that no graph capture ever makes them
+    for (int a = 0; a < dims.nd; ++a) {
+      if (needs_bluestein(dims.d[a])) (void)bluestein_tables(dims.d[a], false);
+    }
     if (p.lanes < 0 || p.lanes > 8) throw InvalidArgument("lanes must be between 0 and 8");
     if (p.plane < 0 || p.plane > 2) throw InvalidArgument("plane must be 0, 1 or 2");
     QPCH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -2291,6 +2419,13 @@ int qpch_amplitude_loss_noalloc(const void* x, const void* amplitude, long long 
   return guarded([&] {
     const qpch::LossCall call =
         qpch::loss_call(x, amplitude, amplitude_stride, grad, loss, ndim, shape, batch, precision, stream);
+    for (int a = 0; a < call.d.nd; ++a) {
+      if (!call.plan.fused && qpch::needs_bluestein(call.d.d[a]) &&
+          !qpch::bluestein_lookup(call.d.d[a], call.precision == 1)) {
+        throw qpch::InvalidArgument("axis length " + std::to_string(call.d.d[a]) +
+                                    " needs Bluestein tables, made by a first qpch_amplitude_loss call of this shape");
+      }
+    }
     if (call.plan.bytes > kNoAllocScratchBytes) {
       throw qpch::InvalidArgument("this call needs " + std::to_string(call.plan.bytes) +
                                   " bytes of scratch; qpch_amplitude_loss_noalloc holds " +

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "fft_engine.hpp"
 #include "passes.hpp"
@@ -22,9 +23,21 @@ static_assert(kGenericMaxN <= 2048 && 2048 % kGenericThreads == 0, "generic_pass
 
 struct GenericPlan {
   int n;
-  int nfac;  // 0: one plain DFT of the whole line
+  int nfac;  // 0: one plain DFT of the whole line (or Bluestein when m > 0)
   int fac[32];
+  // Bluestein (chirp-z) for lengths with a prime factor above 64 from kBluesteinMinN on:
+  // the line times the chirp c_j = exp(-i pi j^2 / n), zero-padded to m = 2^k >= 2n - 1,
+  // transformed (radix-4/2 stages, factors mfac), multiplied by the transformed chirp
+  // filter bhat, transformed back and multiplied by the chirp again.  The inverse
+  // transform conjugates both tables.  m = 0: not used.
+  int m;
+  int nmfac;
+  int mfac[8];
+  const void* chirp;  // n values (float2 or double2, the kernel's precision)
+  const void* bhat;   // m values
 };
+
+constexpr int kBluesteinMinN = 256;
 
 template <class Real>
 struct Cplx;
@@ -52,11 +65,15 @@ __host__ __device__ constexpr int generic_lines_per_block(int N) {
   return N >= 1024 ? 1 : (1024 / N < 16 ? 1024 / N : 16);
 }
 
-// Dynamic LDS of generic_fft: the W_N table, two line buffers per line, the line bases.
+// Elements of one line slot in LDS, and of the twiddle table: the padded length m of a
+// Bluestein plan, else the line length.
+__host__ __device__ constexpr int generic_slot(const GenericPlan& plan) { return plan.m > 0 ? plan.m : plan.n; }
+
+// Dynamic LDS of generic_fft: the twiddle table, two line buffers per line, the line bases.
 template <class C>
-__host__ __device__ constexpr size_t generic_lds_bytes(int N) {
-  const int lpb = generic_lines_per_block(N);
-  return sizeof(C) * static_cast<size_t>(N) * (1 + 2 * lpb) + sizeof(long long) * lpb;
+__host__ __device__ constexpr size_t generic_lds_bytes(const GenericPlan& plan) {
+  const int lpb = generic_lines_per_block(plan.n);
+  return sizeof(C) * static_cast<size_t>(generic_slot(plan)) * (1 + 2 * lpb) + sizeof(long long) * lpb;
 }
 
 // a / b for 0 <= a < 2^20 and 1 <= b: the quotient from the float reciprocal is off by at
@@ -131,12 +148,71 @@ __device__ __forceinline__ void generic_stage(const C* src, C* dst, const C* tw,
   }
 }
 
+// The unnormalised length-n DFT of the nl lines in src (line l at l*n) by Bluestein's chirp-z
+// convolution (plan.m > 0).  Both buffers hold nl slots of m elements; tw is the forward
+// W_m table.  The result is in src on return (line l at l*n).  Ends with a barrier.
+template <bool INV, class C>
+__device__ __forceinline__ void bluestein(C*& src, C*& dst, const C* tw, const GenericPlan& plan, int nl) {
+  const int N = plan.n, Mm = plan.m;
+  const C* chirp = static_cast<const C*>(plan.chirp);
+  const C* bhat = static_cast<const C*>(plan.bhat);
+  // a_j = x_j c_j, zero-padded to m
+  for (int t = threadIdx.x; t < nl * Mm; t += blockDim.x) {
+    const int l = small_div(t, Mm);
+    const int j = t - l * Mm;
+    dst[t] = j < N ? gmul(src[l * N + j], twid<INV>(chirp, j)) : C{0, 0};
+  }
+  __syncthreads();
+  C* cur = dst;
+  C* oth = src;
+  auto stages = [&](auto inverse) {
+    constexpr bool kInv = decltype(inverse)::value;
+    int L = 1;
+    for (int s = 0; s < plan.nmfac; ++s) {
+      const int r = plan.mfac[s];
+      if (r == 4) {
+        generic_stage<4, kInv>(cur, oth, tw, Mm, r, L, nl);
+      } else {
+        generic_stage<2, kInv>(cur, oth, tw, Mm, r, L, nl);
+      }
+      __syncthreads();
+      C* t = cur;
+      cur = oth;
+      oth = t;
+      L *= r;
+    }
+  };
+  stages(std::false_type{});
+  for (int t = threadIdx.x; t < nl * Mm; t += blockDim.x) {
+    const int j = t - small_div(t, Mm) * Mm;
+    cur[t] = gmul(cur[t], twid<INV>(bhat, j));
+  }
+  __syncthreads();
+  stages(std::true_type{});
+  // X_k = c_k conv_k / m (1/m is a power of two: exact)
+  using R = decltype(C{}.x);
+  const R inv_m = R(1) / static_cast<R>(Mm);
+  for (int t = threadIdx.x; t < nl * N; t += blockDim.x) {
+    const int l = small_div(t, N);
+    const int k = t - l * N;
+    const C v = gmul(cur[l * Mm + k], twid<INV>(chirp, k));
+    oth[t] = {v.x * inv_m, v.y * inv_m};
+  }
+  __syncthreads();
+  src = oth;
+  dst = cur;
+}
+
 // The unnormalised DFT of the nl lines in src (LDS, line l at l*N), by the plan's stages or,
 // for nfac == 0, one plain DFT per line; src and dst swap roles once per stage, so the
 // result is in src on return.  Ends with a barrier.
 template <bool INV, class C>
 __device__ __forceinline__ void generic_transform(C*& src, C*& dst, const C* tw, const GenericPlan& plan, int nl) {
   const int N = plan.n;
+  if (plan.m > 0) {
+    bluestein<INV>(src, dst, tw, plan, nl);
+    return;
+  }
   if (plan.nfac == 0) {
     for (int t = threadIdx.x; t < nl * N; t += blockDim.x) {
       const int l = small_div(t, N);
@@ -192,14 +268,18 @@ __global__ void __launch_bounds__(kGenericThreads)
   C* sm = reinterpret_cast<C*>(sm_raw);
   const int N = plan.n;
   const int lpb = generic_lines_per_block(N);
+  const int slot = generic_slot(plan);
   C* tw = sm;
-  C* bufa = sm + N;
-  C* bufb = bufa + lpb * N;
-  long long* lbase = reinterpret_cast<long long*>(bufb + lpb * N);
-  for (int j = threadIdx.x; j < N; j += blockDim.x) {
+  C* bufa = sm + slot;
+  C* bufb = bufa + lpb * slot;
+  long long* lbase = reinterpret_cast<long long*>(bufb + lpb * slot);
+  // W_N in the transform's direction, or the forward W_m of a Bluestein plan (whose
+  // direction is the template argument of its stages)
+  const bool table_forward = forward || plan.m > 0;
+  for (int j = threadIdx.x; j < slot; j += blockDim.x) {
     double s, c;
-    sincospi(2.0 * static_cast<double>(j) / static_cast<double>(N), &s, &c);
-    tw[j] = {static_cast<Real>(c), static_cast<Real>(forward ? -s : s)};
+    sincospi(2.0 * static_cast<double>(j) / static_cast<double>(slot), &s, &c);
+    tw[j] = {static_cast<Real>(c), static_cast<Real>(table_forward ? -s : s)};
   }
   const long long line0 = static_cast<long long>(blockIdx.x) * lpb;
   const int nl = static_cast<int>(lines - line0 < lpb ? lines - line0 : lpb);
@@ -217,7 +297,11 @@ __global__ void __launch_bounds__(kGenericThreads)
   __syncthreads();
   C* src = bufa;
   C* dst = bufb;
-  generic_transform<false>(src, dst, tw, plan, nl);  // the table carries the direction
+  if (plan.m > 0 && !forward) {
+    generic_transform<true>(src, dst, tw, plan, nl);
+  } else {
+    generic_transform<false>(src, dst, tw, plan, nl);  // otherwise the table carries the direction
+  }
   for (int e = threadIdx.x; e < nl * N; e += blockDim.x) {
     const int l = strided ? e - small_div(e, nl) * nl : small_div(e, N);
     const int k = strided ? small_div(e, nl) : e - small_div(e, N) * N;
@@ -226,10 +310,10 @@ __global__ void __launch_bounds__(kGenericThreads)
   }
 }
 
-// Dynamic LDS of generic_pass: the forward W_N table, two line buffers per line, the line
-// bases and one FP64 slot per wave.
-__host__ __device__ constexpr size_t generic_pass_lds_bytes(int N) {
-  return generic_lds_bytes<float2>(N) + sizeof(double) * (kGenericThreads / 64);
+// Dynamic LDS of generic_pass: the forward twiddle table, two line buffers per line, the
+// line bases and one FP64 slot per wave.
+__host__ __device__ constexpr size_t generic_pass_lds_bytes(const GenericPlan& plan) {
+  return generic_lds_bytes<float2>(plan) + sizeof(double) * (kGenericThreads / 64);
 }
 
 // The fused passes of line_pass (passes.hpp) for an axis of any length up to
@@ -245,10 +329,11 @@ __global__ void __launch_bounds__(kGenericThreads)
   float2* sm = reinterpret_cast<float2*>(sm_raw);
   const int N = plan.n;
   const int lpb = generic_lines_per_block(N);
+  const int slot = generic_slot(plan);
   float2* tw = sm;
-  float2* bufa = sm + N;
-  float2* bufb = bufa + lpb * N;
-  long long* lbase = reinterpret_cast<long long*>(bufb + lpb * N);
+  float2* bufa = sm + slot;
+  float2* bufb = bufa + lpb * slot;
+  long long* lbase = reinterpret_cast<long long*>(bufb + lpb * slot);
   double* red = reinterpret_cast<double*>(lbase + lpb);
   const int start = blockIdx.y;
   const long long soff = static_cast<long long>(start) * p.n;
@@ -301,11 +386,12 @@ __global__ void __launch_bounds__(kGenericThreads)
       inside[u] = sup[gi[u]] != 0;
     }
   }
-  // the twiddle table while the loads are in flight
-  for (int j = threadIdx.x; j < N; j += blockDim.x) {
+  // the twiddle table while the loads are in flight (forward W_N^j, or W_m^j of a
+  // Bluestein plan)
+  for (int j = threadIdx.x; j < slot; j += blockDim.x) {
     double sn, cs;
-    sincospi(2.0 * static_cast<double>(j) / static_cast<double>(N), &sn, &cs);
-    tw[j] = {static_cast<float>(cs), static_cast<float>(-sn)};  // forward W_N^j
+    sincospi(2.0 * static_cast<double>(j) / static_cast<double>(slot), &sn, &cs);
+    tw[j] = {static_cast<float>(cs), static_cast<float>(-sn)};
   }
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {

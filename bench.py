@@ -315,8 +315,9 @@ def cpu_full_run(w, threads):
 def time_next_rows(steps):
     """The rows after the hot path: a non-power-of-two problem (240^2 = 2^4 3 5, 8 starts,
     RAAR(50)) through the generic mixed-radix kernels, with the oracle's rate on the same
-    problem beside it; and the device make_problem (autocorrelation support) and
-    aligned_error at 256^2."""
+    problem beside it; a prime row length (64 x 1021, 8 starts, RAAR(50): Bluestein's
+    transform in the generic passes); and the device make_problem (autocorrelation support)
+    and aligned_error at 256^2."""
     import qoraphasecu_amd as qpc
     from qoraphasecu_amd import _core, workloads
 
@@ -333,6 +334,19 @@ def time_next_rows(steps):
         res[variant] = dict(ms_per_run=wall * 1e3, start_iterations_per_s=8 * 50 / wall)
     log(f"  240^2 (generic radix 2/3/5): graph {res['graph']['start_iterations_per_s']:.4g} start-it/s, "
         f"batched {res['batched']['start_iterations_per_s']:.4g}")
+    # a prime row axis (1021: Bluestein's transform in the generic passes)
+    obj3, inten3, mask3 = workloads.problem_2d(64, seed=1001, photons=1e7, beamstop_radius=4, box=24)
+    inten3 = np.pad(inten3, ((0, 0), (478, 479)))  # 64 x 1021, the measured spectrum in the middle
+    mask3 = np.pad(mask3, ((0, 0), (478, 479)))
+    s = _core.Solver(inten3, None, mask3, 8, 0, 0, None, 0.1, "hip", "batched", 10)
+    s.run(ops)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.run(ops)
+    wall = (time.perf_counter() - t0) / steps
+    res["prime_64x1021"] = dict(ms_per_run=wall * 1e3, start_iterations_per_s=8 * 50 / wall)
+    log(f"  64 x 1021 (Bluestein rows), 8 starts, RAAR(50): {res['prime_64x1021']['start_iterations_per_s']:.4g} "
+        f"start-it/s")
     obj2, inten2, mask2 = workloads.problem_2d(256, seed=1000, photons=1e7, beamstop_radius=6)
     _core.make_problem(inten2, None, mask2, 0.1)
     t0 = time.perf_counter()

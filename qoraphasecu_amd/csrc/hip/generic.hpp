@@ -94,11 +94,70 @@ __device__ __forceinline__ C twid(const C* tw, int j) {
   return INV ? C{w.x, -w.y} : w;
 }
 
+// a * (-i) for a forward transform, a * (+i) for an inverse one (exact)
+template <bool INV, class C>
+__device__ __forceinline__ C rot(C a) {
+  return INV ? C{-a.y, a.x} : C{a.y, -a.x};
+}
+
+// In-register DFT of size R in {2, 3, 4, 5} with W_R = exp(-+ 2 pi i / R), natural order in
+// and out.  Radix 3 and 5 use the textbook short forms: with s = x1 + x2, d = x1 - x2,
+//   X0 = x0 + s,  X1,2 = (x0 - s/2) -+ i sin(2pi/3) d;
+// and with s1 = x1 + x4, d1 = x1 - x4, s2 = x2 + x3, d2 = x2 - x3,
+//   X0 = x0 + s1 + s2,
+//   X1,4 = x0 + cos(2pi/5) s1 + cos(4pi/5) s2 -+ i (sin(2pi/5) d1 + sin(4pi/5) d2),
+//   X2,3 = x0 + cos(4pi/5) s1 + cos(2pi/5) s2 -+ i (sin(4pi/5) d1 - sin(2pi/5) d2)
+// (upper signs forward).  Four and two are the split radix-2 butterflies.
+template <int R, bool INV, class C>
+__device__ __forceinline__ void dft_small(C* v) {
+  using Re = decltype(C{}.x);
+  if constexpr (R == 2) {
+    const C a = v[0], b = v[1];
+    v[0] = {a.x + b.x, a.y + b.y};
+    v[1] = {a.x - b.x, a.y - b.y};
+  } else if constexpr (R == 4) {
+    const C s0 = {v[0].x + v[2].x, v[0].y + v[2].y}, d0 = {v[0].x - v[2].x, v[0].y - v[2].y};
+    const C s1 = {v[1].x + v[3].x, v[1].y + v[3].y}, d1 = rot<INV>(C{v[1].x - v[3].x, v[1].y - v[3].y});
+    v[0] = {s0.x + s1.x, s0.y + s1.y};
+    v[2] = {s0.x - s1.x, s0.y - s1.y};
+    v[1] = {d0.x + d1.x, d0.y + d1.y};
+    v[3] = {d0.x - d1.x, d0.y - d1.y};
+  } else if constexpr (R == 3) {
+    constexpr Re kS = Re(0.86602540378443864676);  // sin(2 pi / 3)
+    const C a = v[0];
+    const C s = {v[1].x + v[2].x, v[1].y + v[2].y};
+    const C d = rot<INV>(C{kS * (v[1].x - v[2].x), kS * (v[1].y - v[2].y)});
+    const C t = {a.x - Re(0.5) * s.x, a.y - Re(0.5) * s.y};
+    v[0] = {a.x + s.x, a.y + s.y};
+    v[1] = {t.x + d.x, t.y + d.y};
+    v[2] = {t.x - d.x, t.y - d.y};
+  } else {
+    static_assert(R == 5, "short forms exist for radix 2, 3, 4 and 5");
+    constexpr Re kC1 = Re(0.30901699437494742410);   // cos(2 pi / 5)
+    constexpr Re kC2 = Re(-0.80901699437494742410);  // cos(4 pi / 5)
+    constexpr Re kS1 = Re(0.95105651629515357212);   // sin(2 pi / 5)
+    constexpr Re kS2 = Re(0.58778525229247312917);   // sin(4 pi / 5)
+    const C a = v[0];
+    const C s1 = {v[1].x + v[4].x, v[1].y + v[4].y}, d1 = {v[1].x - v[4].x, v[1].y - v[4].y};
+    const C s2 = {v[2].x + v[3].x, v[2].y + v[3].y}, d2 = {v[2].x - v[3].x, v[2].y - v[3].y};
+    const C t1 = {a.x + kC1 * s1.x + kC2 * s2.x, a.y + kC1 * s1.y + kC2 * s2.y};
+    const C t2 = {a.x + kC2 * s1.x + kC1 * s2.x, a.y + kC2 * s1.y + kC1 * s2.y};
+    const C u1 = rot<INV>(C{kS1 * d1.x + kS2 * d2.x, kS1 * d1.y + kS2 * d2.y});
+    const C u2 = rot<INV>(C{kS2 * d1.x - kS1 * d2.x, kS2 * d1.y - kS1 * d2.y});
+    v[0] = {a.x + s1.x + s2.x, a.y + s1.y + s2.y};
+    v[1] = {t1.x + u1.x, t1.y + u1.y};
+    v[4] = {t1.x - u1.x, t1.y - u1.y};
+    v[2] = {t2.x + u2.x, t2.y + u2.y};
+    v[3] = {t2.x - u2.x, t2.y - u2.y};
+  }
+}
+
 // One autosort stage of radix r over the nl lines of a block.  Output j of butterfly t
 // (k = t mod L) is  sum_q W_N^(q k m) W_r^(j q) src[t + q N/r]  (m = N / (L r)), written to
-// dst[(t - k) r + k + j L].  R > 0 fixes the radix at compile time (inputs held in
-// registers, W_r indices folded); R = 0 takes r at run time and advances (j q) mod r by j
-// per term instead of dividing.  INV conjugates the table's twiddles.
+// dst[(t - k) r + k + j L].  R in {2, 3, 4, 5} fixes the radix at compile time (inputs
+// held in registers, the short forms of dft_small); R = 0 takes r at run time and sums the
+// O(r^2) terms with W_r from the table, advancing (j q) mod r by j per term instead of
+// dividing.  INV conjugates the table's twiddles.
 template <int R, bool INV, class C>
 __device__ __forceinline__ void generic_stage(const C* src, C* dst, const C* tw, int N, int r, int L,
                                               int nl) {
@@ -120,16 +179,9 @@ __device__ __forceinline__ void generic_stage(const C* src, C* dst, const C* tw,
         const int tq = q * k * m;  // stage twiddle W_{L r}^{q k}
         if (tq) v[q] = gmul(v[q], twid<INV>(tw, tq));
       }
+      dft_small<R, INV>(v);
 #pragma unroll
-      for (int j = 0; j < R; ++j) {
-        C acc = {0, 0};
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-          const int e = (j * q) % R;  // W_r^{j q} = W_N^{(j q mod r) N / r}
-          acc = gadd(acc, e ? gmul(v[q], twid<INV>(tw, e * nr)) : v[q]);
-        }
-        dl[outb + j * L] = acc;
-      }
+      for (int j = 0; j < R; ++j) dl[outb + j * L] = v[j];
     } else {
       for (int j = 0; j < rr; ++j) {
         C acc = {0, 0};
@@ -273,13 +325,12 @@ __global__ void __launch_bounds__(kGenericThreads)
   C* bufa = sm + slot;
   C* bufb = bufa + lpb * slot;
   long long* lbase = reinterpret_cast<long long*>(bufb + lpb * slot);
-  // W_N in the transform's direction, or the forward W_m of a Bluestein plan (whose
-  // direction is the template argument of its stages)
-  const bool table_forward = forward || plan.m > 0;
+  // the forward table (W_N, or W_m of a Bluestein plan); an inverse transform conjugates
+  // its entries as it reads them (the stages' template argument carries the direction)
   for (int j = threadIdx.x; j < slot; j += blockDim.x) {
     double s, c;
     sincospi(2.0 * static_cast<double>(j) / static_cast<double>(slot), &s, &c);
-    tw[j] = {static_cast<Real>(c), static_cast<Real>(table_forward ? -s : s)};
+    tw[j] = {static_cast<Real>(c), static_cast<Real>(-s)};
   }
   const long long line0 = static_cast<long long>(blockIdx.x) * lpb;
   const int nl = static_cast<int>(lines - line0 < lpb ? lines - line0 : lpb);
@@ -297,10 +348,10 @@ __global__ void __launch_bounds__(kGenericThreads)
   __syncthreads();
   C* src = bufa;
   C* dst = bufb;
-  if (plan.m > 0 && !forward) {
-    generic_transform<true>(src, dst, tw, plan, nl);
+  if (forward) {
+    generic_transform<false>(src, dst, tw, plan, nl);
   } else {
-    generic_transform<false>(src, dst, tw, plan, nl);  // otherwise the table carries the direction
+    generic_transform<true>(src, dst, tw, plan, nl);
   }
   for (int e = threadIdx.x; e < nl * N; e += blockDim.x) {
     const int l = strided ? e - small_div(e, nl) * nl : small_div(e, N);
@@ -318,10 +369,10 @@ __host__ __device__ constexpr size_t generic_pass_lds_bytes(const GenericPlan& p
 
 // The fused passes of line_pass (passes.hpp) for an axis of any length up to
 // kGenericMaxN, fp32 storage: the same modes, arguments and scale conventions, with
-// generic_lines_per_block(N) lines of one start per work-group (blockIdx.y = start of the
-// launch).  The modulus pass leaves one FP64 error partial per work-group (its waves'
-// sums added in order); the update pass's work-group 0 of each start sums the start's
-// partial_stride partials in order, as line_pass does.
+// generic_lines_per_block(N) lines of one start per work-group (a tile; tiles and starts
+// numbered as below).  The modulus pass leaves one FP64 error partial per work-group (its
+// waves' sums added in order); the update pass's tile-0 work-group of each start sums the
+// start's partial_stride partials in order, as line_pass does.
 template <Mode M>
 __global__ void __launch_bounds__(kGenericThreads)
     generic_pass(PassArgs p, GenericPlan plan, long long lines) {
@@ -335,9 +386,21 @@ __global__ void __launch_bounds__(kGenericThreads)
   float2* bufb = bufa + lpb * slot;
   long long* lbase = reinterpret_cast<long long*>(bufb + lpb * slot);
   double* red = reinterpret_cast<double*>(lbase + lpb);
-  const int start = blockIdx.y;
+  // Block numbering: with p.slots > 0 (a 1D grid, slots a multiple of 8) block B runs tile
+  // B / slots of start B % slots, so all of a start's blocks share one XCD's L2 (a strided
+  // axis's blocks read adjacent pieces of the same 128-byte lines); otherwise the grid is
+  // (tiles, starts).
+  int tile, start;
+  if (p.slots > 0) {
+    start = blockIdx.x % p.slots;
+    tile = blockIdx.x / p.slots;
+    if (start >= p.starts) return;  // padding slot: the whole block leaves together
+  } else {
+    start = blockIdx.y;
+    tile = blockIdx.x;
+  }
   const long long soff = static_cast<long long>(start) * p.n;
-  const long long line0 = static_cast<long long>(blockIdx.x) * lpb;
+  const long long line0 = static_cast<long long>(tile) * lpb;
   const int nl = static_cast<int>(lines - line0 < lpb ? lines - line0 : lpb);
   const int inner = p.inner;
   if (static_cast<int>(threadIdx.x) < nl) {
@@ -417,12 +480,12 @@ __global__ void __launch_bounds__(kGenericThreads)
     }
     const double ws = wave_sum(acc);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ws;
-    if (threadIdx.x == 0 && blockIdx.x == 0 && start == 0 && p.counter) atomicAdd(p.counter, 1);
+    if (threadIdx.x == 0 && tile == 0 && start == 0 && p.counter) atomicAdd(p.counter, 1);
     __syncthreads();
     if (threadIdx.x == 0) {
       double t = 0.0;
       for (int v = 0; v < static_cast<int>(blockDim.x) / 64; ++v) t += red[v];
-      p.partials[static_cast<long long>(start) * p.partial_stride + blockIdx.x] = t;
+      p.partials[static_cast<long long>(start) * p.partial_stride + tile] = t;
     }
     generic_transform<true>(src, dst, tw, plan, nl);
     out_scale = 1.0f;  // folded into the projection
@@ -441,7 +504,7 @@ __global__ void __launch_bounds__(kGenericThreads)
     }
     __syncthreads();
     generic_transform<false>(src, dst, tw, plan, nl);
-    if (blockIdx.x == 0 && threadIdx.x < 64 && p.errs) {
+    if (tile == 0 && threadIdx.x < 64 && p.errs) {
       const double t = partial_sum(p.partials + static_cast<long long>(start) * p.partial_stride, p.partial_stride,
                                    threadIdx.x);
       if (threadIdx.x == 0) {

@@ -297,19 +297,31 @@ RANDOM_SHAPES = [(9, 48), (12, 12), (100, 67), (100, 128), (80, 4), (100, 16), (
 def test_random_shapes_against_oracle(qpch, shape):
     """Mixed axis lengths (generic passes beside power-of-two line passes, a prime axis of
     67 as one plain DFT per line, axes of 2 and 4): one RAAR and one ER iteration from the
-    same start against the oracle within 1e-5 (plus the absolute rounding floor of
-    test_line_lengths_2_to_4096 for the small ones), on the graph and fused schedules."""
+    same start against the oracle within 1e-5, on the graph and fused schedules.  Where ER
+    lands near the float floor (errors of 1e-4 to 1e-3 at (80, 4), (10, 7, 24), (128, 5,
+    27), (32, 49, 2)), an absolute allowance is added: the larger of 3e-7 (the rounding
+    floor of test_line_lengths_2_to_4096) and 3 x the oracle's own float-vs-FP64 distance on
+    the same iteration, the factor of test_gpu_per_iteration.py's yardstick.  At (10, 7, 24)
+    that distance is 1.8e-7 on the ER iteration and the device lands 2.0e-7 from FP64, on
+    the other side of it."""
     obj, amp, mask, sup = _problem(shape, max(1, min(shape) // 4), masked=True)
     start = np.stack([synthetic.random_field(obj.size, 51).reshape(shape)])
     f = oracle.to_fft_order
     ops = [R.raar(1), R.er(1)]
-    o = oracle.Solver(amp, mask, sup, starts=1, start=start)
-    o.run(ops)
-    ores = o.results()[0]
+    res = {}
+    for prec in ("f", "d"):
+        o = oracle.Solver(amp, mask, sup, starts=1, start=start, prec=prec)
+        o.run(ops)
+        res[prec] = o.results()[0]
+    ores = res["f"]
+    want = np.asarray(ores[2], np.float64)
+    drift = np.abs(want - np.asarray(res["d"][2], np.float64))
+    floor = np.maximum(3e-7, 3.0 * drift)
     for schedule in ("graph", "fused"):
         s = qpch.Solver(f(amp), f(mask), f(sup), starts=1, start=np.stack([f(start[0])]), schedule=schedule)
-        errs = R.run_abi(s, ops)[:, 0]
+        errs = R.run_abi(s, ops)[:, 0].astype(np.float64)
         x = s.read_state()[0]
         s.close()
-        assert np.all(np.abs(errs - ores[2]) <= 1e-5 * ores[2] + 3e-7), (schedule, errs, ores[2])
+        print(f"{shape} {schedule}: device vs float oracle {np.abs(errs - want)}, float oracle vs FP64 {drift}")
+        assert np.all(np.abs(errs - want) <= 1e-5 * want + floor), (schedule, errs, want, floor)
         assert rel(x, f(ores[0])) < 1e-5, (schedule, rel(x, f(ores[0])))

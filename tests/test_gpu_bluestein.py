@@ -73,7 +73,7 @@ def test_bluestein_fp64_adjoint_matches_torch():
     xr = x.detach().cpu().requires_grad_(True)
     ref = ((torch.fft.fftn(xr, dim=(1, 2), norm="ortho").abs() - amp.cpu()) ** 2).sum()
     ref.backward()
-    assert abs(float(loss) - float(ref)) <= 1e-10 * float(ref)
+    assert abs(float(loss.detach()) - float(ref.detach())) <= 1e-10 * float(ref.detach())
     assert float((x.grad.cpu() - xr.grad).abs().max() / xr.grad.abs().max()) < 1e-10
 
 

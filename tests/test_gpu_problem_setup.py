@@ -138,6 +138,25 @@ def test_aligned_error_matches_oracle(shift, angle, twin, noise):
         assert got <= 1e-6
 
 
+@pytest.mark.parametrize("shape,shift,twin", [((48, 40), (7, -3), False), ((48, 40), (-5, 9), True),
+                                         ((12, 20, 18), (1, -4, 6), True)])
+def test_aligned_error_generic_shapes_match_oracle(shape, shift, twin):
+    """Axes that are not powers of two (the generic transforms), a random complex object with
+    a compact support, noise on top: aligned_error against the FP64 oracle's."""
+    import qoraphasecu_amd as qpc
+
+    rng = np.random.default_rng(sum(shape))
+    obj = np.zeros(shape, np.complex64)
+    inner = tuple(slice(s // 4, s // 4 + s // 2) for s in shape)
+    obj[inner] = (rng.uniform(0.3, 1.0, obj[inner].shape) * np.exp(1j * rng.uniform(-np.pi, np.pi, obj[inner].shape)))
+    a = _moved(obj, shift, 0.9, twin)
+    a = (a + 0.02 * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape))).astype(np.complex64)
+    got = qpc.aligned_error(a, obj)
+    want = oracle.aligned_error(a.astype(np.complex128), obj.astype(np.complex128), "d")
+    print(f"{shape} shift {shift} twin {twin}: device {got:.7e}, FP64 oracle {want:.7e}")
+    assert abs(got - want) <= 1e-6, (got, want)
+
+
 def test_aligned_error_3d_matches_oracle():
     import qoraphasecu_amd as qpc
     from qoraphasecu_amd import workloads

@@ -42,6 +42,7 @@ class QpchProblem(ctypes.Structure):
         ("xcd", ctypes.c_int),
         ("lanes", ctypes.c_int),
         ("radix", ctypes.c_int),
+        ("persist", ctypes.c_int),
     ]
 
 
@@ -288,7 +289,8 @@ class Solver:
     """A device solver over FFT-order inputs (what the C++ HipSolver drives)."""
 
     def __init__(self, amplitude, mask, support, starts=1, batch=0, seed=0, start=None,
-                 storage="f32", mixed=False, schedule="graph", problems=1, plane=0, xcd=0, lanes=1, radix=0, tuned=False):
+                 storage="f32", mixed=False, schedule="graph", problems=1, plane=0, xcd=0, lanes=1, radix=0, tuned=False,
+                 persist=0):
         """amplitude / mask / support: one problem's FFT-order arrays, or `problems` of them
         stacked along a leading axis; `starts` per problem.  tuned: take the launch choices
         the autotuner makes for this problem (qpch_autotune) instead of the arguments."""
@@ -313,6 +315,7 @@ class Solver:
         p.xcd = xcd
         p.lanes = lanes
         p.radix = radix
+        p.persist = persist
         p.amplitude = _ptr(self._keep[0])
         p.mask = _ptr(self._keep[1])
         p.support = _ptr(self._keep[2])

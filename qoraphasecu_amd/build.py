@@ -62,10 +62,21 @@ def core_ext() -> str:
     return os.path.join(HERE, "_core" + sysconfig.get_config_var("EXT_SUFFIX"))
 
 
+ENGINE_SOURCES = ["engine.hip", "twopass.hip"]
+
+
 def build_engine() -> str:
+    """Each HIP source compiles to its own object (in parallel), then one shared library."""
     out = engine_lib()
     tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", os.path.join(CSRC, "hip", "engine.hip"), "-o", tmp])
+    objs = [os.path.join(HERE, "_build", os.path.splitext(s)[0] + ".o") for s in ENGINE_SOURCES]
+    os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
+    with ThreadPoolExecutor(max_workers=len(ENGINE_SOURCES)) as pool:
+        jobs = [pool.submit(_run, [HIPCC, *HIP_FLAGS, "-c", f"-I{INCLUDE}", os.path.join(CSRC, "hip", src), "-o", obj])
+                for src, obj in zip(ENGINE_SOURCES, objs)]
+        for j in jobs:
+            j.result()
+    _run([HIPCC, *HIP_FLAGS, "-shared", *objs, "-o", tmp])
     os.replace(tmp, out)
     return out
 

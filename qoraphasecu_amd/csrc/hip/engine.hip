@@ -26,6 +26,7 @@
 #include "misc.hpp"
 #include "passes.hpp"
 #include "plane.hpp"
+#include "twopass.hpp"
 #include "blur.hpp"
 
 namespace qpch {
@@ -792,6 +793,12 @@ struct SolverImpl {
   bool resident = false;  // 2D fp32 iterations run whole chunks in image_iterations
   int xcd_mode = 0;  // 0 XCD-affine by start, 2 plain numbering, 3 plain for the modulus pass only
   int radix = 16;        // FFT radix of the line passes (8 or 16)
+  bool persist = false;  // 2D chunks of iterations in one persistent launch where a kernel exists
+  DevBuf<int> psync;  // a line holding the abort flag, then per launch group: next[8 lines], done, count[2][batch]
+  long long psync_group() const {
+    return static_cast<long long>(dev::kXcds + 1 + 2 * batch) * dev::kLineInts;
+  }
+  long long wait_ticks = 0;  // bound of one persistent wait: half a second of wall_clock64
 
   std::vector<PassSpec> iteration_passes(bool lowp) const {
     const int nd = dims.nd;
@@ -820,8 +827,9 @@ struct SolverImpl {
     return v;
   }
 
+  // The argument block of one pass of launch group g (count starts).
   template <class St>
-  void run_pass(const PassSpec& ps, int g, int count, float a, float b) {
+  PassArgs pass_args(const PassSpec& ps, int g, int count, float a, float b) const {
     const int first = g * batch;
     PassArgs p{};
     const long long inner = dims.inner(ps.axis), outer = dims.outer(ps.axis);
@@ -848,6 +856,14 @@ struct SolverImpl {
     p.radix = radix;
     p.in_scale = ps.in_scale;
     p.out_scale = ps.out_scale;
+    p.starts = count;
+    return p;
+  }
+
+  template <class St>
+  void run_pass(const PassSpec& ps, int g, int count, float a, float b) {
+    PassArgs p = pass_args<St>(ps, g, count, a, b);
+    const long long inner = dims.inner(ps.axis);
     if (ps.plane) {
       p.outer = dims.d[0];  // one tile per plane
       launch_plane<St>(dims.d[1], dims.d[2], p, count, on ? on : stream);
@@ -894,10 +910,44 @@ struct SolverImpl {
   }
 
   template <class St>
+  static constexpr int storage_code() {
+    return std::is_same_v<St, float2> ? QPCH_F32 : std::is_same_v<St, __half2> ? QPCH_F16 : QPCH_BF16;
+  }
+
+  // Whether chunks of this storage run as one persistent launch (twopass_iterations).
+  template <class St>
+  bool persistent() const {
+    return persist && dims.nd == 2 && radix == 16 && twopass_supported(dims.d[0], dims.d[1], storage_code<St>());
+  }
+
+  // A chunk of iterations of launch group g in one persistent launch; false where the
+  // shape and storage have no persistent kernel.
+  template <class St>
+  bool persistent_iterations(int g, int len, float a, float b) {
+    if (!persistent<St>()) return false;
+    const auto passes = iteration_passes(!std::is_same_v<St, float2>);
+    PassArgs p0 = pass_args<St>(passes[0], g, batch, a, b);
+    PassArgs p1 = pass_args<St>(passes[1], g, batch, a, b);
+    p0.counter = nullptr;  // the launch advances the slot counter itself
+    dev::SyncArgs q{};
+    int* base = psync.get() + static_cast<long long>(g) * psync_group() + dev::kLineInts;
+    q.next = base;
+    q.done = base + dev::kXcds * dev::kLineInts;
+    q.count = q.done + dev::kLineInts;
+    q.abort = psync.get();
+    q.tiles0 = pass_tiles<sizeof(St)>(dims.d[0], true, 1, dims.d[1], 16);
+    q.tiles1 = twopass_row_tiles(dims.d[0], dims.d[1], storage_code<St>());
+    q.wait_limit = wait_ticks;
+    QPCH_HIP(twopass_launch(dims.d[0], dims.d[1], storage_code<St>(), p0, p1, q, len, 0, on ? on : stream));
+    return true;
+  }
+
+  template <class St>
   void fused_iterations(int g, int len, float a, float b) {
     if constexpr (std::is_same_v<St, float2>) {
       if (resident) return resident_iterations(g, len, a, b);
     }
+    if (persistent_iterations<St>(g, len, a, b)) return;
     const auto passes = iteration_passes(!std::is_same_v<St, float2>);
     for (int it = 0; it < len; ++it) {
       for (const auto& ps : passes) run_pass<St>(ps, g, batch, a, b);
@@ -1402,6 +1452,7 @@ struct SolverImpl {
     }
     if (p.lanes < 0 || p.lanes > 8) throw InvalidArgument("lanes must be between 0 and 8");
     if (p.plane < 0 || p.plane > 2) throw InvalidArgument("plane must be 0, 1 or 2");
+    if (p.persist < 0 || p.persist > 2) throw InvalidArgument("persist must be 0, 1 or 2");
     QPCH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     on = stream;
     lanes = std::max(1, p.lanes);
@@ -1456,6 +1507,17 @@ struct SolverImpl {
       w16.resize(static_cast<size_t>(n) * starts);
     }
     counters.resize(2 * static_cast<size_t>(groups));  // slot counters, then image_iterations' tickets
+    persist = fused && dims.nd == 2 && !resident && p.persist != 2;
+    if (persist) {
+      int dev = 0, khz = 0;
+      QPCH_HIP(hipGetDevice(&dev));
+      QPCH_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+      wait_ticks = static_cast<long long>(std::max(khz, 1000)) * 500;  // 0.5 s
+      const long long cnt = dev::kLineInts + static_cast<long long>(groups) * psync_group();
+      psync.resize(static_cast<size_t>(cnt));
+      hipLaunchKernelGGL(dev::fill_int, dim3(grid_for(cnt)), dim3(256), 0, stream, psync.get(), static_cast<int>(cnt), 0);
+      QPCH_HIP(hipGetLastError());
+    }
     // partial sums of the error: one per block of the modulus pass of each start
     int pb = grid_for(n, 256, 1024);
     if (fused) pb = std::max({pb, axis0_partials<8>(dims, radix), axis0_partials<4>(dims, radix)});
@@ -1547,6 +1609,15 @@ struct SolverImpl {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return ms * 1e3f / static_cast<float>(reps * len);
+  }
+
+  // A persistent launch whose wait ran past its bound leaves the abort flag set: its
+  // results are not valid, and the solver reports a device failure from then on.
+  void check_persistent() {
+    if (!persist) return;
+    int flag = 0;
+    copy_sync(&flag, psync.get(), sizeof(int), hipMemcpyDeviceToHost, stream);
+    if (flag != 0) throw DeviceError("a persistent iteration launch gave up waiting on its work items");
   }
 
   void read_state(float* host) {
@@ -2041,6 +2112,7 @@ int qpch_solver_read_errors(qpch_solver* s, int first, int count, double* host) 
     auto& m = s->impl;
     if (first < 0 || count < 0 || first + count > m.err_cap) throw qpch::InvalidArgument("error slots out of range");
     QPCH_HIP(hipStreamSynchronize(m.stream));
+    m.check_persistent();
     qpch::copy_sync(host, m.errs.get() + static_cast<size_t>(first) * m.starts, sizeof(double) * count * m.starts,
                     hipMemcpyDeviceToHost, m.stream);
   });
@@ -2273,7 +2345,7 @@ std::string tune_key(const qpch_problem& p) {
   for (int a = 0; a < p.ndim; ++a) k += "x" + std::to_string(p.shape[a]);
   k += "/s" + std::to_string(p.starts * (p.problems > 1 ? p.problems : 1)) + "/st" + std::to_string(p.storage) +
        "/m" + std::to_string(p.mixed) + "/b" + std::to_string(p.batch) + "/l" + std::to_string(p.lanes) + "/r" +
-       std::to_string(p.radix);
+       std::to_string(p.radix) + "/p" + std::to_string(p.persist);
   return k;
 }
 }  // namespace
@@ -2384,8 +2456,17 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
       for (int ln : lane_options) trial(qpch_tuning{b, planes[0], 0, 0.0, ln, radices[0]});
     }
     const qpch_tuning split = best;
+    // The persistent launches place every start on an XCD by the hardware's XCD number,
+    // whatever the block numbering: nothing to try there.
+    const bool image = d.nd == 2 && qpch::image_supported(d.d[0], d.d[1]) && problem->plane != 2;
+    auto persistent_for = [&](int storage) {
+      return d.nd == 2 && problem->persist != 2 && tuned_radix == 16 && qpch::twopass_supported(d.d[0], d.d[1], storage) &&
+             !(storage == QPCH_F32 && image);
+    };
+    const bool persistent = persistent_for(problem->storage) && (!lowp || !problem->mixed || persistent_for(QPCH_F32));
     for (int pl : planes) {
       for (int xcd : {0, 2, 3}) {
+        if (xcd != 0 && persistent) continue;
         if (xcd != 0 && !(split.batch >= 8 && split.batch % 8 == 0)) continue;  // numbering only differs then
         for (int rx : radices) {
           if (pl == split.plane && xcd == split.xcd && rx == split.radix) continue;

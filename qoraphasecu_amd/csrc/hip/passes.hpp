@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "fft_engine.hpp"
 #include "storage.hpp"
@@ -69,7 +70,9 @@ struct PassArgs {
 // 128-byte line go to work-groups of the same start, hence of the same XCD's L2; halving
 // the work-group from 32 to 16 columns at 256 x fp16 made the pass 5% faster).  R is the FFT radix: 16 holds 16
 // elements per thread (fewer LDS exchanges), 8 holds 8 (twice the waves in flight).
-template <int N, bool COL, int EB, int R = 16>
+// BF > 0 fixes the lines per work-group (the persistent launches size row tiles to the
+// column pass's work-group; the lines per work-group never change a line's arithmetic).
+template <int N, bool COL, int EB, int R = 16, int BF = 0>
 struct PassGeom {
   using S = Shape1<N, R>;
   static constexpr int kT = S::kT;
@@ -82,7 +85,8 @@ struct PassGeom {
     return pow2_floor(b < lim ? b : lim);
   }
   static constexpr int kColB = 64 / EB;
-  static constexpr int kB = clampB(COL ? (256 / kT > kColB ? 256 / kT : kColB) : (256 / kT > 1 ? 256 / kT : 1));
+  static constexpr int kB =
+      BF > 0 ? BF : clampB(COL ? (256 / kT > kColB ? 256 / kT : kColB) : (256 / kT > 1 ? 256 / kT : 1));
   static constexpr int kThreads = kB * kT;
   static constexpr int kWaves = (kThreads + 63) / 64;  // error partials per work-group
   // Column passes put consecutive lanes on different lines: an odd line stride (in float2)
@@ -147,13 +151,15 @@ struct View {
       : p(base), r(make_rsrc(base, count * static_cast<long long>(sizeof(T)))), e0(first), em(step) {}
 };
 
-template <class St, bool BUF>
+// COH: a device-scope load, for data another CU may have written in the same launch.
+template <bool COH = false, class St, bool BUF>
 __device__ __forceinline__ float2 vld(const View<St, BUF>& v, int m) {
   constexpr int EB = static_cast<int>(sizeof(St));
+  using T = std::remove_const_t<St>;
   if constexpr (BUF) {
-    return Storage<St>::bld(v.r, v.e0 * EB, m * v.em * EB);
+    return Storage<T>::template bld<COH>(v.r, v.e0 * EB, m * v.em * EB);
   } else {
-    return Storage<St>::ld(v.p, static_cast<size_t>(v.e0 + m * v.em));
+    return Storage<T>::template ld<COH>(v.p, static_cast<size_t>(v.e0 + m * v.em));
   }
 }
 template <class St, bool BUF>
@@ -207,47 +213,49 @@ __device__ __forceinline__ double wave_sum(double v) {
 // part[l], part[l + 64], ... in turn, then the wave's shuffle tree; valid in lane 0.  The
 // loads go out eight at a time, so a long row (2048 partials per start at 128^3) costs a
 // few round trips rather than one per element, with the same order of additions.
+template <bool COH = false>
 __device__ __forceinline__ double partial_sum(const double* part, int count, int lane) {
   constexpr int kU = 8;
+  auto ld = [&](int i) {
+    if constexpr (COH) {
+      return load_coherent(part + i);
+    } else {
+      return part[i];
+    }
+  };
   double s = 0.0;
   int k = lane;
   for (; k + (kU - 1) * 64 < count; k += kU * 64) {
     double a[kU];
 #pragma unroll
-    for (int u = 0; u < kU; ++u) a[u] = part[k + u * 64];
+    for (int u = 0; u < kU; ++u) a[u] = ld(k + u * 64);
 #pragma unroll
     for (int u = 0; u < kU; ++u) s += a[u];
   }
-  for (; k < count; k += 64) s += part[k];
+  for (; k < count; k += 64) s += ld(k);
   return wave_sum(s);
 }
 
-// Block numbering: with p.slots > 0 (a multiple of 8) block L handles start L % slots and
-// tile L / slots, so every block of a start lands on the same XCD (blocks are dealt to the
-// eight XCDs round-robin) and the start's arrays stay in that XCD's L2 from one pass to the
-// next; otherwise block L handles tile L % tiles of start L / tiles.
-template <int N, bool COL, Mode M, class St, int R>
-__global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St), R>::kThreads))
-    line_pass(PassArgs p) {
-  using G = PassGeom<N, COL, sizeof(St), R>;
+// Slot argument of line_tile meaning "the error slot is *p.counter - 1" (the modulus pass
+// of the same iteration advanced the counter in an earlier launch).
+constexpr int kSlotFromCounter = -2147483647 - 1;
+
+// One tile of a pass: B lines of start `start` (launch-relative), tile index `tile`.
+// lds holds the tile's line scratch (G::kTile float2); tw the axis's twiddle table W_N^j.
+// With OWN_TW the tile fills tw itself while its loads are in flight and then synchronises
+// the work-group; without, tw is already filled and visible to every thread.  `slot` is
+// the error-history slot the update pass writes (kSlotFromCounter: read the counter).
+// Every thread of the work-group must call this (the transforms hold barriers).
+// COH: the tile's inputs may have been written by other CUs in the same launch (the
+// persistent launches): loads of the wave, the iterate and the partials are device scope.
+template <int N, bool COL, Mode M, class St, int R, bool OWN_TW, bool COH = false, int BF = 0>
+__device__ __forceinline__ void line_tile(const PassArgs& p, int start, int tile, float2* lds, float2* tw,
+                                          int slot) {
+  using G = PassGeom<N, COL, sizeof(St), R, BF>;
   constexpr int kT = G::kT;
   constexpr int kR = G::kR;
   constexpr int kB = G::kB;
   constexpr int kWaves = G::kWaves;
-  __shared__ float2 lds[G::kTile + N];
-  float2* tw = lds + G::kTile;
-
-  int tile, start;
-  if (p.slots > 0) {
-    start = blockIdx.x % p.slots;
-    tile = blockIdx.x / p.slots;
-    if (start >= p.starts) return;  // padding slot: the whole block leaves together
-  } else {
-    const int tiles = gridDim.x / p.starts;
-    start = blockIdx.x / tiles;
-    tile = blockIdx.x % tiles;
-  }
-
   const int tid = threadIdx.x;
   int b, i;
   if constexpr (COL) {
@@ -286,9 +294,11 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St), R>::kThreads))
   if constexpr (M == Mode::kFwd || M == Mode::kInv) {
     const SV vin(p.src ? static_cast<St*>(const_cast<void*>(p.src)) + soff : vw.p, p.n, e0, em);
 #pragma unroll
-    for (int m = 0; m < kR; ++m) v[m] = vld(vin, m);
-    load_twiddles<N>(tw);  // while the loads are in flight
-    __syncthreads();
+    for (int m = 0; m < kR; ++m) v[m] = vld<COH>(vin, m);
+    if constexpr (OWN_TW) {
+      load_twiddles<N>(tw);  // while the loads are in flight
+      __syncthreads();
+    }
     fft_line<N, R, M == Mode::kFwd>(v, i, line, tw);
     if (valid) {
 #pragma unroll
@@ -296,14 +306,16 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St), R>::kThreads))
     }
   } else if constexpr (M == Mode::kModulus) {
 #pragma unroll
-    for (int m = 0; m < kR; ++m) v[m] = vld(vw, m);
+    for (int m = 0; m < kR; ++m) v[m] = vld<COH>(vw, m);
     float amp[kR];
     const long long arow = p.amp_stride * ((p.err_offset + start) / (p.amp_group > 1 ? p.amp_group : 1));
     const View<const float, COL> va(p.amp + arow, p.n, e0, em);
 #pragma unroll
     for (int m = 0; m < kR; ++m) amp[m] = vld_f32(va, m);
-    load_twiddles<N>(tw);
-    __syncthreads();
+    if constexpr (OWN_TW) {
+      load_twiddles<N>(tw);
+      __syncthreads();
+    }
     fft_line<N, R, true>(v, i, line, tw);
     double acc = 0.0;
     if (valid) {
@@ -326,12 +338,14 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St), R>::kThreads))
   } else if constexpr (M == Mode::kGrad) {
     const SV vx(static_cast<St*>(p.x) + soff, p.n, e0, em);
 #pragma unroll
-    for (int m = 0; m < kR; ++m) v[m] = vld(vw, m);
+    for (int m = 0; m < kR; ++m) v[m] = vld<COH>(vw, m);
     float2 xo[kR];
 #pragma unroll
-    for (int m = 0; m < kR; ++m) xo[m] = vld(vx, m);
-    load_twiddles<N>(tw);
-    __syncthreads();
+    for (int m = 0; m < kR; ++m) xo[m] = vld<COH>(vx, m);
+    if constexpr (OWN_TW) {
+      load_twiddles<N>(tw);
+      __syncthreads();
+    }
     fft_line<N, R, false>(v, i, line, tw);
     if (valid) {
       const SV vo(static_cast<St*>(p.out) + soff, p.n, e0, em);
@@ -345,17 +359,19 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St), R>::kThreads))
     const SV vx(static_cast<St*>(p.x) + soff, p.n, e0, em);
     const View<const std::uint8_t, COL> vs(p.support + soff, p.n, e0, em);
 #pragma unroll
-    for (int m = 0; m < kR; ++m) v[m] = vld(vw, m);
+    for (int m = 0; m < kR; ++m) v[m] = vld<COH>(vw, m);
     // the old iterate and the support do not depend on the transform: fetch them now
     float2 xo[kR];
     bool inside[kR];
 #pragma unroll
     for (int m = 0; m < kR; ++m) {
       inside[m] = vld_flag(vs, m);
-      xo[m] = vld(vx, m);
+      xo[m] = vld<COH>(vx, m);
     }
-    load_twiddles<N>(tw);
-    __syncthreads();
+    if constexpr (OWN_TW) {
+      load_twiddles<N>(tw);
+      __syncthreads();
+    }
     fft_line<N, R, false>(v, i, line, tw);
     if (valid) {
 #pragma unroll
@@ -374,16 +390,38 @@ __global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St), R>::kThreads))
     }
     if (tile == 0 && tid < 64 && p.errs) {
       // fixed-order sum of the previous modulus pass's partials of this start
-      const double s = partial_sum(p.partials + static_cast<long long>(start) * p.partial_stride,
+      const double s = partial_sum<COH>(p.partials + static_cast<long long>(start) * p.partial_stride,
                                    p.partial_stride, tid);
       if (tid == 0) {
-        const int slot = *p.counter - 1;
-        if (slot >= 0 && slot < p.err_cap) {
-          p.errs[static_cast<long long>(slot) * p.err_stride + p.err_offset + start] = s;
+        const int e = slot == kSlotFromCounter ? *p.counter - 1 : slot;
+        if (e >= 0 && e < p.err_cap) {
+          p.errs[static_cast<long long>(e) * p.err_stride + p.err_offset + start] = s;
         }
       }
     }
   }
+}
+
+// Block numbering: with p.slots > 0 (a multiple of 8) block L handles start L % slots and
+// tile L / slots, so every block of a start lands on the same XCD (blocks are dealt to the
+// eight XCDs round-robin) and the start's arrays stay in that XCD's L2 from one pass to the
+// next; otherwise block L handles tile L % tiles of start L / tiles.
+template <int N, bool COL, Mode M, class St, int R>
+__global__ void __launch_bounds__((PassGeom<N, COL, sizeof(St), R>::kThreads))
+    line_pass(PassArgs p) {
+  using G = PassGeom<N, COL, sizeof(St), R>;
+  __shared__ float2 lds[G::kTile + N];
+  int tile, start;
+  if (p.slots > 0) {
+    start = blockIdx.x % p.slots;
+    tile = blockIdx.x / p.slots;
+    if (start >= p.starts) return;  // padding slot: the whole block leaves together
+  } else {
+    const int tiles = gridDim.x / p.starts;
+    start = blockIdx.x / tiles;
+    tile = blockIdx.x % tiles;
+  }
+  line_tile<N, COL, M, St, R, true>(p, start, tile, lds, lds + G::kTile, kSlotFromCounter);
 }
 
 }  // namespace qpch::dev

@@ -1,0 +1,200 @@
+// Whole chunks of 2D iterations in one launch: the frequency-axis (column modulus) and
+// real-axis (row update) passes of every iteration as work items of one persistent
+// kernel, so that no launch boundary separates the passes.
+//
+// Each XCD runs its own starts (start s on XCD s mod 8) from its own queue: the work-group
+// reads the XCD it runs on from the hardware (HW_REG_XCC_ID).  A queue's items are the
+// tiles of the two passes of its starts, ordered (iteration, pass, start, tile) and claimed
+// in that order from one counter.  A row tile of start s in iteration t needs every column
+// tile of (t, s) done; a column tile of (t, s) needs every row tile of (t - 1, s) done.
+// Each pass of each start counts its finished tiles.  A worker claims an item only after
+// every earlier item of its queue has been claimed, by a worker that is already running,
+// and an item waits only on earlier items of its queue: the earliest unfinished item can
+// always proceed, whatever the number of workers and whatever else shares the device, so
+// no wait can deadlock.  Each wait still has a time bound (a stuck wait sets a sticky
+// abort flag, which the host reports, and every later wait gives up at once), and the last
+// worker checks that every queue with items was emptied (an XCD without workers would
+// leave its starts undone: also reported through the flag).
+//
+// Every tile runs line_tile, the arithmetic of the two-pass launches, on the same lines
+// with the same partial layout, so results equal the two-pass schedule bit for bit.
+// Producer and consumer of a start's data share one XCD, hence one L2: a finished tile's
+// waves drain their stores to the L2 (s_waitcnt) before thread 0 counts the tile, and a
+// worker reads data that other CUs wrote in this launch with device-scope loads, which do
+// not hit in its CU's L1.
+// Agent-scope fences (an L2 write-back and invalidate per tile) measured 10-50 us per tile
+// round of 512 work-groups, against about 1 us for this protocol.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "passes.hpp"
+
+namespace qpch::dev {
+
+// Device state of one launch group's persistent launches.  next, done and count are zero
+// between launches (the last worker of a launch resets them); abort is sticky.
+constexpr int kXcds = 8;
+constexpr int kLineInts = 32;  // one 128-byte line
+struct SyncArgs {
+  int* next;    // [kXcds][kLineInts]: next item of each XCD's queue (one line each)
+  int* done;    // workers that have finished
+  int* count;   // [2][starts][kLineInts]: finished tiles of the column and the row pass, per
+                // start, one line each (the waiters poll them)
+  int* abort;   // set when a wait ran past its bound or a queue was left with items
+  int tiles0;   // column-pass tiles per start
+  int tiles1;   // row-pass tiles per start
+  long long wait_limit;  // bound of one wait in wall_clock64 ticks
+};
+
+// The XCD this wave runs on.
+__device__ __forceinline__ int xcd_id() {
+  return static_cast<int>(__builtin_amdgcn_s_getreg((20) | (0 << 6) | (3 << 11)) & (kXcds - 1));
+}
+
+// Starts of a launch on XCD x: s = x, x + 8, ...
+__host__ __device__ __forceinline__ int xcd_starts(int starts, int x) {
+  return starts > x ? (starts - x + kXcds - 1) / kXcds : 0;
+}
+
+__device__ __forceinline__ int load_agent(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Thread 0 of a worker: wait until *c >= target, or give up after `limit` ticks of the
+// device's constant-rate clock (setting abort), or at once once abort is set.  A wait on a
+// healthy launch lasts a few tile durations (tens of microseconds); the bound is time, not
+// polls, because a poll's latency grows with the number of waiters.
+__device__ __forceinline__ void wait_count(const int* c, int target, int* abort, long long limit) {
+  const long long t0 = static_cast<long long>(wall_clock64());
+  int polls = 0;
+  while (load_agent(c) < target) {
+    if ((++polls & 63) == 0) {
+      if (load_agent(abort) != 0) return;
+      if (static_cast<long long>(wall_clock64()) - t0 > limit) {
+        __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+      }
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+}
+
+// Work-group shape of the persistent launch: the column pass's (its tiles and error
+// partials exactly those of the two-pass schedule), with row tiles of as many lines as
+// fill the same work-group.
+template <int N0, int N1, class St, int R>
+struct TwoPassGeom {
+  using G0 = PassGeom<N0, true, sizeof(St), R>;
+  static constexpr int kThreads = G0::kThreads;
+  static constexpr int kT1 = Shape1<N1, R>::kT;
+  static constexpr int kB1 = kThreads / kT1 > 0 ? kThreads / kT1 : 1;
+  using G1 = PassGeom<N1, false, sizeof(St), R, kB1>;
+  static constexpr int kTile = G0::kTile > G1::kTile ? G0::kTile : G1::kTile;
+  static constexpr int kLdsBytes = (kTile + N0 + N1) * 8 + 16;
+  static constexpr bool kValid = G1::kThreads == kThreads && kLdsBytes <= 160 * 1024;
+};
+
+// `len` iterations of every start of a launch group on an N0 x N1 image (axis 0 = the
+// column pass, axis 1 = the row pass).  p0 / p1: the argument blocks of the two passes as
+// the two-pass schedule builds them (p0.counter unused); the history slot of iteration t is
+// *p1.counter + t, and the counter advances by len when the launch ends.
+template <int N0, int N1, class St, int R>
+__global__ void __launch_bounds__((TwoPassGeom<N0, N1, St, R>::kThreads))
+    twopass_iterations(PassArgs p0, PassArgs p1, SyncArgs q, int len) {
+  using TG = TwoPassGeom<N0, N1, St, R>;
+  static_assert(TG::kValid, "both passes must use work-groups of one size");
+  __shared__ float2 lds[TG::kTile + N0 + N1];
+  __shared__ int claim;
+  float2* tw0 = lds + TG::kTile;
+  float2* tw1 = tw0 + N0;
+  load_twiddles<N0>(tw0);
+  load_twiddles<N1>(tw1);
+  const int tid = threadIdx.x;
+  const int S = p1.starts;
+  const int xcd = __builtin_amdgcn_readfirstlane(xcd_id());
+  const int mine = xcd_starts(S, xcd);
+  const int per0 = mine * q.tiles0;
+  const int per_iter = per0 + mine * q.tiles1;
+  const int total = per_iter * len;
+  int* next = q.next + xcd * kLineInts;
+  int* count0 = q.count;                           // + start * kLineInts
+  int* count1 = q.count + S * kLineInts;
+  // every worker reads the counter before it takes its ticket, so the ticket's last holder
+  // can advance it without another worker of this launch reading the new value
+  const int slot0 = p1.counter ? *p1.counter : 0;
+  // Thread 0 counts the previous item and claims the next one in ONE block at the top of
+  // the loop, so that every path from one thread-0 block to the next crosses a barrier.
+  // With the count at the bottom and the claim at the top, the compiler threaded the other
+  // lanes from the one past the other, back to the claim's barrier: a second back edge that,
+  // under the exec masks, sent every lane but thread 0 round the loop again on the old
+  // claim, for ever.
+  int* done_tile = nullptr;  // the count the previous item increments
+  for (;;) {
+    if (tid == 0) {
+      if (done_tile) __hip_atomic_fetch_add(done_tile, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      claim = __hip_atomic_fetch_add(next, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    // readfirstlane: one value for the whole work-group, so the exit and every branch on
+    // the item are uniform for the compiler too
+    const int item = __builtin_amdgcn_readfirstlane(claim);
+    if (item >= total) break;
+    const int it = item / per_iter;
+    int r = item - it * per_iter;
+    const bool row = r >= per0;
+    int start, tile, target;
+    const int* dep;
+    if (!row) {
+      const int j = r / q.tiles0;
+      tile = r - j * q.tiles0;
+      start = xcd + kXcds * j;
+      dep = count1 + start * kLineInts;
+      target = it * q.tiles1;
+      done_tile = count0 + start * kLineInts;
+    } else {
+      r -= per0;
+      const int j = r / q.tiles1;
+      tile = r - j * q.tiles1;
+      start = xcd + kXcds * j;
+      dep = count0 + start * kLineInts;
+      target = (it + 1) * q.tiles0;
+      done_tile = count1 + start * kLineInts;
+    }
+    if (tid == 0) {
+      if (target > 0) wait_count(dep, target, q.abort, q.wait_limit);
+    }
+    __syncthreads();  // (also: every thread has read `claim` before it is claimed again)
+    if (!row) {
+      line_tile<N0, true, Mode::kModulus, St, R, false, true>(p0, start, tile, lds, tw0, 0);
+    } else {
+      line_tile<N1, false, Mode::kUpdate, St, R, false, true, TG::kB1>(p1, start, tile, lds, tw1, slot0 + it);
+    }
+    // every wave's stores reach the L2 before thread 0 counts the tile at the top
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+  }
+  // the last worker to finish checks the queues and resets the launch state
+  __syncthreads();
+  if (tid == 0) {
+    const int t = __hip_atomic_fetch_add(q.done, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    claim = t == static_cast<int>(gridDim.x) - 1 ? 1 : 0;
+  }
+  __syncthreads();
+  if (__builtin_amdgcn_readfirstlane(claim)) {
+    for (int k = tid; k < 2 * S; k += blockDim.x) q.count[k * kLineInts] = 0;
+    if (tid == 0) {
+      for (int x = 0; x < kXcds; ++x) {
+        const int items = xcd_starts(S, x) * (q.tiles0 + q.tiles1) * len;
+        if (items > 0 && load_agent(q.next + x * kLineInts) < items) {
+          __hip_atomic_store(q.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        q.next[x * kLineInts] = 0;
+      }
+      *q.done = 0;
+      if (p1.counter) *p1.counter = slot0 + len;
+    }
+  }
+}
+
+}  // namespace qpch::dev

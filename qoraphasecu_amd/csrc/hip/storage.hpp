@@ -14,6 +14,17 @@ namespace qpch::dev {
 // launcher guarantees arrays below 2 GiB per start).
 using Rsrc = __amdgpu_buffer_rsrc_t;
 
+// Loads that another CU of the same XCD may just have written (the persistent launches):
+// device scope, which does not hit in the CU's L1 (a plain or work-group-scope load, or a
+// work-group-scope L1 invalidate, returned the CU's stale L1 copy every time in a probe on
+// the MI355X; a device-scope load never did).  kCoh is the buffer instructions' cache
+// policy bit for it (SC1); pointer loads get it as relaxed agent-scope atomic loads.
+constexpr int kCoh = 16;
+template <class T>
+__device__ __forceinline__ T load_coherent(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ Rsrc make_rsrc(const void* base, long long bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes), 0x00020000);
 }
@@ -29,11 +40,19 @@ struct Storage;
 
 template <>
 struct Storage<float2> {
-  static __device__ __forceinline__ float2 ld(const float2* p, size_t i) { return p[i]; }
+  template <bool COH = false>
+  static __device__ __forceinline__ float2 ld(const float2* p, size_t i) {
+    if constexpr (COH) {
+      return __builtin_bit_cast(float2, load_coherent(reinterpret_cast<const unsigned long long*>(p + i)));
+    } else {
+      return p[i];
+    }
+  }
   static __device__ __forceinline__ void st(float2* p, size_t i, float2 v) { p[i] = v; }
   // byte offsets vo (lane) + so (uniform) into a buffer resource
+  template <bool COH = false>
   static __device__ __forceinline__ float2 bld(Rsrc r, int vo, int so) {
-    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, COH ? kCoh : 0));
   }
   static __device__ __forceinline__ void bst(Rsrc r, int vo, int so, float2 v) {
     using U = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
@@ -43,14 +62,20 @@ struct Storage<float2> {
 
 template <>
 struct Storage<__half2> {
+  template <bool COH = false>
   static __device__ __forceinline__ float2 ld(const __half2* p, size_t i) {
-    return __half22float2(p[i]);
+    if constexpr (COH) {
+      return __half22float2(__builtin_bit_cast(__half2, load_coherent(reinterpret_cast<const unsigned*>(p + i))));
+    } else {
+      return __half22float2(p[i]);
+    }
   }
   static __device__ __forceinline__ void st(__half2* p, size_t i, float2 v) {
     p[i] = __float22half2_rn(v);
   }
+  template <bool COH = false>
   static __device__ __forceinline__ float2 bld(Rsrc r, int vo, int so) {
-    return __half22float2(__builtin_bit_cast(__half2, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0)));
+    return __half22float2(__builtin_bit_cast(__half2, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, COH ? kCoh : 0)));
   }
   static __device__ __forceinline__ void bst(Rsrc r, int vo, int so, float2 v) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, __float22half2_rn(v)), r, vo, so, 0);
@@ -59,15 +84,24 @@ struct Storage<__half2> {
 
 template <>
 struct Storage<__hip_bfloat162> {
+  template <bool COH = false>
   static __device__ __forceinline__ float2 ld(const __hip_bfloat162* p, size_t i) {
-    return __bfloat1622float2(p[i]);
+    if constexpr (COH) {
+      const unsigned u = load_coherent(reinterpret_cast<const unsigned*>(p + i));
+      __hip_bfloat162 h;
+      __builtin_memcpy(static_cast<void*>(&h), &u, sizeof(u));
+      return __bfloat1622float2(h);
+    } else {
+      return __bfloat1622float2(p[i]);
+    }
   }
   static __device__ __forceinline__ void st(__hip_bfloat162* p, size_t i, float2 v) {
     p[i] = __float22bfloat162_rn(v);
   }
   // (__hip_bfloat162 is not trivially copyable: its bits move through memcpy)
+  template <bool COH = false>
   static __device__ __forceinline__ float2 bld(Rsrc r, int vo, int so) {
-    const unsigned u = __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0);
+    const unsigned u = __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, COH ? kCoh : 0);
     __hip_bfloat162 h;
     __builtin_memcpy(static_cast<void*>(&h), &u, sizeof(u));
     return __bfloat1622float2(h);

@@ -1,0 +1,25 @@
+// Host interface of the persistent two-pass launches (twopass.hip, its own translation
+// unit so that it compiles beside the engine).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "passes.hpp"
+#include "persist.hpp"
+
+namespace qpch {
+
+// storage: 0 complex64, 1 half2, 2 bfloat162 (the iterate's storage, as QPCH_F32/F16/BF16)
+bool twopass_supported(int ny, int nx, int storage);
+
+// Row-pass tiles per start of the persistent kernel for this shape (its row tiles hold as
+// many lines as fill the column pass's work-group); 0 when there is no kernel.
+int twopass_row_tiles(int ny, int nx, int storage);
+
+// Launches twopass_iterations for `len` iterations on stream s with `workers` work-groups
+// (0: eight times the busiest XCD's tiles per pass, at most what fits the device at once,
+// on `xcds` XCDs).  Returns the launch status; hipErrorInvalidValue when no kernel exists.
+hipError_t twopass_launch(int ny, int nx, int storage, const dev::PassArgs& p0, const dev::PassArgs& p1,
+                          const dev::SyncArgs& q, int len, int workers, hipStream_t s);
+
+}  // namespace qpch

@@ -1,0 +1,99 @@
+"""The persistent two-pass launches (twopass_iterations: every iteration of a chunk in one
+launch, the tiles of both passes claimed from per-XCD queues) against the launch-per-pass
+schedule (qpch_problem.persist = 2) on the same inputs: error histories, iterates and
+supports equal bit for bit, over recipes with Shrinkwrap, detwin and ER stages, for fp32 and
+fp16 / bf16 storage (ER in fp32), starts that do not fill the eight XCDs evenly, shapes
+whose row tiles are resized to the column pass's work-group, launch groups running as
+concurrent persistent launches on lanes, and whole recipes of several shapes in one graph
+(their persistent launches run side by side).  Also against the oracle: the per-iteration
+error of a plain persistent run within 1e-5 of the oracle's float run."""
+
+import numpy as np
+import pytest
+
+import oracle
+import recipes as R
+import synthetic
+from test_gpu_whole_recipe import RECIPE, _same, _stepwise
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(qpch, shape, persist, **kw):
+    obj, sup, inten = synthetic.triangle(shape, max(4, min(shape[-2:]) // 4))
+    amp, mask, sup = oracle.make_problem(inten, None, None)
+    f = oracle.to_fft_order
+    return qpch.Solver(f(amp), f(mask), f(sup), seed=5, schedule="graph", persist=persist, **kw)
+
+
+CASES = [
+    ((256, 256), dict(starts=32)),  # the headline shape: 4 starts per XCD
+    ((256, 256), dict(starts=12, storage="f16", mixed=True)),  # XCDs with two starts and with one
+    ((256, 256), dict(starts=3, storage="bf16", mixed=True)),  # five XCDs without work
+    ((256, 256), dict(starts=2, batch=1)),  # one start per launch (the plain path)
+    ((128, 128), dict(starts=8)),
+    ((512, 512), dict(starts=4, storage="f16", mixed=True)),  # row tiles of 16 lines
+    ((128, 512), dict(starts=4)),
+    ((1024, 64), dict(starts=2)),  # column work-group of 512 threads
+    ((64, 1024), dict(starts=2, storage="f16")),
+    ((64, 64), dict(starts=8, storage="f16", mixed=True)),  # fp16 64^2 (fp32 runs whole images)
+    ((256, 256), dict(starts=16, batch=4, lanes=4)),  # four persistent launches at once
+]
+
+
+@pytest.mark.parametrize("shape,kw", CASES)
+def test_persistent_equals_pass_launches(qpch, shape, kw):
+    a = _solver(qpch, shape, 2, **kw)
+    b = _solver(qpch, shape, 0, **kw)
+    for rep in range(2):  # the graphs captured, then replayed
+        want = _stepwise(a, RECIPE)
+        got = _stepwise(b, RECIPE)
+        assert np.array_equal(got, want), (shape, kw, rep, np.argwhere(got != want)[:4])
+        _same(a, b)
+    a.close()
+    b.close()
+
+
+def test_persistent_whole_recipes_side_by_side(qpch):
+    shapes = [((128, 128), dict(starts=8)), ((256, 256), dict(starts=8, storage="f16", mixed=True)),
+              ((512, 512), dict(starts=4))]
+    singles = [_solver(qpch, s, 2, **kw) for s, kw in shapes]
+    members = [_solver(qpch, s, 0, **kw) for s, kw in shapes]
+    group = qpch.Group(members)
+    total = R.iterations(RECIPE)
+    for rep in range(2):
+        wants = [_stepwise(s, RECIPE) for s in singles]
+        for m in members:
+            m.begin_run(total)
+        group.run(RECIPE)
+        for m, s, want in zip(members, singles, wants):
+            assert np.array_equal(m.read_errors(0, total), want), rep
+            _same(m, s)
+    group.close()
+    for s in singles + members:
+        s.close()
+
+
+@pytest.mark.parametrize("kind", ["er", "hio", "raar"])
+def test_persistent_per_iteration_error_matches_oracle(qpch, kind):
+    """Chunks of 10 and 20 iterations (two persistent launches) from given starts: the
+    first ten errors of each start within 1e-5 of the oracle's float run (beyond them,
+    float rounding alone separates equally valid float runs; the whole-run bars are in
+    test_gpu_per_iteration.py, whose 256 x 256 cases run these launches too)."""
+    shape = (128, 128)
+    obj, sup, inten = synthetic.triangle(shape, 32)
+    amp, mask, sup = oracle.make_problem(inten, sup, None)
+    start = np.stack([synthetic.random_field(obj.size, 31).reshape(shape),
+                      synthetic.random_field(obj.size, 32).reshape(shape)])
+    ops = [(kind, 30, 0.9, 0.0, 0.0, "max", 0)]
+    o = oracle.Solver(amp, mask, sup, starts=2, start=start, prec="f")
+    o.run(ops)
+    want = [np.asarray(r[2], np.float64) for r in o.results()]
+    f = oracle.to_fft_order
+    s = qpch.Solver(f(amp), f(mask), f(sup), starts=2, start=np.stack([f(x) for x in start]), schedule="graph",
+                    persist=0)
+    errs = R.run_abi(s, ops, sync_every=20).astype(np.float64)
+    s.close()
+    for k in range(2):
+        d = np.abs(errs[:10, k] - want[k][:10]) / want[k][:10]
+        assert np.all(d <= 1e-5), (kind, k, d.max())

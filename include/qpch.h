@@ -97,6 +97,7 @@ typedef struct {
   double us_per_iteration;
   int lanes;  /* as qpch_problem.lanes */
   int radix;  /* as qpch_problem.radix */
+  int persist;  /* as qpch_problem.persist */
 } qpch_tuning;
 
 typedef struct {

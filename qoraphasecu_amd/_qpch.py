@@ -54,6 +54,7 @@ class QpchTuning(ctypes.Structure):
         ("us_per_iteration", ctypes.c_double),
         ("lanes", ctypes.c_int),
         ("radix", ctypes.c_int),
+        ("persist", ctypes.c_int),
     ]
 
 
@@ -329,9 +330,9 @@ class Solver:
         if tuned and os.environ.get("QPCH_TUNE") != "0":  # QPCH_TUNE=0: the defaults, as in the C++ host
             t = QpchTuning()
             check(lib().qpch_autotune(ctypes.byref(p), ctypes.byref(t)))
-            p.batch, p.plane, p.xcd, p.lanes, p.radix = t.batch, t.plane, t.xcd, t.lanes, t.radix
+            p.batch, p.plane, p.xcd, p.lanes, p.radix, p.persist = t.batch, t.plane, t.xcd, t.lanes, t.radix, t.persist
             self.tuning = dict(batch=t.batch, plane=t.plane, xcd=t.xcd, lanes=t.lanes, radix=t.radix,
-                               us_per_iteration=t.us_per_iteration)
+                               persist=t.persist, us_per_iteration=t.us_per_iteration)
         h = ctypes.c_void_p()
         check(lib().qpch_solver_create(ctypes.byref(p), ctypes.byref(h)))
         self._h = h

@@ -2404,30 +2404,35 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
       q.plane = t.plane;
       q.xcd = t.xcd;
       q.radix = t.radix;
+      q.persist = t.persist;
       q.start = nullptr;
       q.schedule = QPCH_GRAPH;
       auto s = std::make_unique<qpch_solver>();
       s->impl.init(q);
       auto& m = s->impl;
-      m.begin_run(128);
+      m.begin_run(160);
       hipEvent_t e0{}, e1{};
       QPCH_HIP(hipEventCreate(&e0));
       QPCH_HIP(hipEventCreate(&e1));
-      // Per iteration: the fastest of three rounds of eight, after a warm-up that captures
-      // the graphs (candidates a few percent apart must not swap places on a hiccup).
+      // Per iteration: the fastest of three rounds of two chunks of ten (the reference's
+      // default sync_every, the shortest chunk the C++ host launches), after a warm-up chunk
+      // that captures the graph (candidates a few percent apart must not swap places on a
+      // hiccup).  Chunks of four made the persistent launch look slower than the per-pass
+      // launches at 256^2 x 32 (21.4 against 19.6 us per iteration), where chunks of ten
+      // and more run it faster (13 and 12.3 against 16.2).
       auto per_iteration = [&](bool refine) {
-        m.iterate(4, 0.9f, -0.8f, refine);
+        m.iterate(10, 0.9f, -0.8f, refine);
         float best_ms = 1e30f;
         for (int round = 0; round < 3; ++round) {
           QPCH_HIP(hipEventRecord(e0, m.stream));
-          for (int r = 0; r < 2; ++r) m.iterate(4, 0.9f, -0.8f, refine);
+          for (int r = 0; r < 2; ++r) m.iterate(10, 0.9f, -0.8f, refine);
           QPCH_HIP(hipEventRecord(e1, m.stream));
           QPCH_HIP(hipEventSynchronize(e1));
           float ms = 0.0f;
           QPCH_HIP(hipEventElapsedTime(&ms, e0, e1));
           best_ms = std::min(best_ms, ms);
         }
-        return best_ms * 1e3 / 8.0;
+        return best_ms * 1e3 / 20.0;
       };
       double us = per_iteration(!lowp);
       // A mixed solver also runs its ER stages in fp32 with the same launch choices:
@@ -2442,7 +2447,8 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
       r.us_per_iteration = measure(t);
       line += " [batch " + std::to_string(t.batch) + (t.lanes > 1 ? " x" + std::to_string(t.lanes) + " lanes" : "") +
               (t.plane ? (t.plane == 1 ? " plane" : " lines") : "") + (t.xcd == 2 ? " xcd-off" : t.xcd == 3 ? " xcd-off-modulus" : "") +
-              (t.radix == 8 ? " radix-8" : "") + ": " + std::to_string(r.us_per_iteration) + " us/it]";
+              (t.radix == 8 ? " radix-8" : "") + (t.persist == 2 ? " pass-launches" : "") + ": " +
+              std::to_string(r.us_per_iteration) + " us/it]";
       // A candidate must beat the incumbent by 2%: timings a few tenths of a percent apart
       // are noise, and the first candidate is the default (all starts in one launch, one
       // lane, XCD-affine numbering), which then stays unless another is clearly faster.
@@ -2453,7 +2459,7 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
     for (int b : batches) {
       std::vector<int> lane_options{1};
       if (problem->lanes != 1 && total / b > 1) lane_options.push_back(std::min(4, total / b));
-      for (int ln : lane_options) trial(qpch_tuning{b, planes[0], 0, 0.0, ln, radices[0]});
+      for (int ln : lane_options) trial(qpch_tuning{b, planes[0], 0, 0.0, ln, radices[0], problem->persist});
     }
     const qpch_tuning split = best;
     // The persistent launches place every start on an XCD by the hardware's XCD number,
@@ -2470,13 +2476,26 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
         if (xcd != 0 && !(split.batch >= 8 && split.batch % 8 == 0)) continue;  // numbering only differs then
         for (int rx : radices) {
           if (pl == split.plane && xcd == split.xcd && rx == split.radix) continue;
-          trial(qpch_tuning{split.batch, pl, xcd, 0.0, split.lanes, rx});
+          trial(qpch_tuning{split.batch, pl, xcd, 0.0, split.lanes, rx, split.persist});
         }
+      }
+    }
+    // Persistent or per-pass launches (the same arithmetic, bit for bit): the persistent
+    // kernel saves each pass's launch ramp and drain and keeps an XCD's starts in its L2,
+    // but pays a queue claim and a wait per tile, which loses where a launch already holds
+    // many tiles per CU (512^2 x 64 fp16: 290 against 267 us per iteration).  When the
+    // per-pass launches win, their XCD numbering is tried too.
+    if (persistent && problem->persist == 0) {
+      const qpch_tuning keep = best;
+      for (int xcd : {0, 2, 3}) {
+        if (xcd != 0 && !(best.batch >= 8 && best.batch % 8 == 0)) continue;
+        trial(qpch_tuning{keep.batch, keep.plane, xcd, 0.0, keep.lanes, keep.radix, 2});
+        if (best.persist != 2) break;  // numbering only matters once the pass launches win
       }
     }
     line += " -> batch " + std::to_string(best.batch) + " lanes " + std::to_string(best.lanes) + " plane " +
             std::to_string(best.plane) + " xcd " + std::to_string(best.xcd) + " radix " + std::to_string(best.radix) +
-            "\n";
+            (best.persist == 2 ? " pass-launches" : "") + "\n";
     std::lock_guard<std::mutex> lock(g_tune_mu);
     g_tune_cache[key] = best;
     g_tune_log += line;

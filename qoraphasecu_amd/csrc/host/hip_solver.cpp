@@ -40,9 +40,9 @@ struct Plan {
   bool batched;
 };
 
-// Launch configuration from the engine's autotuner (starts per launch, 3D plane or line
-// passes, XCD-affine numbering); QPCH_TUNE=0 keeps the defaults.  The configuration does
-// not change results.
+// Launch configuration from the engine's autotuner (starts per launch, concurrent launch
+// groups, XCD-affine numbering, persistent or per-pass launches); QPCH_TUNE=0 keeps the
+// defaults.  The configuration does not change results.
 void tune(qpch_problem& q) {
   if (const char* v = std::getenv("QPCH_TUNE"); v && v[0] == '0') {
     if (q.batch == 0) q.batch = q.starts * (q.problems > 1 ? q.problems : 1);
@@ -58,6 +58,7 @@ void tune(qpch_problem& q) {
   q.xcd = t.xcd;
   q.lanes = t.lanes;
   q.radix = t.radix;
+  q.persist = t.persist;
 }
 
 Plan plan_for(Variant v) {

@@ -796,9 +796,11 @@ struct SolverImpl {
   bool persist = false;  // 2D chunks of iterations in one persistent launch where a kernel exists
   DevBuf<int> psync;  // a line holding the abort flag, then per launch group: next[8 lines], done, count[2][batch]
   long long psync_group() const {
-    return static_cast<long long>(dev::kXcds + 1 + 2 * batch) * dev::kLineInts;
+    return static_cast<long long>(dev::kXcds + 1 + 2 * batch) * dev::kLineInts;  // (kXcds: room for the most)
   }
   long long wait_ticks = 0;  // bound of one persistent wait: half a second of wall_clock64
+  int persist_xcds = 0;      // XCDs the persistent launches spread starts over
+  int persist_workers = 0;   // work-groups per persistent launch (0: sized by the launcher)
 
   std::vector<PassSpec> iteration_passes(bool lowp) const {
     const int nd = dims.nd;
@@ -937,8 +939,9 @@ struct SolverImpl {
     q.abort = psync.get();
     q.tiles0 = pass_tiles<sizeof(St)>(dims.d[0], true, 1, dims.d[1], 16);
     q.tiles1 = twopass_row_tiles(dims.d[0], dims.d[1], storage_code<St>());
+    q.xcds = persist_xcds;
     q.wait_limit = wait_ticks;
-    QPCH_HIP(twopass_launch(dims.d[0], dims.d[1], storage_code<St>(), p0, p1, q, len, 0, on ? on : stream));
+    QPCH_HIP(twopass_launch(dims.d[0], dims.d[1], storage_code<St>(), p0, p1, q, len, persist_workers, on ? on : stream));
     return true;
   }
 
@@ -1507,12 +1510,18 @@ struct SolverImpl {
       w16.resize(static_cast<size_t>(n) * starts);
     }
     counters.resize(2 * static_cast<size_t>(groups));  // slot counters, then image_iterations' tickets
-    persist = fused && dims.nd == 2 && !resident && p.persist != 2;
+    persist_xcds = twopass_xcds();
+    persist = fused && dims.nd == 2 && !resident && p.persist != 2 && persist_xcds > 0;
     if (persist) {
       int dev = 0, khz = 0;
       QPCH_HIP(hipGetDevice(&dev));
       QPCH_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
       wait_ticks = static_cast<long long>(std::max(khz, 1000)) * 500;  // 0.5 s
+      // Test hooks for the safety nets (tests/test_gpu_persistent.py): a wait bound in
+      // clock ticks, and fewer work-groups than XCDs (queues left with items).  Both end in
+      // the abort flag and a DeviceError, never in a hang.
+      if (const char* v = std::getenv("QPCH_TEST_WAIT_TICKS")) wait_ticks = std::max(0LL, std::atoll(v));
+      if (const char* v = std::getenv("QPCH_TEST_PERSIST_WORKERS")) persist_workers = std::max(0, std::atoi(v));
       const long long cnt = dev::kLineInts + static_cast<long long>(groups) * psync_group();
       psync.resize(static_cast<size_t>(cnt));
       hipLaunchKernelGGL(dev::fill_int, dim3(grid_for(cnt)), dim3(256), 0, stream, psync.get(), static_cast<int>(cnt), 0);
@@ -1611,13 +1620,17 @@ struct SolverImpl {
     return ms * 1e3f / static_cast<float>(reps * len);
   }
 
-  // A persistent launch whose wait ran past its bound leaves the abort flag set: its
-  // results are not valid, and the solver reports a device failure from then on.
+  // A persistent launch whose wait ran past its bound, or that left a queue with items (an
+  // XCD without work-groups), sets the abort flag: its results are not valid, and the
+  // solver reports a device failure from then on.
   void check_persistent() {
     if (!persist) return;
     int flag = 0;
     copy_sync(&flag, psync.get(), sizeof(int), hipMemcpyDeviceToHost, stream);
-    if (flag != 0) throw DeviceError("a persistent iteration launch gave up waiting on its work items");
+    if (flag != 0) {
+      throw DeviceError("a persistent iteration launch did not complete its work items (a wait ran past its bound "
+                        "or an XCD's queue was left with items); its results are not valid");
+    }
   }
 
   void read_state(float* host) {
@@ -2466,8 +2479,8 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
     // whatever the block numbering: nothing to try there.
     const bool image = d.nd == 2 && qpch::image_supported(d.d[0], d.d[1]) && problem->plane != 2;
     auto persistent_for = [&](int storage) {
-      return d.nd == 2 && problem->persist != 2 && tuned_radix == 16 && qpch::twopass_supported(d.d[0], d.d[1], storage) &&
-             !(storage == QPCH_F32 && image);
+      return d.nd == 2 && problem->persist != 2 && tuned_radix == 16 && qpch::twopass_xcds() > 0 &&
+             qpch::twopass_supported(d.d[0], d.d[1], storage) && !(storage == QPCH_F32 && image);
     };
     const bool persistent = persistent_for(problem->storage) && (!lowp || !problem->mixed || persistent_for(QPCH_F32));
     for (int pl : planes) {

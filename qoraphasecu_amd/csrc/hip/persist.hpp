@@ -2,8 +2,9 @@
 // real-axis (row update) passes of every iteration as work items of one persistent
 // kernel, so that no launch boundary separates the passes.
 //
-// Each XCD runs its own starts (start s on XCD s mod 8) from its own queue: the work-group
-// reads the XCD it runs on from the hardware (HW_REG_XCC_ID).  A queue's items are the
+// Each XCD runs its own starts (start s on XCD s mod X, X the device's XCD count: 8 on a
+// whole MI355X) from its own queue: the work-group reads the XCD it runs on from the
+// hardware (HW_REG_XCC_ID).  A queue's items are the
 // tiles of the two passes of its starts, ordered (iteration, pass, start, tile) and claimed
 // in that order from one counter.  A row tile of start s in iteration t needs every column
 // tile of (t, s) done; a column tile of (t, s) needs every row tile of (t - 1, s) done.
@@ -34,7 +35,7 @@ namespace qpch::dev {
 
 // Device state of one launch group's persistent launches.  next, done and count are zero
 // between launches (the last worker of a launch resets them); abort is sticky.
-constexpr int kXcds = 8;
+constexpr int kXcds = 8;       // XCDs of a whole MI355X (the most a gfx950 device has)
 constexpr int kLineInts = 32;  // one 128-byte line
 struct SyncArgs {
   int* next;    // [kXcds][kLineInts]: next item of each XCD's queue (one line each)
@@ -44,17 +45,19 @@ struct SyncArgs {
   int* abort;   // set when a wait ran past its bound or a queue was left with items
   int tiles0;   // column-pass tiles per start
   int tiles1;   // row-pass tiles per start
+  int xcds;     // XCDs of the device (1, 2, 4 or 8: a partitioned MI355X has fewer)
   long long wait_limit;  // bound of one wait in wall_clock64 ticks
 };
 
-// The XCD this wave runs on.
-__device__ __forceinline__ int xcd_id() {
-  return static_cast<int>(__builtin_amdgcn_s_getreg((20) | (0 << 6) | (3 << 11)) & (kXcds - 1));
+// The XCD this wave runs on, numbered within the device (the hardware's XCC number modulo
+// the device's XCD count: a partition's XCDs are consecutive and aligned).
+__device__ __forceinline__ int xcd_id(int xcds) {
+  return static_cast<int>(__builtin_amdgcn_s_getreg((20) | (0 << 6) | (3 << 11)) & (xcds - 1));
 }
 
-// Starts of a launch on XCD x: s = x, x + 8, ...
-__host__ __device__ __forceinline__ int xcd_starts(int starts, int x) {
-  return starts > x ? (starts - x + kXcds - 1) / kXcds : 0;
+// Starts of a launch on XCD x of `xcds`: s = x, x + xcds, ...
+__host__ __device__ __forceinline__ int xcd_starts(int starts, int x, int xcds) {
+  return starts > x ? (starts - x + xcds - 1) / xcds : 0;
 }
 
 __device__ __forceinline__ int load_agent(const int* p) {
@@ -69,13 +72,11 @@ __device__ __forceinline__ void wait_count(const int* c, int target, int* abort,
   const long long t0 = static_cast<long long>(wall_clock64());
   int polls = 0;
   while (load_agent(c) < target) {
-    if ((++polls & 63) == 0) {
-      if (load_agent(abort) != 0) return;
-      if (static_cast<long long>(wall_clock64()) - t0 > limit) {
-        __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return;
-      }
+    if (static_cast<long long>(wall_clock64()) - t0 > limit) {
+      __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return;
     }
+    if ((++polls & 63) == 0 && load_agent(abort) != 0) return;
     __builtin_amdgcn_s_sleep(2);
   }
 }
@@ -112,8 +113,8 @@ __global__ void __launch_bounds__((TwoPassGeom<N0, N1, St, R>::kThreads))
   load_twiddles<N1>(tw1);
   const int tid = threadIdx.x;
   const int S = p1.starts;
-  const int xcd = __builtin_amdgcn_readfirstlane(xcd_id());
-  const int mine = xcd_starts(S, xcd);
+  const int xcd = __builtin_amdgcn_readfirstlane(xcd_id(q.xcds));
+  const int mine = xcd_starts(S, xcd, q.xcds);
   const int per0 = mine * q.tiles0;
   const int per_iter = per0 + mine * q.tiles1;
   const int total = per_iter * len;
@@ -148,7 +149,7 @@ __global__ void __launch_bounds__((TwoPassGeom<N0, N1, St, R>::kThreads))
     if (!row) {
       const int j = r / q.tiles0;
       tile = r - j * q.tiles0;
-      start = xcd + kXcds * j;
+      start = xcd + q.xcds * j;
       dep = count1 + start * kLineInts;
       target = it * q.tiles1;
       done_tile = count0 + start * kLineInts;
@@ -156,7 +157,7 @@ __global__ void __launch_bounds__((TwoPassGeom<N0, N1, St, R>::kThreads))
       r -= per0;
       const int j = r / q.tiles1;
       tile = r - j * q.tiles1;
-      start = xcd + kXcds * j;
+      start = xcd + q.xcds * j;
       dep = count0 + start * kLineInts;
       target = (it + 1) * q.tiles0;
       done_tile = count1 + start * kLineInts;
@@ -184,8 +185,8 @@ __global__ void __launch_bounds__((TwoPassGeom<N0, N1, St, R>::kThreads))
   if (__builtin_amdgcn_readfirstlane(claim)) {
     for (int k = tid; k < 2 * S; k += blockDim.x) q.count[k * kLineInts] = 0;
     if (tid == 0) {
-      for (int x = 0; x < kXcds; ++x) {
-        const int items = xcd_starts(S, x) * (q.tiles0 + q.tiles1) * len;
+      for (int x = 0; x < q.xcds; ++x) {
+        const int items = xcd_starts(S, x, q.xcds) * (q.tiles0 + q.tiles1) * len;
         if (items > 0 && load_agent(q.next + x * kLineInts) < items) {
           __hip_atomic_store(q.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }

@@ -39,7 +39,7 @@ int capacity() {
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
       return 0;
     }
-    return std::max(1, per_cu) * cus / dev::kXcds * dev::kXcds;
+    return std::max(1, per_cu) * cus;
   }();
   return cap;
 }
@@ -49,12 +49,12 @@ hipError_t launch(const dev::PassArgs& p0, const dev::PassArgs& p1, const dev::S
                   hipStream_t s) {
   if constexpr (valid<N0, N1, St>()) {
     using TG = dev::TwoPassGeom<N0, N1, St, 16>;
-    // work-groups are dealt to the XCDs round-robin: eight times the busiest XCD's tiles
-    // per pass, at most what fits the device at once
-    const int items = dev::kXcds * dev::xcd_starts(p1.starts, 0) * std::max(q.tiles0, q.tiles1);
-    const int cap = capacity<N0, N1, St>();
+    // work-groups are dealt to the XCDs round-robin: as many per XCD as the busiest XCD
+    // has tiles per pass, at most what fits the device at once (a multiple of the XCDs)
+    const int items = q.xcds * dev::xcd_starts(p1.starts, 0, q.xcds) * std::max(q.tiles0, q.tiles1);
+    const int cap = capacity<N0, N1, St>() / q.xcds * q.xcds;
     if (cap <= 0) return hipErrorInvalidDeviceFunction;
-    const int w = workers > 0 ? workers : std::max(dev::kXcds, std::min(items, cap));
+    const int w = workers > 0 ? workers : std::max(q.xcds, std::min(items, cap));
     hipLaunchKernelGGL((dev::twopass_iterations<N0, N1, St, 16>), dim3(w), dim3(TG::kThreads), 0, s, p0, p1, q, len);
     return hipGetLastError();
   } else {
@@ -63,6 +63,15 @@ hipError_t launch(const dev::PassArgs& p0, const dev::PassArgs& p1, const dev::S
 }
 
 }  // namespace
+
+int twopass_xcds() {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+    return 0;
+  }
+  const int x = cus / 32;
+  return cus % 32 == 0 && (x == 1 || x == 2 || x == 4 || x == 8) ? x : 0;
+}
 
 bool twopass_supported(int ny, int nx, int storage) {
 #define QPCH_CASE(A, B)                                                               \

@@ -16,9 +16,14 @@ bool twopass_supported(int ny, int nx, int storage);
 // many lines as fill the column pass's work-group); 0 when there is no kernel.
 int twopass_row_tiles(int ny, int nx, int storage);
 
+// XCDs of the current device for the persistent launches: its compute units in groups of
+// 32 (an MI355X XCD), when that gives 1, 2, 4 or 8; 0 when the device does not divide so,
+// and the persistent launches are not used there.
+int twopass_xcds();
+
 // Launches twopass_iterations for `len` iterations on stream s with `workers` work-groups
-// (0: eight times the busiest XCD's tiles per pass, at most what fits the device at once,
-// on `xcds` XCDs).  Returns the launch status; hipErrorInvalidValue when no kernel exists.
+// (0: as many per XCD as the busiest XCD has tiles per pass, at most what fits the device
+// at once).  Returns the launch status; hipErrorInvalidValue when no kernel exists.
 hipError_t twopass_launch(int ny, int nx, int storage, const dev::PassArgs& p0, const dev::PassArgs& p1,
                           const dev::SyncArgs& q, int len, int workers, hipStream_t s);
 

@@ -97,3 +97,30 @@ def test_persistent_per_iteration_error_matches_oracle(qpch, kind):
     for k in range(2):
         d = np.abs(errs[:10, k] - want[k][:10]) / want[k][:10]
         assert np.all(d <= 1e-5), (kind, k, d.max())
+
+
+def test_a_wait_past_its_bound_is_reported(qpch, monkeypatch):
+    """The safety net of the waits, forced from the host: with a bound of zero clock ticks,
+    the first wait that finds its dependency unfinished gives up and sets the abort flag
+    (the row tiles of a start wait on its column tiles at least once per launch); the
+    launch still runs to its end, and the host reports a device failure."""
+    monkeypatch.setenv("QPCH_TEST_WAIT_TICKS", "0")
+    s = _solver(qpch, (256, 256), 0, starts=32)
+    s.begin_run(20)
+    s.iterate(20, 0.9, -0.8)
+    with pytest.raises(qpch.DeviceError, match="persistent"):
+        s.read_errors(0, 20)
+    s.close()
+
+
+def test_an_xcd_without_workers_is_reported(qpch, monkeypatch):
+    """The safety net of the queues, forced from the host: one work-group (on XCD 0) runs
+    its own queue's items and nothing else; the last worker finds the other XCDs' queues
+    unclaimed and sets the abort flag, so starts left undone never pass for results."""
+    monkeypatch.setenv("QPCH_TEST_PERSIST_WORKERS", "1")
+    s = _solver(qpch, (128, 128), 0, starts=8)
+    s.begin_run(10)
+    s.iterate(10, 0.9, -0.8)
+    with pytest.raises(qpch.DeviceError, match="persistent"):
+        s.read_errors(0, 10)
+    s.close()

@@ -119,15 +119,20 @@ def time_refinement(steps=500):
         inten.append(i)
         masks.append(m.astype(bool))
     x0 = np.stack(x0)
-    autograd.refine(x0, np.stack(inten), np.stack(sup), mask=np.stack(masks), steps=5)  # warm up
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    out, hist = autograd.refine(x0, np.stack(inten), np.stack(sup), mask=np.stack(masks), steps=steps)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    h = hist.cpu().numpy()
-    log(f"  c5 adam: {wall / steps * 1e3:.3f} ms/step (16 x 256^2), loss {h[0].sum():.4g} -> {h[-1].sum():.4g}")
-    return dict(ms_per_step=wall / steps * 1e3, steps=steps, loss_first=float(h[0].sum()), loss_last=float(h[-1].sum()))
+    res = {}
+    for graph in (True, False):
+        autograd.refine(x0, np.stack(inten), np.stack(sup), mask=np.stack(masks), steps=5, graph=graph)  # warm up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out, hist = autograd.refine(x0, np.stack(inten), np.stack(sup), mask=np.stack(masks), steps=steps, graph=graph)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        h = hist.cpu().numpy()
+        tag = "graph" if graph else "eager"
+        log(f"  c5 adam ({tag}): {wall / steps * 1e3:.3f} ms/step (16 x 256^2), loss {h[0].sum():.4g} -> {h[-1].sum():.4g}")
+        res[tag] = dict(ms_per_step=wall / steps * 1e3, steps=steps, loss_first=float(h[0].sum()),
+                        loss_last=float(h[-1].sum()))
+    return res
 
 
 def time_variant(w, variant, batch, steps, warmup):

@@ -50,10 +50,12 @@ def fft_order_amplitude(intensity, mask=None, ndim=None, dtype=torch.float32, de
 class AmplitudeLoss(torch.autograd.Function):
     """x: complex tensor (batch, *shape) on the GPU; amp: signed amplitude in FFT order,
     shaped like one object (shared) or (batch, *shape).  Returns the per-object losses
-    (float64, shape (batch,))."""
+    (float64, shape (batch,)).  noalloc: call ``qpch_amplitude_loss_noalloc`` (scratch in the
+    library's per-GPU array, nothing allocated: the form a CUDA-graph capture needs; calls
+    on one GPU must not overlap)."""
 
     @staticmethod
-    def forward(ctx, x, amp):
+    def forward(ctx, x, amp, noalloc=False):
         if not x.is_cuda or not x.is_complex():
             raise ValueError("x must be a complex tensor on the GPU")
         if x.dtype not in (torch.complex64, torch.complex128):
@@ -72,8 +74,9 @@ class AmplitudeLoss(torch.autograd.Function):
         grad = torch.empty_like(xc)
         loss = torch.empty(x.shape[0], dtype=torch.float64, device=x.device)
         n = int(np.prod(shape))
-        _qpch.amplitude_loss(xc.data_ptr(), a.data_ptr(), 0 if shared else n, grad.data_ptr(), loss.data_ptr(),
-                             shape, x.shape[0], precision, _stream(x.device))
+        call = _qpch.amplitude_loss_noalloc if noalloc else _qpch.amplitude_loss
+        call(xc.data_ptr(), a.data_ptr(), 0 if shared else n, grad.data_ptr(), loss.data_ptr(), shape, x.shape[0],
+             precision, _stream(x.device))
         ctx.save_for_backward(grad)
         return loss
 
@@ -81,34 +84,72 @@ class AmplitudeLoss(torch.autograd.Function):
     def backward(ctx, grad_loss):
         (grad,) = ctx.saved_tensors
         g = grad_loss.to(grad.real.dtype).reshape(-1, *([1] * (grad.dim() - 1)))
-        return grad * g, None
+        return grad * g, None, None
 
 
-def amplitude_loss(x: torch.Tensor, amplitude_fft_order: torch.Tensor) -> torch.Tensor:
+def amplitude_loss(x: torch.Tensor, amplitude_fft_order: torch.Tensor, noalloc: bool = False) -> torch.Tensor:
     """Per-object amplitude losses of x (batch, *shape), differentiable in x."""
-    return AmplitudeLoss.apply(x, amplitude_fft_order)
+    return AmplitudeLoss.apply(x, amplitude_fft_order, noalloc)
 
 
-def refine(x0, intensity, support, mask=None, steps=500, lr=1e-2, device="cuda"):
+def refine(x0, intensity, support, mask=None, steps=500, lr=1e-2, device="cuda", graph=False, capturable=None,
+           fused=None):
     """Adam on the support-projected iterate: minimises L(P_S z) from z = x0.
 
     x0: (batch, *shape) complex (centre-origin); intensity: centre-origin, shared or per
-    object; support: bool, shared or per object.  Returns (P_S z, loss history (steps, batch))."""
+    object; support: bool, shared or per object.  Returns (P_S z, loss history (steps, batch)).
+
+    graph=True runs three steps eagerly, then captures one whole step (the loss through the
+    no-allocation entry point, its adjoint, the Adam update with its state on the device, and
+    the history row it writes) in a CUDA graph and replays it for the remaining steps: the
+    same kernels in the same order, without a host round of autograd and optimizer calls per
+    step.  capturable (default: graph) selects Adam's on-device step count and bias
+    correction, which a graph needs; with the same setting the two forms give the same bits.
+    fused: torch's single-kernel Adam (default: on for the graph form, whose replays it cuts
+    from about twenty kernels to a few)."""
     x0 = torch.as_tensor(np.asarray(x0), device=device)
     if not x0.is_complex():
         x0 = x0.to(torch.complex64)
     ndim = x0.dim() - 1
     amp = _to_fft_order(signed_amplitude(intensity, mask, device=device), ndim)
     s = torch.as_tensor(np.asarray(support, dtype=bool), device=device).to(x0.real.dtype)
-    z = x0.clone().requires_grad_(True)
-    opt = torch.optim.Adam([z], lr=lr)
-    history = []
-    for _ in range(steps):
-        opt.zero_grad(set_to_none=True)
-        loss = amplitude_loss(z * s, amp)
+    # The parameter is the real view of z (re, im pairs): Adam on a complex parameter
+    # updates exactly those elements, and the fused optimizer kernel takes real tensors.
+    zr = torch.view_as_real(x0.clone()).clone().requires_grad_(True)
+    cap = graph if capturable is None else capturable
+    opt = torch.optim.Adam([zr], lr=lr, capturable=cap, fused=graph if fused is None else fused)
+    history = torch.empty((steps, x0.shape[0]), dtype=torch.float64, device=device)
+
+    def step():
+        loss = amplitude_loss(torch.view_as_complex(zr) * s, amp, noalloc=graph)
         loss.sum().backward()
         opt.step()
-        history.append(loss.detach())
+        return loss.detach()
+
+    if not graph:
+        for k in range(steps):
+            opt.zero_grad(set_to_none=True)
+            history[k] = step()
+    else:
+        # eager warm-up on a side stream (Adam's state and the library's scratch address come
+        # into being here, before any capture); these are real steps of the run
+        warm = min(steps, 3)
+        side = torch.cuda.Stream(device)
+        side.wait_stream(torch.cuda.current_stream(device))
+        with torch.cuda.stream(side):
+            for k in range(warm):
+                opt.zero_grad(set_to_none=True)
+                history[k] = step()
+        torch.cuda.current_stream(device).wait_stream(side)
+        if steps > warm:
+            slot = torch.full((1,), warm, dtype=torch.long, device=device)  # history row of the next step
+            opt.zero_grad(set_to_none=True)  # the captured backward writes the gradient afresh each replay
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                history.index_copy_(0, slot, step()[None])
+                slot.add_(1)
+            for _ in range(steps - warm):
+                g.replay()
     with torch.no_grad():
-        out = z * s
-    return out, torch.stack(history)
+        out = torch.view_as_complex(zr) * s
+    return out, history

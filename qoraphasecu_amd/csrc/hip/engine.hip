@@ -794,6 +794,7 @@ struct SolverImpl {
   int xcd_mode = 0;  // 0 XCD-affine by start, 2 plain numbering, 3 plain for the modulus pass only
   int radix = 16;        // FFT radix of the line passes (8 or 16)
   bool persist = false;  // 2D chunks of iterations in one persistent launch where a kernel exists
+  DevBuf<dev::GenericPlan> generic_plans;  // twopass_generic: the plans of axes 0 and 1 in device memory
   DevBuf<int> psync;  // a line holding the abort flag, then per launch group: next[8 lines], done, count[2][batch]
   long long psync_group() const {
     return static_cast<long long>(dev::kXcds + 1 + 2 * batch) * dev::kLineInts;  // (kXcds: room for the most)
@@ -916,10 +917,18 @@ struct SolverImpl {
     return std::is_same_v<St, float2> ? QPCH_F32 : std::is_same_v<St, __half2> ? QPCH_F16 : QPCH_BF16;
   }
 
-  // Whether chunks of this storage run as one persistent launch (twopass_iterations).
+  // Whether chunks of this storage run as one persistent launch: twopass_iterations
+  // (power-of-two axes) or twopass_generic (fp32, both axes not powers of two).
   template <class St>
   bool persistent() const {
-    return persist && dims.nd == 2 && radix == 16 && twopass_supported(dims.d[0], dims.d[1], storage_code<St>());
+    if (!persist || dims.nd != 2) return false;
+    if (radix == 16 && twopass_supported(dims.d[0], dims.d[1], storage_code<St>())) return true;
+    if constexpr (std::is_same_v<St, float2>) {
+      if (generic_axis(dims.d[0]) && generic_axis(dims.d[1])) {
+        return twopass_generic_supported(generic_plan(dims.d[0]), generic_plan(dims.d[1]));
+      }
+    }
+    return false;
   }
 
   // A chunk of iterations of launch group g in one persistent launch; false where the
@@ -937,11 +946,24 @@ struct SolverImpl {
     q.done = base + dev::kXcds * dev::kLineInts;
     q.count = q.done + dev::kLineInts;
     q.abort = psync.get();
-    q.tiles0 = pass_tiles<sizeof(St)>(dims.d[0], true, 1, dims.d[1], 16);
-    q.tiles1 = twopass_row_tiles(dims.d[0], dims.d[1], storage_code<St>());
     q.xcds = persist_xcds;
     q.wait_limit = wait_ticks;
-    QPCH_HIP(twopass_launch(dims.d[0], dims.d[1], storage_code<St>(), p0, p1, q, len, persist_workers, on ? on : stream));
+    hipStream_t st = on ? on : stream;
+    if (generic_axis(dims.d[0])) {
+      if constexpr (std::is_same_v<St, float2>) {
+        const dev::GenericPlan g0 = generic_plan(dims.d[0]), g1 = generic_plan(dims.d[1]);
+        const long long lines0 = dims.d[1], lines1 = dims.d[0];  // axis 0: columns; axis 1: rows
+        q.tiles0 = static_cast<int>((lines0 + dev::generic_lines_per_block(dims.d[0]) - 1) /
+                                    dev::generic_lines_per_block(dims.d[0]));
+        q.tiles1 = static_cast<int>((lines1 + dev::generic_lines_per_block(dims.d[1]) - 1) /
+                                    dev::generic_lines_per_block(dims.d[1]));
+        QPCH_HIP(twopass_generic_launch(p0, p1, g0, g1, generic_plans.get(), lines0, lines1, q, len, persist_workers, st));
+        return true;
+      }
+    }
+    q.tiles0 = pass_tiles<sizeof(St)>(dims.d[0], true, 1, dims.d[1], 16);
+    q.tiles1 = twopass_row_tiles(dims.d[0], dims.d[1], storage_code<St>());
+    QPCH_HIP(twopass_launch(dims.d[0], dims.d[1], storage_code<St>(), p0, p1, q, len, persist_workers, st));
     return true;
   }
 
@@ -1522,6 +1544,10 @@ struct SolverImpl {
       // the abort flag and a DeviceError, never in a hang.
       if (const char* v = std::getenv("QPCH_TEST_WAIT_TICKS")) wait_ticks = std::max(0LL, std::atoll(v));
       if (const char* v = std::getenv("QPCH_TEST_PERSIST_WORKERS")) persist_workers = std::max(0, std::atoi(v));
+      if (generic_axis(dims.d[0]) && generic_axis(dims.d[1])) {
+        const dev::GenericPlan gp[2] = {generic_plan(dims.d[0]), generic_plan(dims.d[1])};
+        generic_plans = upload(gp, 2, stream);
+      }
       const long long cnt = dev::kLineInts + static_cast<long long>(groups) * psync_group();
       psync.resize(static_cast<size_t>(cnt));
       hipLaunchKernelGGL(dev::fill_int, dim3(grid_for(cnt)), dim3(256), 0, stream, psync.get(), static_cast<int>(cnt), 0);
@@ -2479,8 +2505,11 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
     // whatever the block numbering: nothing to try there.
     const bool image = d.nd == 2 && qpch::image_supported(d.d[0], d.d[1]) && problem->plane != 2;
     auto persistent_for = [&](int storage) {
-      return d.nd == 2 && problem->persist != 2 && tuned_radix == 16 && qpch::twopass_xcds() > 0 &&
-             qpch::twopass_supported(d.d[0], d.d[1], storage) && !(storage == QPCH_F32 && image);
+      if (d.nd != 2 || problem->persist == 2 || qpch::twopass_xcds() == 0) return false;
+      if (storage == QPCH_F32 && qpch::generic_axis(d.d[0]) && qpch::generic_axis(d.d[1])) {
+        return qpch::twopass_generic_supported(qpch::generic_plan(d.d[0]), qpch::generic_plan(d.d[1]));
+      }
+      return tuned_radix == 16 && qpch::twopass_supported(d.d[0], d.d[1], storage) && !(storage == QPCH_F32 && image);
     };
     const bool persistent = persistent_for(problem->storage) && (!lowp || !problem->mixed || persistent_for(QPCH_F32));
     for (int pl : planes) {

@@ -316,7 +316,7 @@ __global__ void __launch_bounds__(kGenericThreads)
     generic_fft(typename Cplx<Real>::T* data, long long n_per_start, int inner, long long lines,
                 GenericPlan plan, int forward, Real scale) {
   using C = typename Cplx<Real>::T;
-  extern __shared__ unsigned char sm_raw[];
+  extern __shared__ __align__(16) unsigned char sm_raw[];
   C* sm = reinterpret_cast<C*>(sm_raw);
   const int N = plan.n;
   const int lpb = generic_lines_per_block(N);
@@ -367,38 +367,24 @@ __host__ __device__ constexpr size_t generic_pass_lds_bytes(const GenericPlan& p
   return generic_lds_bytes<float2>(plan) + sizeof(double) * (kGenericThreads / 64);
 }
 
-// The fused passes of line_pass (passes.hpp) for an axis of any length up to
+// One tile of the fused passes of line_pass (passes.hpp) for an axis of any length up to
 // kGenericMaxN, fp32 storage: the same modes, arguments and scale conventions, with
-// generic_lines_per_block(N) lines of one start per work-group (a tile; tiles and starts
-// numbered as below).  The modulus pass leaves one FP64 error partial per work-group (its
-// waves' sums added in order); the update pass's tile-0 work-group of each start sums the
-// start's partial_stride partials in order, as line_pass does.
-template <Mode M>
-__global__ void __launch_bounds__(kGenericThreads)
-    generic_pass(PassArgs p, GenericPlan plan, long long lines) {
-  extern __shared__ unsigned char sm_raw[];
-  float2* sm = reinterpret_cast<float2*>(sm_raw);
+// generic_lines_per_block(N) lines of start `start` (launch-relative) per work-group.  The
+// modulus pass leaves one FP64 error partial per tile (its waves' sums added in order); the
+// update pass's tile 0 sums the start's partial_stride partials in order, as line_pass
+// does, into history slot `eslot` (kSlotFromCounter: *p.counter - 1).  LDS: tw (the
+// plan's forward table, filled here with OWN_TW, else already filled and visible), two
+// line buffers of generic_lines_per_block(N) x generic_slot(plan) elements, the line bases
+// and one FP64 slot per wave.  COH: inputs written by other CUs in the same launch (the
+// persistent launches) are read with device-scope loads.  Every thread of the work-group
+// must call this (the transforms hold barriers).
+template <Mode M, bool OWN_TW, bool COH = false>
+__device__ __forceinline__ void generic_tile(const PassArgs& p, const GenericPlan& plan, long long lines, int start,
+                                             int tile, float2* tw, float2* bufa, float2* bufb, long long* lbase,
+                                             double* red, int eslot) {
   const int N = plan.n;
   const int lpb = generic_lines_per_block(N);
   const int slot = generic_slot(plan);
-  float2* tw = sm;
-  float2* bufa = sm + slot;
-  float2* bufb = bufa + lpb * slot;
-  long long* lbase = reinterpret_cast<long long*>(bufb + lpb * slot);
-  double* red = reinterpret_cast<double*>(lbase + lpb);
-  // Block numbering: with p.slots > 0 (a 1D grid, slots a multiple of 8) block B runs tile
-  // B / slots of start B % slots, so all of a start's blocks share one XCD's L2 (a strided
-  // axis's blocks read adjacent pieces of the same 128-byte lines); otherwise the grid is
-  // (tiles, starts).
-  int tile, start;
-  if (p.slots > 0) {
-    start = blockIdx.x % p.slots;
-    tile = blockIdx.x / p.slots;
-    if (start >= p.starts) return;  // padding slot: the whole block leaves together
-  } else {
-    start = blockIdx.y;
-    tile = blockIdx.x;
-  }
   const long long soff = static_cast<long long>(start) * p.n;
   const long long line0 = static_cast<long long>(tile) * lpb;
   const int nl = static_cast<int>(lines - line0 < lpb ? lines - line0 : lpb);
@@ -442,19 +428,21 @@ __global__ void __launch_bounds__(kGenericThreads)
     }
     li[u] = l * N + k;
     gi[u] = lbase[l] + static_cast<long long>(k) * inner;
-    v0[u] = in[gi[u]];
+    v0[u] = Storage<float2>::ld<COH>(in, static_cast<size_t>(gi[u]));
     if constexpr (M == Mode::kModulus) amp[u] = p.amp[arow + gi[u]];
     if constexpr (M == Mode::kUpdate) {
-      xo[u] = x[gi[u]];
+      xo[u] = Storage<float2>::ld<COH>(x, static_cast<size_t>(gi[u]));
       inside[u] = sup[gi[u]] != 0;
     }
   }
-  // the twiddle table while the loads are in flight (forward W_N^j, or W_m^j of a
-  // Bluestein plan)
-  for (int j = threadIdx.x; j < slot; j += blockDim.x) {
-    double sn, cs;
-    sincospi(2.0 * static_cast<double>(j) / static_cast<double>(slot), &sn, &cs);
-    tw[j] = {static_cast<float>(cs), static_cast<float>(-sn)};
+  if constexpr (OWN_TW) {
+    // the twiddle table while the loads are in flight (forward W_N^j, or W_m^j of a
+    // Bluestein plan)
+    for (int j = threadIdx.x; j < slot; j += blockDim.x) {
+      double sn, cs;
+      sincospi(2.0 * static_cast<double>(j) / static_cast<double>(slot), &sn, &cs);
+      tw[j] = {static_cast<float>(cs), static_cast<float>(-sn)};
+    }
   }
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
@@ -505,12 +493,12 @@ __global__ void __launch_bounds__(kGenericThreads)
     __syncthreads();
     generic_transform<false>(src, dst, tw, plan, nl);
     if (tile == 0 && threadIdx.x < 64 && p.errs) {
-      const double t = partial_sum(p.partials + static_cast<long long>(start) * p.partial_stride, p.partial_stride,
-                                   threadIdx.x);
+      const double t = partial_sum<COH>(p.partials + static_cast<long long>(start) * p.partial_stride,
+                                        p.partial_stride, threadIdx.x);
       if (threadIdx.x == 0) {
-        const int slot = *p.counter - 1;
-        if (slot >= 0 && slot < p.err_cap) {
-          p.errs[static_cast<long long>(slot) * p.err_stride + p.err_offset + start] = t;
+        const int e = eslot == kSlotFromCounter ? *p.counter - 1 : eslot;
+        if (e >= 0 && e < p.err_cap) {
+          p.errs[static_cast<long long>(e) * p.err_stride + p.err_offset + start] = t;
         }
       }
     }
@@ -524,8 +512,41 @@ __global__ void __launch_bounds__(kGenericThreads)
   }
 }
 
+// The fused passes along an axis that is not a power of two: one generic_tile per
+// work-group.
+template <Mode M>
+__global__ void __launch_bounds__(kGenericThreads)
+    generic_pass(PassArgs p, GenericPlan plan, long long lines) {
+  extern __shared__ __align__(16) unsigned char sm_raw[];
+  float2* sm = reinterpret_cast<float2*>(sm_raw);
+  const int lpb = generic_lines_per_block(plan.n);
+  const int slot = generic_slot(plan);
+  float2* tw = sm;
+  float2* bufa = sm + slot;
+  float2* bufb = bufa + lpb * slot;
+  long long* lbase = reinterpret_cast<long long*>(bufb + lpb * slot);
+  double* red = reinterpret_cast<double*>(lbase + lpb);
+  // Block numbering: with p.slots > 0 (a 1D grid, slots a multiple of 8) block B runs tile
+  // B / slots of start B % slots, so all of a start's blocks share one XCD's L2 (a strided
+  // axis's blocks read adjacent pieces of the same 128-byte lines); otherwise the grid is
+  // (tiles, starts).
+  int tile, start;
+  if (p.slots > 0) {
+    start = blockIdx.x % p.slots;
+    tile = blockIdx.x / p.slots;
+    if (start >= p.starts) return;  // padding slot: the whole block leaves together
+  } else {
+    start = blockIdx.y;
+    tile = blockIdx.x;
+  }
+  generic_tile<M, true>(p, plan, lines, start, tile, tw, bufa, bufb, lbase, red, kSlotFromCounter);
+}
+
+// The non-template kernels below have internal linkage: this header is included by more
+// than one translation unit of the library, and each uses its own copy (or none).
+
 // Element-wise modulus projection over whole starts (unfused iteration).
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
     modulus_elementwise(float2* w, const float* amp, long long n, float in_scale,
                         float out_scale, double* partials, int* counter) {
   __shared__ double red[16];
@@ -545,7 +566,7 @@ __global__ void __launch_bounds__(256)
 }
 
 // Element-wise update (unfused iteration), plus the fixed-order error reduction.
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
     update_elementwise(float2* x, const float2* pm, const std::uint8_t* support, long long n,
                        float a, float b, float in_scale, const double* partials,
                        int partial_stride, double* errs, int err_stride, int err_offset,
@@ -617,7 +638,7 @@ __global__ void grad_from(const typename Cplx<Real>::T* x, const typename Cplx<R
 }
 
 // loss[s] = sum of the `count` partials of start s, in order.
-__global__ void sum_partials(const double* partials, int count, double* loss, int starts) {
+static __global__ void sum_partials(const double* partials, int count, double* loss, int starts) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= starts) return;
   double t = 0.0;

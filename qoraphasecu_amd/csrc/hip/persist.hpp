@@ -29,6 +29,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "generic.hpp"
 #include "passes.hpp"
 
 namespace qpch::dev {
@@ -96,21 +97,16 @@ struct TwoPassGeom {
   static constexpr bool kValid = G1::kThreads == kThreads && kLdsBytes <= 160 * 1024;
 };
 
-// `len` iterations of every start of a launch group on an N0 x N1 image (axis 0 = the
-// column pass, axis 1 = the row pass).  p0 / p1: the argument blocks of the two passes as
-// the two-pass schedule builds them (p0.counter unused); the history slot of iteration t is
-// *p1.counter + t, and the counter advances by len when the launch ends.
-template <int N0, int N1, class St, int R>
-__global__ void __launch_bounds__((TwoPassGeom<N0, N1, St, R>::kThreads))
-    twopass_iterations(PassArgs p0, PassArgs p1, SyncArgs q, int len) {
-  using TG = TwoPassGeom<N0, N1, St, R>;
-  static_assert(TG::kValid, "both passes must use work-groups of one size");
-  __shared__ float2 lds[TG::kTile + N0 + N1];
-  __shared__ int claim;
-  float2* tw0 = lds + TG::kTile;
-  float2* tw1 = tw0 + N0;
-  load_twiddles<N0>(tw0);
-  load_twiddles<N1>(tw1);
+// The work-item loop of a persistent launch, shared by its kernels: `col(start, tile)` runs
+// a column (modulus) tile, `row(start, tile, slot)` a row (update) tile that writes history
+// slot `slot`; both must be called by every thread of the work-group, and the twiddle
+// tables they use must be filled before the call (the loop's first barrier publishes them).
+// p1: the update pass's argument block (starts, counter); `claim`: a work-group shared int.
+// The history slot of iteration t is *p1.counter + t, and the counter advances by len
+// when the launch ends.
+template <class ColTile, class RowTile>
+__device__ __forceinline__ void persistent_loop(const PassArgs& p1, const SyncArgs& q, int len, int& claim,
+                                                ColTile&& col, RowTile&& row) {
   const int tid = threadIdx.x;
   const int S = p1.starts;
   const int xcd = __builtin_amdgcn_readfirstlane(xcd_id(q.xcds));
@@ -143,10 +139,10 @@ __global__ void __launch_bounds__((TwoPassGeom<N0, N1, St, R>::kThreads))
     if (item >= total) break;
     const int it = item / per_iter;
     int r = item - it * per_iter;
-    const bool row = r >= per0;
+    const bool is_row = r >= per0;
     int start, tile, target;
     const int* dep;
-    if (!row) {
+    if (!is_row) {
       const int j = r / q.tiles0;
       tile = r - j * q.tiles0;
       start = xcd + q.xcds * j;
@@ -162,14 +158,12 @@ __global__ void __launch_bounds__((TwoPassGeom<N0, N1, St, R>::kThreads))
       target = (it + 1) * q.tiles0;
       done_tile = count1 + start * kLineInts;
     }
-    if (tid == 0) {
-      if (target > 0) wait_count(dep, target, q.abort, q.wait_limit);
-    }
+    if (tid == 0 && target > 0) wait_count(dep, target, q.abort, q.wait_limit);
     __syncthreads();  // (also: every thread has read `claim` before it is claimed again)
-    if (!row) {
-      line_tile<N0, true, Mode::kModulus, St, R, false, true>(p0, start, tile, lds, tw0, 0);
+    if (!is_row) {
+      col(start, tile);
     } else {
-      line_tile<N1, false, Mode::kUpdate, St, R, false, true, TG::kB1>(p1, start, tile, lds, tw1, slot0 + it);
+      row(start, tile, slot0 + it);
     }
     // every wave's stores reach the L2 before thread 0 counts the tile at the top
     __builtin_amdgcn_s_waitcnt(0);
@@ -196,6 +190,82 @@ __global__ void __launch_bounds__((TwoPassGeom<N0, N1, St, R>::kThreads))
       if (p1.counter) *p1.counter = slot0 + len;
     }
   }
+}
+
+// `len` iterations of every start of a launch group on an N0 x N1 image (axis 0 = the
+// column pass, axis 1 = the row pass), power-of-two axes.  p0 / p1: the argument blocks of
+// the two passes as the two-pass schedule builds them (p0.counter unused).
+template <int N0, int N1, class St, int R>
+__global__ void __launch_bounds__((TwoPassGeom<N0, N1, St, R>::kThreads))
+    twopass_iterations(PassArgs p0, PassArgs p1, SyncArgs q, int len) {
+  using TG = TwoPassGeom<N0, N1, St, R>;
+  static_assert(TG::kValid, "both passes must use work-groups of one size");
+  __shared__ float2 lds[TG::kTile + N0 + N1];
+  __shared__ int claim;
+  float2* tw0 = lds + TG::kTile;
+  float2* tw1 = tw0 + N0;
+  load_twiddles<N0>(tw0);
+  load_twiddles<N1>(tw1);
+  persistent_loop(
+      p1, q, len, claim,
+      [&](int start, int tile) { line_tile<N0, true, Mode::kModulus, St, R, false, true>(p0, start, tile, lds, tw0, 0); },
+      [&](int start, int tile, int slot) {
+        line_tile<N1, false, Mode::kUpdate, St, R, false, true, TG::kB1>(p1, start, tile, lds, tw1, slot);
+      });
+}
+
+// Static LDS of twopass_generic: the claim word, padded to the 16-byte alignment of the
+// dynamic LDS that follows it.
+constexpr size_t kGenericStaticLds = 16;
+
+// Dynamic LDS of twopass_generic: both axes' twiddle tables, two line buffers as large as
+// either axis's tile needs, the line bases and one FP64 slot per wave.
+__host__ __device__ constexpr size_t twopass_generic_lds_bytes(const GenericPlan& a, const GenericPlan& b) {
+  const int la = generic_lines_per_block(a.n), lb = generic_lines_per_block(b.n);
+  const int buf = la * generic_slot(a) > lb * generic_slot(b) ? la * generic_slot(a) : lb * generic_slot(b);
+  return sizeof(float2) * (static_cast<size_t>(generic_slot(a)) + generic_slot(b) + 2 * static_cast<size_t>(buf)) +
+         sizeof(long long) * (la > lb ? la : lb) + sizeof(double) * (kGenericThreads / 64);
+}
+
+// The same launch for images whose two axes are both not powers of two (fp32): generic_tile
+// (generic.hpp, the per-pass kernels' body) for the tiles of both passes.  (A template, so
+// that each translation unit including this header may instantiate it.)
+// The two plans come through device memory (plans[0]: axis 0, plans[1]: axis 1): passed by
+// value, the two structures' fields kept 266 scalar registers spilled.
+template <int kThreads = kGenericThreads>
+__global__ void __launch_bounds__(kThreads)
+    twopass_generic(PassArgs p0, PassArgs p1, const GenericPlan* __restrict__ plans, long long lines0, long long lines1,
+                    SyncArgs q, int len) {
+  // 16-byte aligned: without it the dynamic LDS began 4 bytes in, after `claim`, and every
+  // float2 and double access in it was misaligned (the tiles ran 2-3 times slower)
+  extern __shared__ __align__(16) unsigned char sm_raw[];
+  __shared__ int claim;
+  const GenericPlan& plan0 = plans[0];
+  const GenericPlan& plan1 = plans[1];
+  const int s0 = generic_slot(plan0), s1 = generic_slot(plan1);
+  const int l0 = generic_lines_per_block(plan0.n), l1 = generic_lines_per_block(plan1.n);
+  const int buf = l0 * s0 > l1 * s1 ? l0 * s0 : l1 * s1;
+  float2* tw0 = reinterpret_cast<float2*>(sm_raw);
+  float2* tw1 = tw0 + s0;
+  float2* bufa = tw1 + s1;
+  float2* bufb = bufa + buf;
+  long long* lbase = reinterpret_cast<long long*>(bufb + buf);
+  double* red = reinterpret_cast<double*>(lbase + (l0 > l1 ? l0 : l1));
+  for (int j = threadIdx.x; j < s0 + s1; j += blockDim.x) {
+    const int m = j < s0 ? s0 : s1;
+    const int k = j < s0 ? j : j - s0;
+    double sn, cs;
+    sincospi(2.0 * static_cast<double>(k) / static_cast<double>(m), &sn, &cs);
+    tw0[j] = {static_cast<float>(cs), static_cast<float>(-sn)};
+  }
+  persistent_loop(
+      p1, q, len, claim,
+      [&](int start, int tile) {
+        generic_tile<Mode::kModulus, false, true>(p0, plan0, lines0, start, tile, tw0, bufa, bufb, lbase, red, 0);
+      },
+      [&](int start, int tile, int slot) {
+        generic_tile<Mode::kUpdate, false, true>(p1, plan1, lines1, start, tile, tw1, bufa, bufb, lbase, red, slot);
+      });
 }
 
 }  // namespace qpch::dev

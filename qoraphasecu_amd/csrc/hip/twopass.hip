@@ -110,4 +110,27 @@ hipError_t twopass_launch(int ny, int nx, int storage, const dev::PassArgs& p0, 
   return hipErrorInvalidValue;
 }
 
+bool twopass_generic_supported(const dev::GenericPlan& plan0, const dev::GenericPlan& plan1) {
+  return dev::twopass_generic_lds_bytes(plan0, plan1) + dev::kGenericStaticLds <= 160 * 1024;
+}
+
+hipError_t twopass_generic_launch(const dev::PassArgs& p0, const dev::PassArgs& p1, const dev::GenericPlan& plan0,
+                                  const dev::GenericPlan& plan1, const dev::GenericPlan* plans_dev, long long lines0,
+                                  long long lines1, const dev::SyncArgs& q, int len, int workers, hipStream_t s) {
+  const size_t lds = dev::twopass_generic_lds_bytes(plan0, plan1);
+  if (lds + dev::kGenericStaticLds > 160 * 1024) return hipErrorInvalidValue;
+  int per_cu = 0, dev = 0, cus = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::twopass_generic<>, dev::kGenericThreads,
+                                                              lds);
+  if (e == hipSuccess) e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  const int cap = std::max(1, per_cu) * cus / q.xcds * q.xcds;
+  const int items = q.xcds * dev::xcd_starts(p1.starts, 0, q.xcds) * std::max(q.tiles0, q.tiles1);
+  const int w = workers > 0 ? workers : std::max(q.xcds, std::min(items, cap));
+  hipLaunchKernelGGL(dev::twopass_generic<>, dim3(w), dim3(dev::kGenericThreads), lds, s, p0, p1, plans_dev, lines0,
+                     lines1, q, len);
+  return hipGetLastError();
+}
+
 }  // namespace qpch

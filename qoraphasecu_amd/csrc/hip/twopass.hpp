@@ -27,4 +27,13 @@ int twopass_xcds();
 hipError_t twopass_launch(int ny, int nx, int storage, const dev::PassArgs& p0, const dev::PassArgs& p1,
                           const dev::SyncArgs& q, int len, int workers, hipStream_t s);
 
+// The persistent launch for 2D images whose two axes are both not powers of two (fp32):
+// whether its LDS (both plans' tables and tiles) fits a work-group, and the launch.
+bool twopass_generic_supported(const dev::GenericPlan& plan0, const dev::GenericPlan& plan1);
+// plans_dev: the two plans in device memory (axis 0, then axis 1); plan0 / plan1: the same
+// plans on the host (for the LDS size).
+hipError_t twopass_generic_launch(const dev::PassArgs& p0, const dev::PassArgs& p1, const dev::GenericPlan& plan0,
+                                  const dev::GenericPlan& plan1, const dev::GenericPlan* plans_dev, long long lines0,
+                                  long long lines1, const dev::SyncArgs& q, int len, int workers, hipStream_t s);
+
 }  // namespace qpch

@@ -16,6 +16,9 @@
 namespace qpch::dev {
 
 constexpr int kGenericMaxN = 2048;
+// Every generic kernel is launched with exactly kGenericThreads threads, and the
+// work-group loops stride by this constant: blockDim.x is a vector load from the launch's
+// implicit arguments, and before each stage loop it cost a wait on all outstanding loads.
 constexpr int kGenericThreads = 256;
 // generic_pass keeps a block's elements (at most 2048: one line up to kGenericMaxN, or up
 // to 16 lines of fewer than 1024) in 2048 / kGenericThreads registers per thread.
@@ -164,7 +167,7 @@ __device__ __forceinline__ void generic_stage(const C* src, C* dst, const C* tw,
   const int rr = R > 0 ? R : r;
   const int nr = N / rr;
   const int m = N / (L * rr);
-  for (int t = threadIdx.x; t < nl * nr; t += blockDim.x) {
+  for (int t = threadIdx.x; t < nl * nr; t += kGenericThreads) {
     const int l = small_div(t, nr);
     const int tl = t - l * nr;
     const int k = tl - small_div(tl, L) * L;
@@ -172,12 +175,18 @@ __device__ __forceinline__ void generic_stage(const C* src, C* dst, const C* tw,
     const C* sl = src + l * N;
     C* dl = dst + l * N;
     if constexpr (R > 0) {
-      C v[R];
+      // every input and twiddle read issued before the first use: no branch around the
+      // twiddle reads (entry 0 is always there), the product selected where tq != 0
+      C v[R], w[R];
 #pragma unroll
       for (int q = 0; q < R; ++q) {
         v[q] = sl[tl + q * nr];
-        const int tq = q * k * m;  // stage twiddle W_{L r}^{q k}
-        if (tq) v[q] = gmul(v[q], twid<INV>(tw, tq));
+        if (q > 0) w[q] = twid<INV>(tw, q * k * m);  // stage twiddle W_{L r}^{q k}
+      }
+#pragma unroll
+      for (int q = 1; q < R; ++q) {
+        const C t = gmul(v[q], w[q]);
+        if (k != 0) v[q] = t;  // tq = q k m is zero exactly when k is
       }
       dft_small<R, INV>(v);
 #pragma unroll
@@ -209,7 +218,7 @@ __device__ __forceinline__ void bluestein(C*& src, C*& dst, const C* tw, const G
   const C* chirp = static_cast<const C*>(plan.chirp);
   const C* bhat = static_cast<const C*>(plan.bhat);
   // a_j = x_j c_j, zero-padded to m
-  for (int t = threadIdx.x; t < nl * Mm; t += blockDim.x) {
+  for (int t = threadIdx.x; t < nl * Mm; t += kGenericThreads) {
     const int l = small_div(t, Mm);
     const int j = t - l * Mm;
     dst[t] = j < N ? gmul(src[l * N + j], twid<INV>(chirp, j)) : C{0, 0};
@@ -235,7 +244,7 @@ __device__ __forceinline__ void bluestein(C*& src, C*& dst, const C* tw, const G
     }
   };
   stages(std::false_type{});
-  for (int t = threadIdx.x; t < nl * Mm; t += blockDim.x) {
+  for (int t = threadIdx.x; t < nl * Mm; t += kGenericThreads) {
     const int j = t - small_div(t, Mm) * Mm;
     cur[t] = gmul(cur[t], twid<INV>(bhat, j));
   }
@@ -244,7 +253,7 @@ __device__ __forceinline__ void bluestein(C*& src, C*& dst, const C* tw, const G
   // X_k = c_k conv_k / m (1/m is a power of two: exact)
   using R = decltype(C{}.x);
   const R inv_m = R(1) / static_cast<R>(Mm);
-  for (int t = threadIdx.x; t < nl * N; t += blockDim.x) {
+  for (int t = threadIdx.x; t < nl * N; t += kGenericThreads) {
     const int l = small_div(t, N);
     const int k = t - l * N;
     const C v = gmul(cur[l * Mm + k], twid<INV>(chirp, k));
@@ -266,7 +275,7 @@ __device__ __forceinline__ void generic_transform(C*& src, C*& dst, const C* tw,
     return;
   }
   if (plan.nfac == 0) {
-    for (int t = threadIdx.x; t < nl * N; t += blockDim.x) {
+    for (int t = threadIdx.x; t < nl * N; t += kGenericThreads) {
       const int l = small_div(t, N);
       const int j = t - l * N;
       const C* sl = src + l * N;
@@ -327,7 +336,7 @@ __global__ void __launch_bounds__(kGenericThreads)
   long long* lbase = reinterpret_cast<long long*>(bufb + lpb * slot);
   // the forward table (W_N, or W_m of a Bluestein plan); an inverse transform conjugates
   // its entries as it reads them (the stages' template argument carries the direction)
-  for (int j = threadIdx.x; j < slot; j += blockDim.x) {
+  for (int j = threadIdx.x; j < slot; j += kGenericThreads) {
     double s, c;
     sincospi(2.0 * static_cast<double>(j) / static_cast<double>(slot), &s, &c);
     tw[j] = {static_cast<Real>(c), static_cast<Real>(-s)};
@@ -340,7 +349,7 @@ __global__ void __launch_bounds__(kGenericThreads)
   }
   __syncthreads();
   const bool strided = inner > 1;
-  for (int e = threadIdx.x; e < nl * N; e += blockDim.x) {
+  for (int e = threadIdx.x; e < nl * N; e += kGenericThreads) {
     const int l = strided ? e - small_div(e, nl) * nl : small_div(e, N);
     const int k = strided ? small_div(e, nl) : e - small_div(e, N) * N;
     bufa[l * N + k] = data[lbase[l] + static_cast<long long>(k) * inner];
@@ -353,7 +362,7 @@ __global__ void __launch_bounds__(kGenericThreads)
   } else {
     generic_transform<true>(src, dst, tw, plan, nl);
   }
-  for (int e = threadIdx.x; e < nl * N; e += blockDim.x) {
+  for (int e = threadIdx.x; e < nl * N; e += kGenericThreads) {
     const int l = strided ? e - small_div(e, nl) * nl : small_div(e, N);
     const int k = strided ? small_div(e, nl) : e - small_div(e, N) * N;
     const C v = src[l * N + k];
@@ -438,7 +447,7 @@ __device__ __forceinline__ void generic_tile(const PassArgs& p, const GenericPla
   if constexpr (OWN_TW) {
     // the twiddle table while the loads are in flight (forward W_N^j, or W_m^j of a
     // Bluestein plan)
-    for (int j = threadIdx.x; j < slot; j += blockDim.x) {
+    for (int j = threadIdx.x; j < slot; j += kGenericThreads) {
       double sn, cs;
       sincospi(2.0 * static_cast<double>(j) / static_cast<double>(slot), &sn, &cs);
       tw[j] = {static_cast<float>(cs), static_cast<float>(-sn)};
@@ -472,7 +481,7 @@ __device__ __forceinline__ void generic_tile(const PassArgs& p, const GenericPla
     __syncthreads();
     if (threadIdx.x == 0) {
       double t = 0.0;
-      for (int v = 0; v < static_cast<int>(blockDim.x) / 64; ++v) t += red[v];
+      for (int v = 0; v < kGenericThreads / 64; ++v) t += red[v];
       p.partials[static_cast<long long>(start) * p.partial_stride + tile] = t;
     }
     generic_transform<true>(src, dst, tw, plan, nl);

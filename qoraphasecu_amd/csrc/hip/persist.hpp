@@ -251,7 +251,7 @@ __global__ void __launch_bounds__(kThreads)
   float2* bufb = bufa + buf;
   long long* lbase = reinterpret_cast<long long*>(bufb + buf);
   double* red = reinterpret_cast<double*>(lbase + (l0 > l1 ? l0 : l1));
-  for (int j = threadIdx.x; j < s0 + s1; j += blockDim.x) {
+  for (int j = threadIdx.x; j < s0 + s1; j += kThreads) {
     const int m = j < s0 ? s0 : s1;
     const int k = j < s0 ? j : j - s0;
     double sn, cs;

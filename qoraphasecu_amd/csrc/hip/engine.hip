@@ -794,6 +794,9 @@ struct SolverImpl {
   int xcd_mode = 0;  // 0 XCD-affine by start, 2 plain numbering, 3 plain for the modulus pass only
   int radix = 16;        // FFT radix of the line passes (8 or 16)
   bool persist = false;  // 2D chunks of iterations in one persistent launch where a kernel exists
+  // Images whose two axes are not powers of two run persistent launches only when asked
+  // (qpch_problem.persist = 1): at 240^2 the per-pass launches are the faster default.
+  bool persist_generic = false;
   DevBuf<dev::GenericPlan> generic_plans;  // twopass_generic: the plans of axes 0 and 1 in device memory
   DevBuf<int> psync;  // a line holding the abort flag, then per launch group: next[8 lines], done, count[2][batch]
   long long psync_group() const {
@@ -918,13 +921,13 @@ struct SolverImpl {
   }
 
   // Whether chunks of this storage run as one persistent launch: twopass_iterations
-  // (power-of-two axes) or twopass_generic (fp32, both axes not powers of two).
+  // (power-of-two axes) or twopass_generic (fp32, both axes not powers of two, persist = 1).
   template <class St>
   bool persistent() const {
     if (!persist || dims.nd != 2) return false;
     if (radix == 16 && twopass_supported(dims.d[0], dims.d[1], storage_code<St>())) return true;
     if constexpr (std::is_same_v<St, float2>) {
-      if (generic_axis(dims.d[0]) && generic_axis(dims.d[1])) {
+      if (persist_generic && generic_axis(dims.d[0]) && generic_axis(dims.d[1])) {
         return twopass_generic_supported(generic_plan(dims.d[0]), generic_plan(dims.d[1]));
       }
     }
@@ -1534,6 +1537,7 @@ struct SolverImpl {
     counters.resize(2 * static_cast<size_t>(groups));  // slot counters, then image_iterations' tickets
     persist_xcds = twopass_xcds();
     persist = fused && dims.nd == 2 && !resident && p.persist != 2 && persist_xcds > 0;
+    persist_generic = persist && p.persist == 1;
     if (persist) {
       int dev = 0, khz = 0;
       QPCH_HIP(hipGetDevice(&dev));
@@ -2481,12 +2485,15 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
       (void)hipEventDestroy(e1);
       return us;
     };
+    // images whose two axes are not powers of two: persistent launches only with persist = 1
+    const bool generic2d = d.nd == 2 && qpch::generic_axis(d.d[0]) && qpch::generic_axis(d.d[1]);
     auto trial = [&](const qpch_tuning& t) {
       qpch_tuning r = t;
       r.us_per_iteration = measure(t);
       line += " [batch " + std::to_string(t.batch) + (t.lanes > 1 ? " x" + std::to_string(t.lanes) + " lanes" : "") +
               (t.plane ? (t.plane == 1 ? " plane" : " lines") : "") + (t.xcd == 2 ? " xcd-off" : t.xcd == 3 ? " xcd-off-modulus" : "") +
-              (t.radix == 8 ? " radix-8" : "") + (t.persist == 2 ? " pass-launches" : "") + ": " +
+              (t.radix == 8 ? " radix-8" : "") + (t.persist == 2 ? " pass-launches" : "") +
+              (t.persist == 1 && generic2d ? " persistent" : "") + ": " +
               std::to_string(r.us_per_iteration) + " us/it]";
       // A candidate must beat the incumbent by 2%: timings a few tenths of a percent apart
       // are noise, and the first candidate is the default (all starts in one launch, one
@@ -2511,7 +2518,10 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
       }
       return tuned_radix == 16 && qpch::twopass_supported(d.d[0], d.d[1], storage) && !(storage == QPCH_F32 && image);
     };
-    const bool persistent = persistent_for(problem->storage) && (!lowp || !problem->mixed || persistent_for(QPCH_F32));
+    const bool can_persist =
+        persistent_for(problem->storage) && (!lowp || !problem->mixed || persistent_for(QPCH_F32));
+    // whether the default candidates (problem->persist) already run persistent launches
+    const bool persistent = can_persist && (!generic2d || problem->persist == 1);
     for (int pl : planes) {
       for (int xcd : {0, 2, 3}) {
         if (xcd != 0 && persistent) continue;
@@ -2527,7 +2537,9 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
     // but pays a queue claim and a wait per tile, which loses where a launch already holds
     // many tiles per CU (512^2 x 64 fp16: 290 against 267 us per iteration).  When the
     // per-pass launches win, their XCD numbering is tried too.
-    if (persistent && problem->persist == 0) {
+    if (can_persist && generic2d && problem->persist == 0) {
+      trial(qpch_tuning{best.batch, best.plane, 0, 0.0, best.lanes, best.radix, 1});
+    } else if (persistent && problem->persist == 0) {
       const qpch_tuning keep = best;
       for (int xcd : {0, 2, 3}) {
         if (xcd != 0 && !(best.batch >= 8 && best.batch % 8 == 0)) continue;
@@ -2537,7 +2549,8 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
     }
     line += " -> batch " + std::to_string(best.batch) + " lanes " + std::to_string(best.lanes) + " plane " +
             std::to_string(best.plane) + " xcd " + std::to_string(best.xcd) + " radix " + std::to_string(best.radix) +
-            (best.persist == 2 ? " pass-launches" : "") + "\n";
+            (best.persist == 2 ? " pass-launches" : "") + (best.persist == 1 && generic2d ? " persistent" : "") +
+            "\n";
     std::lock_guard<std::mutex> lock(g_tune_mu);
     g_tune_cache[key] = best;
     g_tune_log += line;

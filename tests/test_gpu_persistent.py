@@ -51,7 +51,7 @@ CASES = [
 @pytest.mark.parametrize("shape,kw", CASES)
 def test_persistent_equals_pass_launches(qpch, shape, kw):
     a = _solver(qpch, shape, 2, **kw)
-    b = _solver(qpch, shape, 0, **kw)
+    b = _solver(qpch, shape, 1, **kw)  # 1: the generic shapes' persistent launches too
     for rep in range(2):  # the graphs captured, then replayed
         want = _stepwise(a, RECIPE)
         got = _stepwise(b, RECIPE)

@@ -62,7 +62,7 @@ def core_ext() -> str:
     return os.path.join(HERE, "_core" + sysconfig.get_config_var("EXT_SUFFIX"))
 
 
-ENGINE_SOURCES = ["engine.hip", "twopass.hip"]
+ENGINE_SOURCES = ["engine.hip", "twopass.hip", "smooth.hip"]
 
 
 def build_engine() -> str:

@@ -26,6 +26,7 @@
 #include "misc.hpp"
 #include "passes.hpp"
 #include "plane.hpp"
+#include "smooth_pass.hpp"
 #include "twopass.hpp"
 #include "blur.hpp"
 
@@ -579,7 +580,6 @@ void launch_generic(float2* data, long long n_per_start, int batch, int N, long 
 // A fused pass (line_pass semantics) along an axis of any length up to kGenericMaxN, fp32.
 void launch_generic_pass(int N, Mode m, PassArgs a, int starts, hipStream_t s) {
   if (N > dev::kGenericMaxN) throw InvalidArgument("no generic pass for axis length " + std::to_string(N));
-  const dev::GenericPlan plan = generic_plan(N);
   const long long lines = static_cast<long long>(a.outer) * a.inner;
   const long long lpb = dev::generic_lines_per_block(N);
   const unsigned tiles = static_cast<unsigned>((lines + lpb - 1) / lpb);
@@ -591,6 +591,14 @@ void launch_generic_pass(int N, Mode m, PassArgs a, int starts, hipStream_t s) {
   a.starts = starts;
   a.slots = (a.inner > 1 && !a.no_xcd) ? xcd_slots(starts) : 0;
   const dim3 grid = a.slots > 0 ? dim3(tiles * a.slots) : dim3(tiles, starts);
+  // Smooth lengths (N = R1 R2, R1 a power of two up to 16, R2 in {3, 5, 15}) run the
+  // iteration passes with the line in registers (smooth.hpp): same tiles, partials and
+  // numbering
+  if ((m == Mode::kModulus || m == Mode::kUpdate) && smooth_axis(N)) {
+    QPCH_HIP(smooth_launch(N, a.inner > 1, m, a, lines, grid, s));
+    return;
+  }
+  const dev::GenericPlan plan = generic_plan(N);
   const size_t lds = dev::generic_pass_lds_bytes(plan);
   check_lds(lds);
   switch (m) {
@@ -927,7 +935,8 @@ struct SolverImpl {
     if (!persist || dims.nd != 2) return false;
     if (radix == 16 && twopass_supported(dims.d[0], dims.d[1], storage_code<St>())) return true;
     if constexpr (std::is_same_v<St, float2>) {
-      if (persist_generic && generic_axis(dims.d[0]) && generic_axis(dims.d[1])) {
+      if (persist_generic && generic_axis(dims.d[0]) && generic_axis(dims.d[1]) && !smooth_axis(dims.d[0]) &&
+          !smooth_axis(dims.d[1])) {
         return twopass_generic_supported(generic_plan(dims.d[0]), generic_plan(dims.d[1]));
       }
     }
@@ -2514,7 +2523,9 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
     auto persistent_for = [&](int storage) {
       if (d.nd != 2 || problem->persist == 2 || qpch::twopass_xcds() == 0) return false;
       if (storage == QPCH_F32 && qpch::generic_axis(d.d[0]) && qpch::generic_axis(d.d[1])) {
-        return qpch::twopass_generic_supported(qpch::generic_plan(d.d[0]), qpch::generic_plan(d.d[1]));
+        // (smooth axes run smooth_pass, which the persistent kernel does not)
+        return !qpch::smooth_axis(d.d[0]) && !qpch::smooth_axis(d.d[1]) &&
+               qpch::twopass_generic_supported(qpch::generic_plan(d.d[0]), qpch::generic_plan(d.d[1]));
       }
       return tuned_radix == 16 && qpch::twopass_supported(d.d[0], d.d[1], storage) && !(storage == QPCH_F32 && image);
     };

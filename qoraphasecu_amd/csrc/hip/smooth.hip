@@ -1,0 +1,70 @@
+// The smooth-length passes (smooth.hpp), instantiated per factor pair, axis kind and mode.
+#include "smooth_pass.hpp"
+
+#include "smooth.hpp"
+
+namespace qpch {
+namespace {
+
+// The factor pairs: R1 in {2, 4, 8, 16} times R2 in {3, 5, 15}.
+#define QPCH_FOR_EACH_SMOOTH(X) \
+  X(2, 3) X(2, 5) X(2, 15) X(4, 3) X(4, 5) X(4, 15) X(8, 3) X(8, 5) X(8, 15) X(16, 3) X(16, 5) X(16, 15)
+
+bool factors(int N, int& r1, int& r2) {
+  for (int r : {15, 5, 3}) {
+    if (N % r) continue;
+    const int q = N / r;
+    if (q == 2 || q == 4 || q == 8 || q == 16) {
+      r1 = q;
+      r2 = r;
+      return true;
+    }
+  }
+  return false;
+}
+
+template <int R1, int R2, bool COL, dev::Mode M>
+void launch(const dev::PassArgs& a, long long lines, dim3 grid, hipStream_t s) {
+  hipLaunchKernelGGL((dev::smooth_pass<R1, R2, COL, M>), grid, dim3(dev::SmoothGeom<R1, R2, COL>::kThreads), 0, s,
+                     a, lines);
+}
+
+template <int R1, int R2>
+void launch_pair(bool col, dev::Mode m, const dev::PassArgs& a, long long lines, dim3 grid, hipStream_t s) {
+  if (col) {
+    if (m == dev::Mode::kModulus) {
+      launch<R1, R2, true, dev::Mode::kModulus>(a, lines, grid, s);
+    } else {
+      launch<R1, R2, true, dev::Mode::kUpdate>(a, lines, grid, s);
+    }
+  } else {
+    if (m == dev::Mode::kModulus) {
+      launch<R1, R2, false, dev::Mode::kModulus>(a, lines, grid, s);
+    } else {
+      launch<R1, R2, false, dev::Mode::kUpdate>(a, lines, grid, s);
+    }
+  }
+}
+
+}  // namespace
+
+bool smooth_axis(int N) {
+  int r1 = 0, r2 = 0;
+  return factors(N, r1, r2);
+}
+
+hipError_t smooth_launch(int N, bool col, dev::Mode m, const dev::PassArgs& a, long long lines, dim3 grid,
+                         hipStream_t s) {
+  int r1 = 0, r2 = 0;
+  if (!factors(N, r1, r2) || (m != dev::Mode::kModulus && m != dev::Mode::kUpdate)) return hipErrorInvalidValue;
+#define QPCH_SMOOTH_CASE(A, B) \
+  if (r1 == A && r2 == B) {    \
+    launch_pair<A, B>(col, m, a, lines, grid, s); \
+    return hipGetLastError();  \
+  }
+  QPCH_FOR_EACH_SMOOTH(QPCH_SMOOTH_CASE)
+#undef QPCH_SMOOTH_CASE
+  return hipErrorInvalidValue;
+}
+
+}  // namespace qpch

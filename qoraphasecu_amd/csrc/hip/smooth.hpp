@@ -1,0 +1,278 @@
+// Fused passes along axes of smooth length N = R1 * R2 (R1 a power of two up to 16, R2 in
+// {3, 5, 15}), with the line held in registers: one in-register radix-R1 stage, one LDS
+// exchange, one in-register radix-R2 stage, where generic_pass (generic.hpp) runs four to
+// six radix <= 5 stages through LDS with a barrier each (240 = 16 * 15: two stages
+// instead of four each way).
+//
+// Register layouts of a line (thread i of the line's T threads):
+//   A: thread i < R2 holds x[i + m R2], m < R1
+//   B: thread i < R1 holds x[i + m R1], m < R2
+// The forward transform takes A to B: the radix-R1 DFT of A's registers (stage 1 of the
+// autosort form, L = 1) goes to LDS at i R1 + q; thread j then reads j + q R1, twiddles
+// input q by W_N^(q j) and takes the radix-R2 DFT, whose outputs are the thread's own
+// layout-B positions (L R2 = N).  The inverse takes B to A the same way with the radices
+// in the other order.  So a modulus pass loads and stores in layout A (projection in B)
+// and an update pass loads the wave in B and the iterate in A (update in A).
+//
+// Radix 15 is the prime-factor (Good-Thomas) form of 3 x 5: input n1, n2 at
+// (5 n1 + 3 n2) mod 15, five radix-3 and three radix-5 short forms without twiddles,
+// output k1, k2 at (10 k1 + 6 k2) mod 15 (10 = 5 * (5^-1 mod 3), 6 = 3 * (3^-1 mod 5)).
+//
+// The pass semantics are generic_tile's (lines, scales, one FP64 error partial per
+// work-group, the fixed-order sum in the update pass, block numbering), so the two kernels
+// are interchangeable per axis; the arithmetic (the transform's rounding) is this
+// kernel's own.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fft_engine.hpp"
+#include "generic.hpp"
+#include "passes.hpp"
+
+namespace qpch::dev {
+
+__host__ __device__ constexpr int smooth_pow2_ceil(int v) { return v <= 1 ? 1 : 2 * smooth_pow2_ceil((v + 1) / 2); }
+
+template <int R1, int R2, bool COL>
+struct SmoothGeom {
+  static_assert(R1 == 2 || R1 == 4 || R1 == 8 || R1 == 16, "R1: a power of two up to 16");
+  static_assert(R2 == 3 || R2 == 5 || R2 == 15, "R2: 3, 5 or 15");
+  static constexpr int kN = R1 * R2;
+  static constexpr int kV = R1 > R2 ? R1 : R2;                 // registers per thread
+  static constexpr int kT = smooth_pow2_ceil(kV);              // threads per line
+  static constexpr int kB = generic_lines_per_block(kN);       // lines per work-group (as generic_pass)
+  static constexpr int kThreads = kB * kT;
+  static constexpr int kWaves = (kThreads + 63) / 64;
+  static constexpr int kLine = kN + kN / R1;                   // one pad slot per R1 elements
+  static constexpr int kStride = COL ? (kLine | 1) : kLine;    // odd for column tiles (banks)
+  static_assert(kThreads % 64 == 0, "whole waves (the error partial's shuffles)");
+};
+
+// LDS position of line element p: stage-1 writes of the forward transform go to i R1 + q,
+// a stride of R1 across the lanes; the pad makes it R1 + 1.
+template <int R1>
+__device__ __forceinline__ int smooth_pos(int p) {
+  return p + p / R1;
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft15(float2* v) {
+  float2 u[3][5];
+#pragma unroll
+  for (int n1 = 0; n1 < 3; ++n1) {
+#pragma unroll
+    for (int n2 = 0; n2 < 5; ++n2) u[n1][n2] = v[(5 * n1 + 3 * n2) % 15];
+  }
+#pragma unroll
+  for (int n2 = 0; n2 < 5; ++n2) {
+    float2 t[3] = {u[0][n2], u[1][n2], u[2][n2]};
+    dft_small<3, INV>(t);
+#pragma unroll
+    for (int k1 = 0; k1 < 3; ++k1) u[k1][n2] = t[k1];
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < 3; ++k1) dft_small<5, INV>(u[k1]);
+#pragma unroll
+  for (int k1 = 0; k1 < 3; ++k1) {
+#pragma unroll
+    for (int k2 = 0; k2 < 5; ++k2) v[(10 * k1 + 6 * k2) % 15] = u[k1][k2];
+  }
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft_odd(float2* v) {
+  if constexpr (R == 15) {
+    dft15<INV>(v);
+  } else {
+    dft_small<R, INV>(v);
+  }
+}
+
+// Layout A -> layout B (forward).  inA / inB: whether this thread holds a line position in
+// layout A / B.  tw: W_N^j, j < N (forward table).  Every thread of the work-group calls it.
+template <int R1, int R2>
+__device__ __forceinline__ void smooth_fwd(float2* v, int i, bool inA, bool inB, float2* line, const float2* tw) {
+  if (inA) {
+    dft_regs<R1, true>(v);
+#pragma unroll
+    for (int q = 0; q < R1; ++q) line[smooth_pos<R1>(i * R1 + q)] = v[q];
+  }
+  __syncthreads();
+  if (inB) {
+    float2 w[R2];
+#pragma unroll
+    for (int q = 0; q < R2; ++q) {
+      v[q] = line[smooth_pos<R1>(i + q * R1)];
+      w[q] = tw[q * i];
+    }
+#pragma unroll
+    for (int q = 1; q < R2; ++q) {
+      const float2 t = cmul(v[q], w[q]);
+      if (i != 0) v[q] = t;  // W^0 = 1: the input stays as it is
+    }
+    dft_odd<R2, false>(v);
+  }
+  __syncthreads();
+}
+
+// Layout B -> layout A (inverse, unnormalised).
+template <int R1, int R2>
+__device__ __forceinline__ void smooth_inv(float2* v, int i, bool inA, bool inB, float2* line, const float2* tw) {
+  if (inB) {
+    dft_odd<R2, true>(v);
+#pragma unroll
+    for (int q = 0; q < R2; ++q) line[smooth_pos<R1>(i * R2 + q)] = v[q];
+  }
+  __syncthreads();
+  if (inA) {
+    float2 w[R1];
+#pragma unroll
+    for (int q = 0; q < R1; ++q) {
+      v[q] = line[smooth_pos<R1>(i + q * R2)];
+      w[q] = tw[q * i];
+    }
+#pragma unroll
+    for (int q = 1; q < R1; ++q) {
+      const float2 t = cmulc(v[q], w[q]);
+      if (i != 0) v[q] = t;
+    }
+    dft_regs<R1, false>(v);
+  }
+  __syncthreads();
+}
+
+// The modulus or update pass over one tile (kB lines) of start `start`: generic_tile's
+// semantics for these two modes (fp32).  lds: kB line slots of kStride float2, then the
+// twiddle table (kN float2), then one double per wave.
+template <int R1, int R2, bool COL, Mode M>
+__device__ __forceinline__ void smooth_tile(const PassArgs& p, long long lines, int start, int tile, float2* lds) {
+  static_assert(M == Mode::kModulus || M == Mode::kUpdate, "the iteration passes only");
+  using G = SmoothGeom<R1, R2, COL>;
+  constexpr int N = G::kN;
+  float2* tw = lds + G::kB * G::kStride;
+  double* red = reinterpret_cast<double*>(tw + N);
+  const int tid = threadIdx.x;
+  int b, i;
+  if constexpr (COL) {
+    b = tid % G::kB;
+    i = tid / G::kB;
+  } else {
+    i = tid % G::kT;
+    b = tid / G::kT;
+  }
+  const long long L = static_cast<long long>(tile) * G::kB + b;
+  const bool valid = L < lines;
+  const bool inA = i < R2, inB = i < R1;
+  // Loads are unconditional (all in flight at once): lanes past the last line read line 0
+  // of the start, idle lanes of a layout read its thread 0's positions; neither stores.
+  const long long Lc = valid ? L : 0;
+  const int inner = p.inner;
+  const long long base = (Lc / inner) * N * inner + Lc % inner;  // element 0 of the line within the start
+  const int ia = inA ? i : 0, ib = inB ? i : 0;
+  const long long soff = static_cast<long long>(start) * p.n;
+  float2* w = static_cast<float2*>(p.w) + soff;
+  float2* x = static_cast<float2*>(p.x) + soff;
+  float2* line = lds + b * G::kStride;
+  float2 v[G::kV];
+  if constexpr (M == Mode::kModulus) {
+    const long long arow = p.amp_stride * ((p.err_offset + start) / (p.amp_group > 1 ? p.amp_group : 1));
+    float amp[R2];
+#pragma unroll
+    for (int m = 0; m < R1; ++m) v[m] = w[base + static_cast<long long>(ia + m * R2) * inner];
+#pragma unroll
+    for (int m = 0; m < R2; ++m) amp[m] = p.amp[arow + base + static_cast<long long>(ib + m * R1) * inner];
+    for (int j = tid; j < N; j += G::kThreads) {  // the table while the loads are in flight
+      double sn, cs;
+      sincospi(2.0 * static_cast<double>(j) / static_cast<double>(N), &sn, &cs);
+      tw[j] = {static_cast<float>(cs), static_cast<float>(-sn)};
+    }
+    __syncthreads();
+    smooth_fwd<R1, R2>(v, i, inA, inB, line, tw);
+    double acc = 0.0;
+    if (valid && inB) {
+      const float io = p.in_scale * p.out_scale;
+#pragma unroll
+      for (int m = 0; m < R2; ++m) v[m] = project_scaled(v[m], amp[m], p.in_scale, io, acc);
+    }
+    const double ws = wave_sum(acc);
+    if ((tid & 63) == 0) red[tid >> 6] = ws;
+    if (tid == 0 && tile == 0 && start == 0 && p.counter) atomicAdd(p.counter, 1);
+    smooth_inv<R1, R2>(v, i, inA, inB, line, tw);  // its barriers also publish red
+    if (tid == 0) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < G::kWaves; ++k) t += red[k];
+      p.partials[static_cast<long long>(start) * p.partial_stride + tile] = t;
+    }
+    if (valid && inA) {
+#pragma unroll
+      for (int m = 0; m < R1; ++m) w[base + static_cast<long long>(i + m * R2) * inner] = v[m];  // out_scale in io
+    }
+  } else {
+    const std::uint8_t* sup = p.support + soff;
+    float2 xo[R1];
+    bool inside[R1];
+#pragma unroll
+    for (int m = 0; m < R2; ++m) v[m] = w[base + static_cast<long long>(ib + m * R1) * inner];
+#pragma unroll
+    for (int m = 0; m < R1; ++m) {
+      const long long e = base + static_cast<long long>(ia + m * R2) * inner;
+      xo[m] = x[e];
+      inside[m] = sup[e] != 0;
+    }
+    for (int j = tid; j < N; j += G::kThreads) {
+      double sn, cs;
+      sincospi(2.0 * static_cast<double>(j) / static_cast<double>(N), &sn, &cs);
+      tw[j] = {static_cast<float>(cs), static_cast<float>(-sn)};
+    }
+    __syncthreads();
+    smooth_inv<R1, R2>(v, i, inA, inB, line, tw);
+    if (valid && inA) {
+#pragma unroll
+      for (int m = 0; m < R1; ++m) {
+        const float2 pm = {v[m].x * p.in_scale, v[m].y * p.in_scale};
+        float2 nx = pm;
+        if (!inside[m]) nx = {p.a * xo[m].x + p.b * pm.x, p.a * xo[m].y + p.b * pm.y};
+        x[base + static_cast<long long>(i + m * R2) * inner] = nx;
+        v[m] = nx;
+      }
+    }
+    smooth_fwd<R1, R2>(v, i, inA, inB, line, tw);
+    if (valid && inB) {
+#pragma unroll
+      for (int m = 0; m < R2; ++m) {
+        w[base + static_cast<long long>(i + m * R1) * inner] = {v[m].x * p.out_scale, v[m].y * p.out_scale};
+      }
+    }
+    if (tile == 0 && tid < 64 && p.errs) {
+      // fixed-order sum of the previous modulus pass's partials of this start
+      const double t = partial_sum<false>(p.partials + static_cast<long long>(start) * p.partial_stride,
+                                          p.partial_stride, tid);
+      if (tid == 0) {
+        const int e = *p.counter - 1;
+        if (e >= 0 && e < p.err_cap) p.errs[static_cast<long long>(e) * p.err_stride + p.err_offset + start] = t;
+      }
+    }
+  }
+}
+
+// One smooth_tile per work-group; block numbering as generic_pass's.
+template <int R1, int R2, bool COL, Mode M>
+__global__ void __launch_bounds__((SmoothGeom<R1, R2, COL>::kThreads))
+    smooth_pass(PassArgs p, long long lines) {
+  using G = SmoothGeom<R1, R2, COL>;
+  __shared__ __align__(16) float2 lds[G::kB * G::kStride + G::kN + G::kWaves];
+  int tile, start;
+  if (p.slots > 0) {
+    start = blockIdx.x % p.slots;
+    tile = blockIdx.x / p.slots;
+    if (start >= p.starts) return;  // padding slot: the whole block leaves together
+  } else {
+    start = blockIdx.y;
+    tile = blockIdx.x;
+  }
+  smooth_tile<R1, R2, COL, M>(p, lines, start, tile, lds);
+}
+
+}  // namespace qpch::dev

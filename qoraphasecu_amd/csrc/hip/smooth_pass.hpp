@@ -1,0 +1,22 @@
+// Host interface of the smooth-length passes (smooth.hpp, instantiated in smooth.hip, its
+// own translation unit so that it compiles beside the engine).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "passes.hpp"
+
+namespace qpch {
+
+// Whether an axis of length N runs the modulus and update passes in smooth_pass: N = R1 R2
+// with R1 in {2, 4, 8, 16} and R2 in {3, 5, 15} (R2 = 15 preferred).
+bool smooth_axis(int N);
+
+// Launches smooth_pass for the modulus (Mode::kModulus) or update (Mode::kUpdate) pass of
+// an axis of length N with `lines` lines per start, on the grid generic_pass would use
+// (block numbering from a.slots).  col: the axis is strided (inner > 1).  Returns the
+// launch status; hipErrorInvalidValue when no kernel exists.
+hipError_t smooth_launch(int N, bool col, dev::Mode m, const dev::PassArgs& a, long long lines, dim3 grid,
+                         hipStream_t s);
+
+}  // namespace qpch

@@ -5,7 +5,8 @@ supports equal bit for bit, over recipes with Shrinkwrap, detwin and ER stages, 
 fp16 / bf16 storage (ER in fp32), starts that do not fill the eight XCDs evenly, shapes
 whose row tiles are resized to the column pass's work-group, launch groups running as
 concurrent persistent launches on lanes, images whose two axes are not powers of two
-(twopass_generic: mixed-radix, plain-DFT and Bluestein axes), and whole recipes of several
+(twopass_generic: mixed-radix, plain-DFT and Bluestein axes; smooth lengths keep the
+per-pass launches), and whole recipes of several
 shapes in one graph (their persistent launches run side by side).  Also against the oracle: the per-iteration
 error of a plain persistent run within 1e-5 of the oracle's float run."""
 
@@ -39,12 +40,13 @@ CASES = [
     ((64, 1024), dict(starts=2, storage="f16")),
     ((64, 64), dict(starts=8, storage="f16", mixed=True)),  # fp16 64^2 (fp32 runs whole images)
     ((256, 256), dict(starts=16, batch=4, lanes=4)),  # four persistent launches at once
-    # twopass_generic: both axes not powers of two (fp32)
-    ((240, 240), dict(starts=8)),
-    ((48, 40), dict(starts=3, batch=1)),
-    ((67, 30), dict(starts=4)),  # a prime axis below 256: one plain DFT per line
-    ((257, 24), dict(starts=2)),  # Bluestein columns
-    ((24, 1021), dict(starts=2)),  # Bluestein rows, one line per work-group
+    # twopass_generic: both axes not powers of two nor smooth lengths (fp32)
+    ((252, 252), dict(starts=8)),  # 252 = 4 * 3 * 3 * 7
+    ((56, 44), dict(starts=3, batch=1)),
+    ((67, 42), dict(starts=4)),  # a prime axis below 256: one plain DFT per line
+    ((257, 36), dict(starts=2)),  # Bluestein columns
+    ((36, 1021), dict(starts=2)),  # Bluestein rows, one line per work-group
+    ((240, 240), dict(starts=8)),  # smooth axes (16 * 15): the per-pass launches either way
 ]
 
 

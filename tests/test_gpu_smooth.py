@@ -1,0 +1,71 @@
+"""The smooth-length passes (smooth.hpp: axes N = R1 R2 with R1 in {2, 4, 8, 16} and R2 in
+{3, 5, 15}, the line held in registers) against the oracle: one teacher-forced HIO, RAAR
+and ER iteration from the same starts, error and object within 1e-5 relative, on every
+factor pair as a column (strided) axis and as a row axis, beside power-of-two line passes,
+and as the strided z axis of 3D shapes (before the plane pass and before line passes);
+and a 30-iteration RAAR run per shape kind, the per-iteration error within the bar of
+test_gpu_per_iteration.py's 48 x 40 case (1e-5, or float rounding's measured spread)."""
+
+import numpy as np
+import pytest
+
+import oracle
+import recipes as R
+import synthetic
+from test_gpu_abi import _problem, rel
+
+pytestmark = pytest.mark.gpu
+
+# every N = R1 R2 on axis 0 (columns) and on axis 1 (rows)
+PAIRS = [(6, 10), (10, 6), (30, 12), (12, 30), (20, 60), (60, 20), (24, 40), (40, 24), (120, 48), (48, 120),
+         (80, 240), (240, 80)]
+MIXED = [(64, 240), (240, 64), (30, 16, 16), (24, 64, 64), (16, 24, 40), (120, 8, 32)]
+
+
+@pytest.mark.parametrize("shape", PAIRS + MIXED)
+@pytest.mark.parametrize("schedule", ["graph", "fused"])
+def test_smooth_teacher_forced_iteration(qpch, shape, schedule):
+    obj, amp, mask, sup = _problem(shape, max(3, min(shape[-2:]) // 4), masked=True)
+    start = np.stack([synthetic.random_field(obj.size, 41).reshape(shape),
+                      synthetic.random_field(obj.size, 42).reshape(shape)])
+    sf = np.stack([oracle.to_fft_order(s) for s in start])
+    for kind, ops in [("hio", [R.hio(1)]), ("raar", [R.raar(1)]), ("er", [R.er(1)])]:
+        o = oracle.Solver(amp, mask, sup, starts=2, start=start)
+        o.run(ops)
+        ores = o.results()
+        s = qpch.Solver(oracle.to_fft_order(amp), oracle.to_fft_order(mask), oracle.to_fft_order(sup),
+                        starts=2, start=sf, schedule=schedule)
+        errs = R.run_abi(s, ops)
+        xs = s.read_state()
+        s.close()
+        for k in range(2):
+            d = abs(errs[0, k] - ores[k][2][0]) / ores[k][2][0]
+            xo = oracle.to_fft_order(ores[k][0])
+            print(shape, schedule, kind, k, f"error {d:.1e} object {rel(xs[k], xo):.1e}")
+            assert d <= 1e-5, (shape, kind, k, d)
+            assert rel(xs[k], xo) < 1e-5, (shape, kind, k, rel(xs[k], xo))
+
+
+@pytest.mark.parametrize("shape", [(240, 240), (80, 48), (30, 16, 16)])
+def test_smooth_per_iteration_error(qpch, shape):
+    """RAAR(30) from given starts: the device's error within max(1e-5, 3 x the spread of the
+    oracle's float run against its FP64 run) of the oracle's float run at every iteration."""
+    obj, sup, inten = synthetic.triangle(shape, max(4, min(shape[-2:]) // 4))
+    amp, mask, sup = oracle.make_problem(inten, sup, None)
+    start = np.stack([synthetic.random_field(obj.size, 51).reshape(shape),
+                      synthetic.random_field(obj.size, 52).reshape(shape)])
+    ops = [("raar", 30, 0.9, 0.0, 0.0, "max", 0)]
+    hist = {}
+    for prec in ("f", "d"):
+        o = oracle.Solver(amp, mask, sup, starts=2, start=start, prec=prec)
+        o.run(ops)
+        hist[prec] = np.stack([np.asarray(r[2], np.float64) for r in o.results()], axis=1)
+    f = oracle.to_fft_order
+    s = qpch.Solver(f(amp), f(mask), f(sup), starts=2, start=np.stack([f(x) for x in start]), schedule="graph")
+    errs = R.run_abi(s, ops).astype(np.float64)
+    s.close()
+    d = np.max(np.abs(errs - hist["f"]) / hist["f"], axis=1)
+    drift = np.maximum.accumulate(np.max(np.abs(hist["f"] - hist["d"]) / hist["d"], axis=1))
+    bar = np.maximum(1e-5, 3.0 * drift)
+    print(shape, f"device vs float oracle worst {d.max():.2e}; float oracle vs FP64 {drift.max():.2e}")
+    assert np.all(d <= bar), (shape, d.max(), int(np.argmax(d > bar)))

@@ -85,9 +85,11 @@ typedef struct {
   int radix;           /* FFT radix of the line passes: 0 or 16 = radix 16, 8 = radix 8 for axes
                           of 64 to 2048 (other axes stay radix 16) */
   int persist;         /* 0 = 2D chunks of iterations as one persistent launch where a kernel
-                          exists for power-of-two axes (64 to 1024, radix 16), 1 = also for
-                          fp32 images whose two axes are not powers of two, 2 = a launch per
-                          pass; every choice is the same arithmetic, bit for bit */
+                          exists for power-of-two axes (64 to 1024, radix 16) and for square
+                          fp32 images of a smooth length (R1 R2, R1 in 2..16 a power of two,
+                          R2 in {3, 5, 15}), 1 = also for the other fp32 images whose two axes
+                          are not powers of two, 2 = a launch per pass; every choice is the
+                          same arithmetic, bit for bit */
 } qpch_problem;
 
 /* Launch configuration chosen by qpch_autotune. */

@@ -802,8 +802,9 @@ struct SolverImpl {
   int xcd_mode = 0;  // 0 XCD-affine by start, 2 plain numbering, 3 plain for the modulus pass only
   int radix = 16;        // FFT radix of the line passes (8 or 16)
   bool persist = false;  // 2D chunks of iterations in one persistent launch where a kernel exists
-  // Images whose two axes are not powers of two run persistent launches only when asked
-  // (qpch_problem.persist = 1): at 240^2 the per-pass launches are the faster default.
+  // Images whose two axes are not powers of two run persistent launches by default only
+  // when they are square of a smooth length (twopass_smooth); the others (twopass_generic)
+  // only when asked (qpch_problem.persist = 1): there the per-pass launches are faster.
   bool persist_generic = false;
   DevBuf<dev::GenericPlan> generic_plans;  // twopass_generic: the plans of axes 0 and 1 in device memory
   DevBuf<int> psync;  // a line holding the abort flag, then per launch group: next[8 lines], done, count[2][batch]
@@ -935,6 +936,7 @@ struct SolverImpl {
     if (!persist || dims.nd != 2) return false;
     if (radix == 16 && twopass_supported(dims.d[0], dims.d[1], storage_code<St>())) return true;
     if constexpr (std::is_same_v<St, float2>) {
+      if (twopass_smooth_supported(dims.d[0], dims.d[1])) return true;
       if (persist_generic && generic_axis(dims.d[0]) && generic_axis(dims.d[1]) && !smooth_axis(dims.d[0]) &&
           !smooth_axis(dims.d[1])) {
         return twopass_generic_supported(generic_plan(dims.d[0]), generic_plan(dims.d[1]));
@@ -963,6 +965,12 @@ struct SolverImpl {
     hipStream_t st = on ? on : stream;
     if (generic_axis(dims.d[0])) {
       if constexpr (std::is_same_v<St, float2>) {
+        if (twopass_smooth_supported(dims.d[0], dims.d[1])) {
+          const int N = dims.d[0];
+          q.tiles0 = q.tiles1 = (N + dev::generic_lines_per_block(N) - 1) / dev::generic_lines_per_block(N);
+          QPCH_HIP(twopass_smooth_launch(N, p0, p1, q, len, persist_workers, st));
+          return true;
+        }
         const dev::GenericPlan g0 = generic_plan(dims.d[0]), g1 = generic_plan(dims.d[1]);
         const long long lines0 = dims.d[1], lines1 = dims.d[0];  // axis 0: columns; axis 1: rows
         q.tiles0 = static_cast<int>((lines0 + dev::generic_lines_per_block(dims.d[0]) - 1) /
@@ -2494,8 +2502,10 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
       (void)hipEventDestroy(e1);
       return us;
     };
-    // images whose two axes are not powers of two: persistent launches only with persist = 1
-    const bool generic2d = d.nd == 2 && qpch::generic_axis(d.d[0]) && qpch::generic_axis(d.d[1]);
+    // images whose two axes are not powers of two, other than square smooth ones: persistent
+    // launches only with persist = 1
+    const bool generic2d = d.nd == 2 && qpch::generic_axis(d.d[0]) && qpch::generic_axis(d.d[1]) &&
+                           !qpch::twopass_smooth_supported(d.d[0], d.d[1]);
     auto trial = [&](const qpch_tuning& t) {
       qpch_tuning r = t;
       r.us_per_iteration = measure(t);
@@ -2523,7 +2533,9 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
     auto persistent_for = [&](int storage) {
       if (d.nd != 2 || problem->persist == 2 || qpch::twopass_xcds() == 0) return false;
       if (storage == QPCH_F32 && qpch::generic_axis(d.d[0]) && qpch::generic_axis(d.d[1])) {
-        // (smooth axes run smooth_pass, which the persistent kernel does not)
+        // (smooth axes run smooth_pass: the persistent launch of square images of a smooth
+        // length runs it too, twopass_generic does not)
+        if (qpch::twopass_smooth_supported(d.d[0], d.d[1])) return true;
         return !qpch::smooth_axis(d.d[0]) && !qpch::smooth_axis(d.d[1]) &&
                qpch::twopass_generic_supported(qpch::generic_plan(d.d[0]), qpch::generic_plan(d.d[1]));
       }

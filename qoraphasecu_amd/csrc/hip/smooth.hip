@@ -3,6 +3,8 @@
 
 #include "smooth.hpp"
 
+#include <algorithm>
+
 namespace qpch {
 namespace {
 
@@ -46,7 +48,40 @@ void launch_pair(bool col, dev::Mode m, const dev::PassArgs& a, long long lines,
   }
 }
 
+template <int R1, int R2>
+hipError_t launch_twopass(const dev::PassArgs& p0, const dev::PassArgs& p1, const dev::SyncArgs& q, int len,
+                          int workers, hipStream_t s) {
+  constexpr int kThreads = dev::SmoothGeom<R1, R2, true>::kThreads;
+  int per_cu = 0, dev = 0, cus = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::twopass_smooth<R1, R2>, kThreads, 0);
+  if (e == hipSuccess) e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  const int cap = std::max(1, per_cu) * cus / q.xcds * q.xcds;
+  const int items = q.xcds * dev::xcd_starts(p1.starts, 0, q.xcds) * std::max(q.tiles0, q.tiles1);
+  const int w = workers > 0 ? workers : std::max(q.xcds, std::min(items, cap));
+  const long long lines = dev::SmoothGeom<R1, R2, true>::kN;  // per start, both passes (square image)
+  hipLaunchKernelGGL((dev::twopass_smooth<R1, R2>), dim3(w), dim3(kThreads), 0, s, p0, p1, lines, lines, q, len);
+  return hipGetLastError();
+}
+
 }  // namespace
+
+bool twopass_smooth_supported(int ny, int nx) {
+  int r1 = 0, r2 = 0;
+  return ny == nx && factors(ny, r1, r2);
+}
+
+hipError_t twopass_smooth_launch(int N, const dev::PassArgs& p0, const dev::PassArgs& p1, const dev::SyncArgs& q,
+                                 int len, int workers, hipStream_t s) {
+  int r1 = 0, r2 = 0;
+  if (!factors(N, r1, r2)) return hipErrorInvalidValue;
+#define QPCH_SMOOTH_CASE(A, B) \
+  if (r1 == A && r2 == B) return launch_twopass<A, B>(p0, p1, q, len, workers, s);
+  QPCH_FOR_EACH_SMOOTH(QPCH_SMOOTH_CASE)
+#undef QPCH_SMOOTH_CASE
+  return hipErrorInvalidValue;
+}
 
 bool smooth_axis(int N) {
   int r1 = 0, r2 = 0;

@@ -29,6 +29,8 @@
 #include "fft_engine.hpp"
 #include "generic.hpp"
 #include "passes.hpp"
+#include "persist.hpp"
+#include "storage.hpp"
 
 namespace qpch::dev {
 
@@ -48,6 +50,17 @@ struct SmoothGeom {
   static constexpr int kStride = COL ? (kLine | 1) : kLine;    // odd for column tiles (banks)
   static_assert(kThreads % 64 == 0, "whole waves (the error partial's shuffles)");
 };
+
+// W_N^j = exp(-2 pi i j / N), j < N, into LDS: evaluated in double, rounded once (the
+// values of the generic kernels' tables).
+template <int N, int kThreads>
+__device__ __forceinline__ void smooth_twiddles(float2* tw) {
+  for (int j = threadIdx.x; j < N; j += kThreads) {
+    double sn, cs;
+    sincospi(2.0 * static_cast<double>(j) / static_cast<double>(N), &sn, &cs);
+    tw[j] = {static_cast<float>(cs), static_cast<float>(-sn)};
+  }
+}
 
 // LDS position of line element p: stage-1 writes of the forward transform go to i R1 + q,
 // a stride of R1 across the lanes; the pad makes it R1 + 1.
@@ -143,15 +156,19 @@ __device__ __forceinline__ void smooth_inv(float2* v, int i, bool inA, bool inB,
 }
 
 // The modulus or update pass over one tile (kB lines) of start `start`: generic_tile's
-// semantics for these two modes (fp32).  lds: kB line slots of kStride float2, then the
-// twiddle table (kN float2), then one double per wave.
-template <int R1, int R2, bool COL, Mode M>
-__device__ __forceinline__ void smooth_tile(const PassArgs& p, long long lines, int start, int tile, float2* lds) {
+// semantics for these two modes (fp32).  lds: kB line slots of kStride float2; tw: the
+// twiddle table W_N^j (kN float2), filled here with OWN_TW (while the loads are in
+// flight), else already filled and visible; red: one double per wave.  `slot`: the error
+// history slot of the update pass (kSlotFromCounter: *p.counter - 1).  COH: the wave, the
+// iterate and the partials may have been written by other CUs in the same launch (the
+// persistent launches): device-scope loads.
+template <int R1, int R2, bool COL, Mode M, bool OWN_TW = true, bool COH = false>
+__device__ __forceinline__ void smooth_tile(const PassArgs& p, long long lines, int start, int tile, float2* lds,
+                                            float2* tw, double* red, int slot) {
   static_assert(M == Mode::kModulus || M == Mode::kUpdate, "the iteration passes only");
   using G = SmoothGeom<R1, R2, COL>;
   constexpr int N = G::kN;
-  float2* tw = lds + G::kB * G::kStride;
-  double* red = reinterpret_cast<double*>(tw + N);
+  using S = Storage<float2>;
   const int tid = threadIdx.x;
   int b, i;
   if constexpr (COL) {
@@ -179,14 +196,10 @@ __device__ __forceinline__ void smooth_tile(const PassArgs& p, long long lines, 
     const long long arow = p.amp_stride * ((p.err_offset + start) / (p.amp_group > 1 ? p.amp_group : 1));
     float amp[R2];
 #pragma unroll
-    for (int m = 0; m < R1; ++m) v[m] = w[base + static_cast<long long>(ia + m * R2) * inner];
+    for (int m = 0; m < R1; ++m) v[m] = S::ld<COH>(w, static_cast<size_t>(base + static_cast<long long>(ia + m * R2) * inner));
 #pragma unroll
     for (int m = 0; m < R2; ++m) amp[m] = p.amp[arow + base + static_cast<long long>(ib + m * R1) * inner];
-    for (int j = tid; j < N; j += G::kThreads) {  // the table while the loads are in flight
-      double sn, cs;
-      sincospi(2.0 * static_cast<double>(j) / static_cast<double>(N), &sn, &cs);
-      tw[j] = {static_cast<float>(cs), static_cast<float>(-sn)};
-    }
+    if constexpr (OWN_TW) smooth_twiddles<N, G::kThreads>(tw);  // while the loads are in flight
     __syncthreads();
     smooth_fwd<R1, R2>(v, i, inA, inB, line, tw);
     double acc = 0.0;
@@ -214,18 +227,14 @@ __device__ __forceinline__ void smooth_tile(const PassArgs& p, long long lines, 
     float2 xo[R1];
     bool inside[R1];
 #pragma unroll
-    for (int m = 0; m < R2; ++m) v[m] = w[base + static_cast<long long>(ib + m * R1) * inner];
+    for (int m = 0; m < R2; ++m) v[m] = S::ld<COH>(w, static_cast<size_t>(base + static_cast<long long>(ib + m * R1) * inner));
 #pragma unroll
     for (int m = 0; m < R1; ++m) {
       const long long e = base + static_cast<long long>(ia + m * R2) * inner;
-      xo[m] = x[e];
+      xo[m] = S::ld<COH>(x, static_cast<size_t>(e));
       inside[m] = sup[e] != 0;
     }
-    for (int j = tid; j < N; j += G::kThreads) {
-      double sn, cs;
-      sincospi(2.0 * static_cast<double>(j) / static_cast<double>(N), &sn, &cs);
-      tw[j] = {static_cast<float>(cs), static_cast<float>(-sn)};
-    }
+    if constexpr (OWN_TW) smooth_twiddles<N, G::kThreads>(tw);
     __syncthreads();
     smooth_inv<R1, R2>(v, i, inA, inB, line, tw);
     if (valid && inA) {
@@ -247,10 +256,10 @@ __device__ __forceinline__ void smooth_tile(const PassArgs& p, long long lines, 
     }
     if (tile == 0 && tid < 64 && p.errs) {
       // fixed-order sum of the previous modulus pass's partials of this start
-      const double t = partial_sum<false>(p.partials + static_cast<long long>(start) * p.partial_stride,
-                                          p.partial_stride, tid);
+      const double t = partial_sum<COH>(p.partials + static_cast<long long>(start) * p.partial_stride,
+                                        p.partial_stride, tid);
       if (tid == 0) {
-        const int e = *p.counter - 1;
+        const int e = slot == kSlotFromCounter ? *p.counter - 1 : slot;
         if (e >= 0 && e < p.err_cap) p.errs[static_cast<long long>(e) * p.err_stride + p.err_offset + start] = t;
       }
     }
@@ -272,7 +281,35 @@ __global__ void __launch_bounds__((SmoothGeom<R1, R2, COL>::kThreads))
     start = blockIdx.y;
     tile = blockIdx.x;
   }
-  smooth_tile<R1, R2, COL, M>(p, lines, start, tile, lds);
+  float2* tw = lds + G::kB * G::kStride;
+  smooth_tile<R1, R2, COL, M>(p, lines, start, tile, lds, tw, reinterpret_cast<double*>(tw + G::kN), kSlotFromCounter);
+}
+
+// `len` iterations of every start of a launch group on an N x N image whose axis length is
+// smooth, in one persistent launch: persistent_loop (persist.hpp) over the tiles of the
+// column modulus pass and the row update pass, each a smooth_tile (the per-pass kernels'
+// body; the same work-group size for both, as the axes are equal), with device-scope
+// loads of what the launch itself writes.  Both tables are filled once.
+template <int R1, int R2>
+__global__ void __launch_bounds__((SmoothGeom<R1, R2, true>::kThreads))
+    twopass_smooth(PassArgs p0, PassArgs p1, long long lines0, long long lines1, SyncArgs q, int len) {
+  using GC = SmoothGeom<R1, R2, true>;
+  using GR = SmoothGeom<R1, R2, false>;
+  static_assert(GC::kThreads == GR::kThreads, "one work-group size for both passes");
+  constexpr int kSlots = GC::kB * GC::kStride > GR::kB * GR::kStride ? GC::kB * GC::kStride : GR::kB * GR::kStride;
+  __shared__ __align__(16) float2 lds[kSlots + GC::kN + GC::kWaves];
+  __shared__ int claim;
+  float2* tw = lds + kSlots;
+  double* red = reinterpret_cast<double*>(tw + GC::kN);
+  smooth_twiddles<GC::kN, GC::kThreads>(tw);  // both axes: one table
+  persistent_loop(
+      p1, q, len, claim,
+      [&](int start, int tile) {
+        smooth_tile<R1, R2, true, Mode::kModulus, false, true>(p0, lines0, start, tile, lds, tw, red, 0);
+      },
+      [&](int start, int tile, int slot) {
+        smooth_tile<R1, R2, false, Mode::kUpdate, false, true>(p1, lines1, start, tile, lds, tw, red, slot);
+      });
 }
 
 }  // namespace qpch::dev

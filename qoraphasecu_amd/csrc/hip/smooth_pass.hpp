@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "passes.hpp"
+#include "persist.hpp"
 
 namespace qpch {
 
@@ -18,5 +19,13 @@ bool smooth_axis(int N);
 // launch status; hipErrorInvalidValue when no kernel exists.
 hipError_t smooth_launch(int N, bool col, dev::Mode m, const dev::PassArgs& a, long long lines, dim3 grid,
                          hipStream_t s);
+
+// The persistent launch for square images N x N of smooth axis length (twopass_smooth):
+// whether a kernel exists, and the launch of `len` iterations with `workers` work-groups
+// (0: as many per XCD as the busiest XCD has tiles per pass, at most what fits the device
+// at once).  q.tiles0 / q.tiles1: the tiles per start of the two passes.
+bool twopass_smooth_supported(int ny, int nx);
+hipError_t twopass_smooth_launch(int N, const dev::PassArgs& p0, const dev::PassArgs& p1, const dev::SyncArgs& q,
+                                 int len, int workers, hipStream_t s);
 
 }  // namespace qpch

@@ -5,8 +5,8 @@ supports equal bit for bit, over recipes with Shrinkwrap, detwin and ER stages, 
 fp16 / bf16 storage (ER in fp32), starts that do not fill the eight XCDs evenly, shapes
 whose row tiles are resized to the column pass's work-group, launch groups running as
 concurrent persistent launches on lanes, images whose two axes are not powers of two
-(twopass_generic: mixed-radix, plain-DFT and Bluestein axes; smooth lengths keep the
-per-pass launches), and whole recipes of several
+(twopass_generic: mixed-radix, plain-DFT and Bluestein axes; twopass_smooth: square images
+of a smooth length, whose passes hold the line in registers), and whole recipes of several
 shapes in one graph (their persistent launches run side by side).  Also against the oracle: the per-iteration
 error of a plain persistent run within 1e-5 of the oracle's float run."""
 
@@ -46,7 +46,12 @@ CASES = [
     ((67, 42), dict(starts=4)),  # a prime axis below 256: one plain DFT per line
     ((257, 36), dict(starts=2)),  # Bluestein columns
     ((36, 1021), dict(starts=2)),  # Bluestein rows, one line per work-group
-    ((240, 240), dict(starts=8)),  # smooth axes (16 * 15): the per-pass launches either way
+    # twopass_smooth: square images of a smooth length (smooth.hpp)
+    ((240, 240), dict(starts=8)),  # 16 * 15
+    ((48, 48), dict(starts=3, batch=1)),  # 16 * 3
+    ((120, 120), dict(starts=12)),  # 8 * 15
+    ((30, 30), dict(starts=8)),  # 2 * 15: two tiles of 16 lines, the second ragged
+    ((80, 240), dict(starts=4)),  # smooth, not square: the per-pass launches either way
 ]
 
 

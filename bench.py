@@ -352,8 +352,8 @@ def cpu_full_run(w, threads):
 
 
 def time_next_rows(steps):
-    """The rows after the hot path: a non-power-of-two problem (240^2 = 2^4 3 5, 8 starts,
-    RAAR(50)) through the generic mixed-radix kernels, with the oracle's rate on the same
+    """The rows after the hot path: a non-power-of-two problem (240^2 = 16 x 15, 8 starts,
+    RAAR(50)) through the smooth-length kernels, with the oracle's rate on the same
     problem beside it; a prime row length (64 x 1021, 8 starts, RAAR(50): Bluestein's
     transform in the generic passes); and the device make_problem (autocorrelation support)
     and aligned_error at 256^2."""
@@ -371,7 +371,7 @@ def time_next_rows(steps):
             s.run(ops)
         wall = (time.perf_counter() - t0) / steps
         res[variant] = dict(ms_per_run=wall * 1e3, start_iterations_per_s=8 * 50 / wall)
-    log(f"  240^2 (generic radix 2/3/5): graph {res['graph']['start_iterations_per_s']:.4g} start-it/s, "
+    log(f"  240^2 (smooth length 16 x 15, line in registers): graph {res['graph']['start_iterations_per_s']:.4g} start-it/s, "
         f"batched {res['batched']['start_iterations_per_s']:.4g}")
     # a prime row axis (1021: Bluestein's transform in the generic passes)
     obj3, inten3, mask3 = workloads.problem_2d(64, seed=1001, photons=1e7, beamstop_radius=4, box=24)

@@ -360,13 +360,20 @@ __device__ __forceinline__ void line_tile(const PassArgs& p, int start, int tile
     const View<const std::uint8_t, COL> vs(p.support + soff, p.n, e0, em);
 #pragma unroll
     for (int m = 0; m < kR; ++m) v[m] = vld<COH>(vw, m);
-    // the old iterate and the support do not depend on the transform: fetch them now
+    // the old iterate and the support do not depend on the transform: fetch them now.  ER
+    // (a = b = 0) skips the iterate: outside the support a x + b P_M x is zero whatever x
+    // holds (only the sign of that zero can differ), and the pass moves 8 (fp32) or 4
+    // (fp16) bytes less per element.
     float2 xo[kR];
     bool inside[kR];
 #pragma unroll
-    for (int m = 0; m < kR; ++m) {
-      inside[m] = vld_flag(vs, m);
-      xo[m] = vld<COH>(vx, m);
+    for (int m = 0; m < kR; ++m) inside[m] = vld_flag(vs, m);
+    if (p.a != 0.0f || p.b != 0.0f) {
+#pragma unroll
+      for (int m = 0; m < kR; ++m) xo[m] = vld<COH>(vx, m);
+    } else {
+#pragma unroll
+      for (int m = 0; m < kR; ++m) xo[m] = float2{0.0f, 0.0f};
     }
     if constexpr (OWN_TW) {
       load_twiddles<N>(tw);

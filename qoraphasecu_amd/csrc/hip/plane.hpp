@@ -112,15 +112,22 @@ __global__ void __launch_bounds__((PlaneGeom<NY, NX>::kThreads))
   // transforms spills with fp32 storage, and with fp16 storage it measured no faster.  They
   // are read eight elements at a time, unconditionally, so that sixteen loads are in flight
   // together; sixteen at a time spills at 128 x 128.)
+  // ER (a = b = 0) skips the old iterate, as the line passes do: outside the support the
+  // update is zero whatever it holds.
   constexpr int kG = 8;
+  const bool need_x = p.a != 0.0f || p.b != 0.0f;
 #pragma unroll
   for (int h = 0; h < 16; h += kG) {
     float2 xo[kG];
     bool inside[kG];
 #pragma unroll
-    for (int q = 0; q < kG; ++q) {
-      inside[q] = vld_flag(scol, h + q);
-      xo[q] = vld(xcol, h + q);
+    for (int q = 0; q < kG; ++q) inside[q] = vld_flag(scol, h + q);
+    if (need_x) {
+#pragma unroll
+      for (int q = 0; q < kG; ++q) xo[q] = vld(xcol, h + q);
+    } else {
+#pragma unroll
+      for (int q = 0; q < kG; ++q) xo[q] = float2{0.0f, 0.0f};
     }
 #pragma unroll
     for (int q = 0; q < kG; ++q) {

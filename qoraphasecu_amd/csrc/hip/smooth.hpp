@@ -1,8 +1,8 @@
-// Fused passes along axes of smooth length N = R1 * R2 (R1 a power of two up to 16, R2 in
-// {3, 5, 15}), with the line held in registers: one in-register radix-R1 stage, one LDS
-// exchange, one in-register radix-R2 stage, where generic_pass (generic.hpp) runs four to
-// six radix <= 5 stages through LDS with a barrier each (240 = 16 * 15: two stages
-// instead of four each way).
+// Fused passes along axes of smooth length N = R1 * R2 (R1 a power of two, R2 in {3, 5,
+// 15}), with the line held in registers: a radix-R1 transform (in registers up to 16, the
+// radix-16 line engine from 32 on), one LDS exchange, one in-register radix-R2 stage,
+// where generic_pass (generic.hpp) runs one radix <= 5 stage per LDS round trip and
+// barrier (240 = 16 * 15: two stages instead of four each way).
 //
 // Register layouts of a line (thread i of the line's T threads):
 //   A: thread i < R2 holds x[i + m R2], m < R1
@@ -36,19 +36,45 @@ namespace qpch::dev {
 
 __host__ __device__ constexpr int smooth_pow2_ceil(int v) { return v <= 1 ? 1 : 2 * smooth_pow2_ceil((v + 1) / 2); }
 
+// R1 <= 16: the power-of-two stage is one in-register DFT (layouts A and B above).  R1 >= 32
+// (the line engine): layout A is R2 interleaved subsequences x[i + p R2], p < R1, each
+// transformed by the radix-16 engine (fft_line, fft_engine.hpp) on R1/16 threads of 16
+// registers: thread (i, i') holds x[i + (i' + m R1/16) R2], m < 16.  The exchange to layout
+// B (thread j < R1 holds X[j + m R1], m < R2) goes through LDS at k R2 + i, and thread j
+// twiddles S_i[j] by W_N^(i j) before the radix-R2 DFT over i.  The inverse mirrors it.
 template <int R1, int R2, bool COL>
 struct SmoothGeom {
-  static_assert(R1 == 2 || R1 == 4 || R1 == 8 || R1 == 16, "R1: a power of two up to 16");
+  static_assert(R1 >= 2 && R1 <= 512 && (R1 & (R1 - 1)) == 0, "R1: a power of two from 2 to 512");
   static_assert(R2 == 3 || R2 == 5 || R2 == 15, "R2: 3, 5 or 15");
   static constexpr int kN = R1 * R2;
-  static constexpr int kV = R1 > R2 ? R1 : R2;                 // registers per thread
-  static constexpr int kT = smooth_pow2_ceil(kV);              // threads per line
+  static_assert(kN <= kGenericMaxN, "the generic kernels' length limit");
+  static constexpr bool kEngine = R1 > 16;                     // the radix-16 line engine for R1
+  static constexpr int kSub = kEngine ? R1 / 16 : 1;           // threads per subsequence (layout A)
+  static constexpr int kA = kEngine ? 16 : R1;                 // registers in layout A
+  static constexpr int kTA = R2 * kSub;                        // threads holding layout A
+  static constexpr int kV = kA > R2 ? kA : R2;                 // registers per thread
+  static constexpr int kT = smooth_pow2_ceil(kTA > R1 ? kTA : R1);  // threads per line
   static constexpr int kB = generic_lines_per_block(kN);       // lines per work-group (as generic_pass)
   static constexpr int kThreads = kB * kT;
   static constexpr int kWaves = (kThreads + 63) / 64;
-  static constexpr int kLine = kN + kN / R1;                   // one pad slot per R1 elements
+  // line scratch: R1 <= 16, the line with one pad slot per R1 elements; the engine, one
+  // padded fft_line region per subsequence slot i = t / kSub (all kT / kSub = 16 of them, so
+  // that threads without a subsequence transform in a region of their own)
+  static constexpr int kRegion = R1 + R1 / 16;
+  static constexpr int kLine = kEngine ? (kT / kSub) * kRegion : kN + kN / R1;
   static constexpr int kStride = COL ? (kLine | 1) : kLine;    // odd for column tiles (banks)
+  static constexpr int kTwP = kEngine ? R1 : 0;                // W_R1 table of the engine
   static_assert(kThreads % 64 == 0, "whole waves (the error partial's shuffles)");
+  static_assert(!kEngine || kT == R1, "one thread per layout-B column");
+  // line position held in layout-A register m by thread t (t < kTA), and in layout B (t < R1)
+  __device__ static int posA(int t, int m) {
+    if constexpr (kEngine) {
+      return t / kSub + (t % kSub + m * kSub) * R2;
+    } else {
+      return t + m * R2;
+    }
+  }
+  __device__ static int posB(int t, int m) { return t + m * R1; }
 };
 
 // W_N^j = exp(-2 pi i j / N), j < N, into LDS: evaluated in double, rounded once (the
@@ -102,57 +128,108 @@ __device__ __forceinline__ void dft_odd(float2* v) {
   }
 }
 
-// Layout A -> layout B (forward).  inA / inB: whether this thread holds a line position in
-// layout A / B.  tw: W_N^j, j < N (forward table).  Every thread of the work-group calls it.
+// Layout A -> layout B (forward).  t: the thread's index within its line; line: the line's
+// scratch; tw: W_N^j, j < N (forward table); twP: W_R1^j, j < R1 (the engine's table, R1 >=
+// 32).  Every thread of the work-group calls it.
 template <int R1, int R2>
-__device__ __forceinline__ void smooth_fwd(float2* v, int i, bool inA, bool inB, float2* line, const float2* tw) {
-  if (inA) {
-    dft_regs<R1, true>(v);
+__device__ __forceinline__ void smooth_fwd(float2* v, int t, float2* line, const float2* tw, const float2* twP) {
+  using G = SmoothGeom<R1, R2, false>;
+  if constexpr (!G::kEngine) {
+    const int i = t;
+    if (i < R2) {
+      dft_regs<R1, true>(v);
 #pragma unroll
-    for (int q = 0; q < R1; ++q) line[smooth_pos<R1>(i * R1 + q)] = v[q];
-  }
-  __syncthreads();
-  if (inB) {
-    float2 w[R2];
-#pragma unroll
-    for (int q = 0; q < R2; ++q) {
-      v[q] = line[smooth_pos<R1>(i + q * R1)];
-      w[q] = tw[q * i];
+      for (int q = 0; q < R1; ++q) line[smooth_pos<R1>(i * R1 + q)] = v[q];
     }
+    __syncthreads();
+    if (i < R1) {
+      float2 w[R2];
 #pragma unroll
-    for (int q = 1; q < R2; ++q) {
-      const float2 t = cmul(v[q], w[q]);
-      if (i != 0) v[q] = t;  // W^0 = 1: the input stays as it is
+      for (int q = 0; q < R2; ++q) {
+        v[q] = line[smooth_pos<R1>(i + q * R1)];
+        w[q] = tw[q * i];
+      }
+#pragma unroll
+      for (int q = 1; q < R2; ++q) {
+        const float2 u = cmul(v[q], w[q]);
+        if (i != 0) v[q] = u;  // W^0 = 1: the input stays as it is
+      }
+      dft_odd<R2, false>(v);
     }
-    dft_odd<R2, false>(v);
+    __syncthreads();
+  } else {
+    constexpr int kSub = G::kSub;
+    const int i = t / kSub, ip = t % kSub;
+    fft_line<R1, 16, true>(v, ip, line + i * G::kRegion, twP);  // S_i, natural order (ends past its last LDS read)
+    if (t < G::kTA) {
+#pragma unroll
+      for (int m = 0; m < 16; ++m) line[(ip + m * kSub) * R2 + i] = v[m];
+    }
+    __syncthreads();
+    if (t < R1) {
+      float2 w[R2];
+#pragma unroll
+      for (int q = 0; q < R2; ++q) {
+        v[q] = line[t * R2 + q];
+        w[q] = tw[q * t];
+      }
+#pragma unroll
+      for (int q = 1; q < R2; ++q) {
+        const float2 u = cmul(v[q], w[q]);
+        if (t != 0) v[q] = u;
+      }
+      dft_odd<R2, false>(v);
+    }
+    __syncthreads();
   }
-  __syncthreads();
 }
 
 // Layout B -> layout A (inverse, unnormalised).
 template <int R1, int R2>
-__device__ __forceinline__ void smooth_inv(float2* v, int i, bool inA, bool inB, float2* line, const float2* tw) {
-  if (inB) {
-    dft_odd<R2, true>(v);
+__device__ __forceinline__ void smooth_inv(float2* v, int t, float2* line, const float2* tw, const float2* twP) {
+  using G = SmoothGeom<R1, R2, false>;
+  if constexpr (!G::kEngine) {
+    const int i = t;
+    if (i < R1) {
+      dft_odd<R2, true>(v);
 #pragma unroll
-    for (int q = 0; q < R2; ++q) line[smooth_pos<R1>(i * R2 + q)] = v[q];
-  }
-  __syncthreads();
-  if (inA) {
-    float2 w[R1];
-#pragma unroll
-    for (int q = 0; q < R1; ++q) {
-      v[q] = line[smooth_pos<R1>(i + q * R2)];
-      w[q] = tw[q * i];
+      for (int q = 0; q < R2; ++q) line[smooth_pos<R1>(i * R2 + q)] = v[q];
     }
+    __syncthreads();
+    if (i < R2) {
+      float2 w[R1];
 #pragma unroll
-    for (int q = 1; q < R1; ++q) {
-      const float2 t = cmulc(v[q], w[q]);
-      if (i != 0) v[q] = t;
+      for (int q = 0; q < R1; ++q) {
+        v[q] = line[smooth_pos<R1>(i + q * R2)];
+        w[q] = tw[q * i];
+      }
+#pragma unroll
+      for (int q = 1; q < R1; ++q) {
+        const float2 u = cmulc(v[q], w[q]);
+        if (i != 0) v[q] = u;
+      }
+      dft_regs<R1, false>(v);
     }
-    dft_regs<R1, false>(v);
+    __syncthreads();
+  } else {
+    constexpr int kSub = G::kSub;
+    if (t < R1) {
+      dft_odd<R2, true>(v);  // output i of column t: T_i[t] before its twiddle
+#pragma unroll
+      for (int i = 0; i < R2; ++i) {
+        const float2 u = cmulc(v[i], tw[i * t]);
+        line[t * R2 + i] = (t != 0 && i != 0) ? u : v[i];
+      }
+    }
+    __syncthreads();
+    const int i = t / kSub, ip = t % kSub;
+    if (t < G::kTA) {
+#pragma unroll
+      for (int m = 0; m < 16; ++m) v[m] = line[(ip + m * kSub) * R2 + i];
+    }
+    __syncthreads();
+    fft_line<R1, 16, false>(v, ip, line + i * G::kRegion, twP);
   }
-  __syncthreads();
 }
 
 // The modulus or update pass over one tile (kB lines) of start `start`: generic_tile's
@@ -164,29 +241,29 @@ __device__ __forceinline__ void smooth_inv(float2* v, int i, bool inA, bool inB,
 // persistent launches): device-scope loads.
 template <int R1, int R2, bool COL, Mode M, bool OWN_TW = true, bool COH = false>
 __device__ __forceinline__ void smooth_tile(const PassArgs& p, long long lines, int start, int tile, float2* lds,
-                                            float2* tw, double* red, int slot) {
+                                            float2* tw, float2* twP, double* red, int slot) {
   static_assert(M == Mode::kModulus || M == Mode::kUpdate, "the iteration passes only");
   using G = SmoothGeom<R1, R2, COL>;
   constexpr int N = G::kN;
   using S = Storage<float2>;
   const int tid = threadIdx.x;
-  int b, i;
+  int b, t;  // the thread's line in the tile, and its index within the line
   if constexpr (COL) {
     b = tid % G::kB;
-    i = tid / G::kB;
+    t = tid / G::kB;
   } else {
-    i = tid % G::kT;
+    t = tid % G::kT;
     b = tid / G::kT;
   }
   const long long L = static_cast<long long>(tile) * G::kB + b;
   const bool valid = L < lines;
-  const bool inA = i < R2, inB = i < R1;
+  const bool inA = t < G::kTA, inB = t < R1;
   // Loads are unconditional (all in flight at once): lanes past the last line read line 0
   // of the start, idle lanes of a layout read its thread 0's positions; neither stores.
   const long long Lc = valid ? L : 0;
   const int inner = p.inner;
   const long long base = (Lc / inner) * N * inner + Lc % inner;  // element 0 of the line within the start
-  const int ia = inA ? i : 0, ib = inB ? i : 0;
+  const int ta = inA ? t : 0, tb = inB ? t : 0;
   const long long soff = static_cast<long long>(start) * p.n;
   float2* w = static_cast<float2*>(p.w) + soff;
   float2* x = static_cast<float2*>(p.x) + soff;
@@ -196,12 +273,17 @@ __device__ __forceinline__ void smooth_tile(const PassArgs& p, long long lines, 
     const long long arow = p.amp_stride * ((p.err_offset + start) / (p.amp_group > 1 ? p.amp_group : 1));
     float amp[R2];
 #pragma unroll
-    for (int m = 0; m < R1; ++m) v[m] = S::ld<COH>(w, static_cast<size_t>(base + static_cast<long long>(ia + m * R2) * inner));
+    for (int m = 0; m < G::kA; ++m) {
+      v[m] = S::ld<COH>(w, static_cast<size_t>(base + static_cast<long long>(G::posA(ta, m)) * inner));
+    }
 #pragma unroll
-    for (int m = 0; m < R2; ++m) amp[m] = p.amp[arow + base + static_cast<long long>(ib + m * R1) * inner];
-    if constexpr (OWN_TW) smooth_twiddles<N, G::kThreads>(tw);  // while the loads are in flight
+    for (int m = 0; m < R2; ++m) amp[m] = p.amp[arow + base + static_cast<long long>(G::posB(tb, m)) * inner];
+    if constexpr (OWN_TW) {  // while the loads are in flight
+      smooth_twiddles<N, G::kThreads>(tw);
+      if constexpr (G::kEngine) smooth_twiddles<R1, G::kThreads>(twP);
+    }
     __syncthreads();
-    smooth_fwd<R1, R2>(v, i, inA, inB, line, tw);
+    smooth_fwd<R1, R2>(v, t, line, tw, twP);
     double acc = 0.0;
     if (valid && inB) {
       const float io = p.in_scale * p.out_scale;
@@ -211,7 +293,7 @@ __device__ __forceinline__ void smooth_tile(const PassArgs& p, long long lines, 
     const double ws = wave_sum(acc);
     if ((tid & 63) == 0) red[tid >> 6] = ws;
     if (tid == 0 && tile == 0 && start == 0 && p.counter) atomicAdd(p.counter, 1);
-    smooth_inv<R1, R2>(v, i, inA, inB, line, tw);  // its barriers also publish red
+    smooth_inv<R1, R2>(v, t, line, tw, twP);  // its barriers also publish red
     if (tid == 0) {
       double t = 0.0;
 #pragma unroll
@@ -220,38 +302,43 @@ __device__ __forceinline__ void smooth_tile(const PassArgs& p, long long lines, 
     }
     if (valid && inA) {
 #pragma unroll
-      for (int m = 0; m < R1; ++m) w[base + static_cast<long long>(i + m * R2) * inner] = v[m];  // out_scale in io
+      for (int m = 0; m < G::kA; ++m) w[base + static_cast<long long>(G::posA(t, m)) * inner] = v[m];  // out_scale in io
     }
   } else {
     const std::uint8_t* sup = p.support + soff;
-    float2 xo[R1];
-    bool inside[R1];
+    float2 xo[G::kA];
+    bool inside[G::kA];
 #pragma unroll
-    for (int m = 0; m < R2; ++m) v[m] = S::ld<COH>(w, static_cast<size_t>(base + static_cast<long long>(ib + m * R1) * inner));
+    for (int m = 0; m < R2; ++m) {
+      v[m] = S::ld<COH>(w, static_cast<size_t>(base + static_cast<long long>(G::posB(tb, m)) * inner));
+    }
 #pragma unroll
-    for (int m = 0; m < R1; ++m) {
-      const long long e = base + static_cast<long long>(ia + m * R2) * inner;
+    for (int m = 0; m < G::kA; ++m) {
+      const long long e = base + static_cast<long long>(G::posA(ta, m)) * inner;
       xo[m] = S::ld<COH>(x, static_cast<size_t>(e));
       inside[m] = sup[e] != 0;
     }
-    if constexpr (OWN_TW) smooth_twiddles<N, G::kThreads>(tw);
+    if constexpr (OWN_TW) {
+      smooth_twiddles<N, G::kThreads>(tw);
+      if constexpr (G::kEngine) smooth_twiddles<R1, G::kThreads>(twP);
+    }
     __syncthreads();
-    smooth_inv<R1, R2>(v, i, inA, inB, line, tw);
+    smooth_inv<R1, R2>(v, t, line, tw, twP);
     if (valid && inA) {
 #pragma unroll
-      for (int m = 0; m < R1; ++m) {
+      for (int m = 0; m < G::kA; ++m) {
         const float2 pm = {v[m].x * p.in_scale, v[m].y * p.in_scale};
         float2 nx = pm;
         if (!inside[m]) nx = {p.a * xo[m].x + p.b * pm.x, p.a * xo[m].y + p.b * pm.y};
-        x[base + static_cast<long long>(i + m * R2) * inner] = nx;
+        x[base + static_cast<long long>(G::posA(t, m)) * inner] = nx;
         v[m] = nx;
       }
     }
-    smooth_fwd<R1, R2>(v, i, inA, inB, line, tw);
+    smooth_fwd<R1, R2>(v, t, line, tw, twP);
     if (valid && inB) {
 #pragma unroll
       for (int m = 0; m < R2; ++m) {
-        w[base + static_cast<long long>(i + m * R1) * inner] = {v[m].x * p.out_scale, v[m].y * p.out_scale};
+        w[base + static_cast<long long>(G::posB(t, m)) * inner] = {v[m].x * p.out_scale, v[m].y * p.out_scale};
       }
     }
     if (tile == 0 && tid < 64 && p.errs) {
@@ -271,7 +358,7 @@ template <int R1, int R2, bool COL, Mode M>
 __global__ void __launch_bounds__((SmoothGeom<R1, R2, COL>::kThreads))
     smooth_pass(PassArgs p, long long lines) {
   using G = SmoothGeom<R1, R2, COL>;
-  __shared__ __align__(16) float2 lds[G::kB * G::kStride + G::kN + G::kWaves];
+  __shared__ __align__(16) float2 lds[G::kB * G::kStride + G::kN + G::kTwP + G::kWaves];
   int tile, start;
   if (p.slots > 0) {
     start = blockIdx.x % p.slots;
@@ -282,7 +369,9 @@ __global__ void __launch_bounds__((SmoothGeom<R1, R2, COL>::kThreads))
     tile = blockIdx.x;
   }
   float2* tw = lds + G::kB * G::kStride;
-  smooth_tile<R1, R2, COL, M>(p, lines, start, tile, lds, tw, reinterpret_cast<double*>(tw + G::kN), kSlotFromCounter);
+  float2* twP = tw + G::kN;
+  smooth_tile<R1, R2, COL, M>(p, lines, start, tile, lds, tw, twP, reinterpret_cast<double*>(twP + G::kTwP),
+                              kSlotFromCounter);
 }
 
 // `len` iterations of every start of a launch group on an N x N image whose axis length is
@@ -297,18 +386,20 @@ __global__ void __launch_bounds__((SmoothGeom<R1, R2, true>::kThreads))
   using GR = SmoothGeom<R1, R2, false>;
   static_assert(GC::kThreads == GR::kThreads, "one work-group size for both passes");
   constexpr int kSlots = GC::kB * GC::kStride > GR::kB * GR::kStride ? GC::kB * GC::kStride : GR::kB * GR::kStride;
-  __shared__ __align__(16) float2 lds[kSlots + GC::kN + GC::kWaves];
+  __shared__ __align__(16) float2 lds[kSlots + GC::kN + GC::kTwP + GC::kWaves];
   __shared__ int claim;
   float2* tw = lds + kSlots;
-  double* red = reinterpret_cast<double*>(tw + GC::kN);
-  smooth_twiddles<GC::kN, GC::kThreads>(tw);  // both axes: one table
+  float2* twP = tw + GC::kN;
+  double* red = reinterpret_cast<double*>(twP + GC::kTwP);
+  smooth_twiddles<GC::kN, GC::kThreads>(tw);  // both axes: one table (and one engine table)
+  if constexpr (GC::kEngine) smooth_twiddles<R1, GC::kThreads>(twP);
   persistent_loop(
       p1, q, len, claim,
       [&](int start, int tile) {
-        smooth_tile<R1, R2, true, Mode::kModulus, false, true>(p0, lines0, start, tile, lds, tw, red, 0);
+        smooth_tile<R1, R2, true, Mode::kModulus, false, true>(p0, lines0, start, tile, lds, tw, twP, red, 0);
       },
       [&](int start, int tile, int slot) {
-        smooth_tile<R1, R2, false, Mode::kUpdate, false, true>(p1, lines1, start, tile, lds, tw, red, slot);
+        smooth_tile<R1, R2, false, Mode::kUpdate, false, true>(p1, lines1, start, tile, lds, tw, twP, red, slot);
       });
 }
 

@@ -1,5 +1,5 @@
-"""The smooth-length passes (smooth.hpp: axes N = R1 R2 with R1 in {2, 4, 8, 16} and R2 in
-{3, 5, 15}, the line held in registers) against the oracle: one teacher-forced HIO, RAAR
+"""The smooth-length passes (smooth.hpp: axes N = R1 R2 <= 2048 with R1 a power of two from 2
+to 256 and R2 in {3, 5, 15}, 192 excepted, the line held in registers) against the oracle: one teacher-forced HIO, RAAR
 and ER iteration from the same starts, error and object within 1e-5 relative, on every
 factor pair as a column (strided) axis and as a row axis, beside power-of-two line passes,
 and as the strided z axis of 3D shapes (before the plane pass and before line passes);
@@ -16,10 +16,14 @@ from test_gpu_abi import _problem, rel
 
 pytestmark = pytest.mark.gpu
 
-# every N = R1 R2 on axis 0 (columns) and on axis 1 (rows)
+# every N = R1 R2 on axis 0 (columns) and on axis 1 (rows): R1 <= 16 (both stages in
+# registers), then R1 >= 32 (the power-of-two stage through the radix-16 line engine)
 PAIRS = [(6, 10), (10, 6), (30, 12), (12, 30), (20, 60), (60, 20), (24, 40), (40, 24), (120, 48), (48, 120),
-         (80, 240), (240, 80)]
-MIXED = [(64, 240), (240, 64), (30, 16, 16), (24, 64, 64), (16, 24, 40), (120, 8, 32)]
+         (80, 240), (240, 80),
+         (96, 160), (160, 96), (480, 320), (320, 480), (320, 960), (960, 320), (384, 640), (640, 384),
+         (1920, 16), (16, 1920), (768, 32), (32, 768), (1280, 16), (16, 1280)]
+MIXED = [(64, 240), (240, 64), (30, 16, 16), (24, 64, 64), (16, 24, 40), (120, 8, 32), (96, 16, 16),
+         (16, 32, 480)]
 
 
 @pytest.mark.parametrize("shape", PAIRS + MIXED)
@@ -46,7 +50,7 @@ def test_smooth_teacher_forced_iteration(qpch, shape, schedule):
             assert rel(xs[k], xo) < 1e-5, (shape, kind, k, rel(xs[k], xo))
 
 
-@pytest.mark.parametrize("shape", [(240, 240), (80, 48), (30, 16, 16)])
+@pytest.mark.parametrize("shape", [(240, 240), (80, 48), (30, 16, 16), (480, 96), (1280, 16)])
 def test_smooth_per_iteration_error(qpch, shape):
     """RAAR(30) from given starts: the device's error within max(1e-5, 3 x the spread of the
     oracle's float run against its FP64 run) of the oracle's float run at every iteration."""

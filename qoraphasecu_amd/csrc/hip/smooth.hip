@@ -8,29 +8,26 @@
 namespace qpch {
 namespace {
 
-// The factor pairs: R1 a power of two from 2 to 256 times R2 in {3, 5, 15}, up to 2048,
-// except 64 x 3 and 512 x 3.  Those two (192 and 1536) measured slower than the generic
-// kernel: layout A keeps only 3 of every 16 threads of a line busy through the radix-R1
-// stage (192^2 x 8 starts: 25.2 against 22.4 us per iteration; 1536^2 x 2: 294 against
-// 236).
+// The factor pairs: R1 a power of two from 2 to 512 times R2 in {3, 5, 15}, up to 2048.
 #define QPCH_FOR_EACH_SMOOTH(X)                                                                  \
   X(2, 3) X(2, 5) X(2, 15) X(4, 3) X(4, 5) X(4, 15) X(8, 3) X(8, 5) X(8, 15) X(16, 3) X(16, 5)   \
-  X(16, 15) X(32, 3) X(32, 5) X(32, 15) X(64, 5) X(64, 15) X(128, 3) X(128, 5) X(128, 15)        \
-  X(256, 3) X(256, 5)
-// Square images with a persistent launch: R1 up to 16, and 32 x 5.  Above about 256 points
-// the per-pass launches measured faster (480^2 x 8: 39 against 64 us per iteration; 160^2
-// x 8: 14.0 against 11.1 the other way), and the persistent 32 x 3 kernel spills.
+  X(16, 15) X(32, 3) X(32, 5) X(32, 15) X(64, 3) X(64, 5) X(64, 15) X(128, 3) X(128, 5) X(128, 15)        \
+  X(256, 3) X(256, 5) X(512, 3)
+// Square images with a persistent launch: R1 up to 16, and 96 = 32 x 3 and 160 = 32 x 5
+// (us per iteration of 8 starts, per pass against persistent: 96^2 10.3 / 7.8, 160^2
+// 11.8 / 8.4, 240^2 16.9 / 15.2).  The others measured faster per pass: 192^2 13.8 / 16.8,
+// 480^2 39 / 64.
 #define QPCH_FOR_EACH_SMOOTH_PERSISTENT(X)                                                       \
   X(2, 3) X(2, 5) X(2, 15) X(4, 3) X(4, 5) X(4, 15) X(8, 3) X(8, 5) X(8, 15) X(16, 3) X(16, 5)   \
-  X(16, 15) X(32, 5)
+  X(16, 15) X(32, 3) X(32, 5)
 
-bool persistent_pair(int r1, int r2) { return r1 <= 16 || (r1 == 32 && r2 == 5); }
+bool persistent_pair(int r1, int r2) { return r1 <= 16 || (r1 == 32 && r2 <= 5); }
 
 bool factors(int N, int& r1, int& r2) {
   for (int r : {15, 5, 3}) {
     if (N % r) continue;
     const int q = N / r;
-    if (q >= 2 && q <= 256 && (q & (q - 1)) == 0 && N <= dev::kGenericMaxN && !(r == 3 && q == 64)) {
+    if (q >= 2 && q <= 512 && (q & (q - 1)) == 0 && N <= dev::kGenericMaxN) {
       r1 = q;
       r2 = r;
       return true;

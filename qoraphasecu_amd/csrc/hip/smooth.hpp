@@ -38,19 +38,27 @@ __host__ __device__ constexpr int smooth_pow2_ceil(int v) { return v <= 1 ? 1 : 
 
 // R1 <= 16: the power-of-two stage is one in-register DFT (layouts A and B above).  R1 >= 32
 // (the line engine): layout A is R2 interleaved subsequences x[i + p R2], p < R1, each
-// transformed by the radix-16 engine (fft_line, fft_engine.hpp) on R1/16 threads of 16
-// registers: thread (i, i') holds x[i + (i' + m R1/16) R2], m < 16.  The exchange to layout
-// B (thread j < R1 holds X[j + m R1], m < R2) goes through LDS at k R2 + i, and thread j
-// twiddles S_i[j] by W_N^(i j) before the radix-R2 DFT over i.  The inverse mirrors it.
+// transformed by the line engine (fft_line, fft_engine.hpp) at radix RA on R1/RA threads of
+// RA registers: thread (i, i') holds x[i + (i' + m R1/RA) R2], m < RA.  The exchange to
+// layout B (thread j < R1 holds X[j + m R1], m < R2) goes through LDS at k R2 + i, and
+// thread j twiddles S_i[j] by W_N^(i j) before the radix-R2 DFT over i.  The inverse
+// mirrors it.  RA shrinks with R2 so that more of a line's R1 threads work in layout A:
+// 16 for R2 = 15 (15 of 16 busy), 8 for 5 (5 of 8), 4 for 3 (3 of 4).  With radix 16 for
+// every R2, 192 = 64 x 3 and 1536 = 512 x 3 were slower than the generic kernel; radix 8
+// for R2 = 3 measured 5-13% slower than 4.
+__host__ __device__ constexpr int smooth_engine_radix(int r2) {
+  return r2 == 15 ? 16 : r2 == 5 ? 8 : 4;
+}
 template <int R1, int R2, bool COL>
 struct SmoothGeom {
   static_assert(R1 >= 2 && R1 <= 512 && (R1 & (R1 - 1)) == 0, "R1: a power of two from 2 to 512");
   static_assert(R2 == 3 || R2 == 5 || R2 == 15, "R2: 3, 5 or 15");
   static constexpr int kN = R1 * R2;
   static_assert(kN <= kGenericMaxN, "the generic kernels' length limit");
-  static constexpr bool kEngine = R1 > 16;                     // the radix-16 line engine for R1
-  static constexpr int kSub = kEngine ? R1 / 16 : 1;           // threads per subsequence (layout A)
-  static constexpr int kA = kEngine ? 16 : R1;                 // registers in layout A
+  static constexpr bool kEngine = R1 > 16;                     // the line engine for R1
+  static constexpr int kRA = smooth_engine_radix(R2);          // its radix
+  static constexpr int kSub = kEngine ? R1 / kRA : 1;          // threads per subsequence (layout A)
+  static constexpr int kA = kEngine ? kRA : R1;                // registers in layout A
   static constexpr int kTA = R2 * kSub;                        // threads holding layout A
   static constexpr int kV = kA > R2 ? kA : R2;                 // registers per thread
   static constexpr int kT = smooth_pow2_ceil(kTA > R1 ? kTA : R1);  // threads per line
@@ -58,14 +66,16 @@ struct SmoothGeom {
   static constexpr int kThreads = kB * kT;
   static constexpr int kWaves = (kThreads + 63) / 64;
   // line scratch: R1 <= 16, the line with one pad slot per R1 elements; the engine, one
-  // padded fft_line region per subsequence slot i = t / kSub (all kT / kSub = 16 of them, so
-  // that threads without a subsequence transform in a region of their own)
-  static constexpr int kRegion = R1 + R1 / 16;
+  // padded fft_line region per subsequence slot i = t / kSub (all kT / kSub = RA of them, so
+  // that threads without a subsequence transform in a region of their own; RA + 1 >= R2, so
+  // they also hold the N-element exchange)
+  static constexpr int kRegion = R1 + R1 / kRA;
   static constexpr int kLine = kEngine ? (kT / kSub) * kRegion : kN + kN / R1;
   static constexpr int kStride = COL ? (kLine | 1) : kLine;    // odd for column tiles (banks)
   static constexpr int kTwP = kEngine ? R1 : 0;                // W_R1 table of the engine
   static_assert(kThreads % 64 == 0, "whole waves (the error partial's shuffles)");
   static_assert(!kEngine || kT == R1, "one thread per layout-B column");
+  static_assert(!kEngine || kLine >= kN, "the exchange fits the engine regions");
   // line position held in layout-A register m by thread t (t < kTA), and in layout B (t < R1)
   __device__ static int posA(int t, int m) {
     if constexpr (kEngine) {
@@ -160,10 +170,10 @@ __device__ __forceinline__ void smooth_fwd(float2* v, int t, float2* line, const
   } else {
     constexpr int kSub = G::kSub;
     const int i = t / kSub, ip = t % kSub;
-    fft_line<R1, 16, true>(v, ip, line + i * G::kRegion, twP);  // S_i, natural order (ends past its last LDS read)
+    fft_line<R1, G::kRA, true>(v, ip, line + i * G::kRegion, twP);  // S_i, natural order (ends past its last LDS read)
     if (t < G::kTA) {
 #pragma unroll
-      for (int m = 0; m < 16; ++m) line[(ip + m * kSub) * R2 + i] = v[m];
+      for (int m = 0; m < G::kRA; ++m) line[(ip + m * kSub) * R2 + i] = v[m];
     }
     __syncthreads();
     if (t < R1) {
@@ -225,10 +235,10 @@ __device__ __forceinline__ void smooth_inv(float2* v, int t, float2* line, const
     const int i = t / kSub, ip = t % kSub;
     if (t < G::kTA) {
 #pragma unroll
-      for (int m = 0; m < 16; ++m) v[m] = line[(ip + m * kSub) * R2 + i];
+      for (int m = 0; m < G::kRA; ++m) v[m] = line[(ip + m * kSub) * R2 + i];
     }
     __syncthreads();
-    fft_line<R1, 16, false>(v, ip, line + i * G::kRegion, twP);
+    fft_line<R1, G::kRA, false>(v, ip, line + i * G::kRegion, twP);
   }
 }
 

@@ -10,8 +10,7 @@
 namespace qpch {
 
 // Whether an axis of length N runs the modulus and update passes in smooth_pass: N = R1 R2
-// <= 2048 with R1 a power of two from 2 to 256 and R2 in {3, 5, 15} (R2 = 15 preferred),
-// except 192 = 64 x 3.
+// <= 2048 with R1 a power of two from 2 to 512 and R2 in {3, 5, 15} (R2 = 15 preferred).
 bool smooth_axis(int N);
 
 // Launches smooth_pass for the modulus (Mode::kModulus) or update (Mode::kUpdate) pass of
@@ -22,7 +21,7 @@ hipError_t smooth_launch(int N, bool col, dev::Mode m, const dev::PassArgs& a, l
                          hipStream_t s);
 
 // The persistent launch for square images N x N of smooth axis length (twopass_smooth; the
-// lengths up to 240 and 160 = 32 x 5):
+// lengths R1 R2 with R1 up to 16, and 96 and 160):
 // whether a kernel exists, and the launch of `len` iterations with `workers` work-groups
 // (0: as many per XCD as the busiest XCD has tiles per pass, at most what fits the device
 // at once).  q.tiles0 / q.tiles1: the tiles per start of the two passes.

@@ -1,0 +1,70 @@
+// The chirp-z passes (chirp.hpp), instantiated per transform length M, axis kind and mode.
+#include "chirp_pass.hpp"
+
+#include "chirp.hpp"
+#include "smooth_pass.hpp"
+
+namespace qpch {
+namespace {
+
+template <int M, bool COL, dev::Mode MODE>
+void launch(const dev::PassArgs& a, long long lines, int n, dim3 grid, const float2* chirp, const float2* bhat,
+            hipStream_t s) {
+  hipLaunchKernelGGL((dev::chirp_pass<M, COL, MODE>), grid, dim3(dev::ChirpGeom<M, COL>::kThreads), 0, s, a, lines,
+                     n, chirp, bhat);
+}
+
+template <int M>
+void launch_m(bool col, dev::Mode m, const dev::PassArgs& a, long long lines, int n, dim3 grid, const float2* chirp,
+              const float2* bhat, hipStream_t s) {
+  if (col) {
+    if (m == dev::Mode::kModulus) {
+      launch<M, true, dev::Mode::kModulus>(a, lines, n, grid, chirp, bhat, s);
+    } else {
+      launch<M, true, dev::Mode::kUpdate>(a, lines, n, grid, chirp, bhat, s);
+    }
+  } else {
+    if (m == dev::Mode::kModulus) {
+      launch<M, false, dev::Mode::kModulus>(a, lines, n, grid, chirp, bhat, s);
+    } else {
+      launch<M, false, dev::Mode::kUpdate>(a, lines, n, grid, chirp, bhat, s);
+    }
+  }
+}
+
+}  // namespace
+
+int chirp_length(int N) {
+  int m = 1;
+  while (m < 2 * N - 1) m *= 2;
+  return m;
+}
+
+bool chirp_axis(int N) {
+  if (N < 2 || N > dev::kGenericMaxN || (N & (N - 1)) == 0 || smooth_axis(N)) return false;
+  int rest = N;
+  for (int f = 2; f <= 64 && rest > 1; ++f) {
+    while (rest % f == 0) rest /= f;
+  }
+  return rest > 1;  // a prime factor above 64 remains
+}
+
+int chirp_lines(int N) { return dev::chirp_lines_per_block(chirp_length(N)); }
+
+hipError_t chirp_launch(int N, bool col, dev::Mode m, const dev::PassArgs& a, long long lines, dim3 grid,
+                        const void* chirp, const void* bhat, hipStream_t s) {
+  if (!chirp_axis(N) || (m != dev::Mode::kModulus && m != dev::Mode::kUpdate)) return hipErrorInvalidValue;
+  const auto* c = static_cast<const float2*>(chirp);
+  const auto* b = static_cast<const float2*>(bhat);
+  switch (chirp_length(N)) {
+    case 256: launch_m<256>(col, m, a, lines, N, grid, c, b, s); break;
+    case 512: launch_m<512>(col, m, a, lines, N, grid, c, b, s); break;
+    case 1024: launch_m<1024>(col, m, a, lines, N, grid, c, b, s); break;
+    case 2048: launch_m<2048>(col, m, a, lines, N, grid, c, b, s); break;
+    case 4096: launch_m<4096>(col, m, a, lines, N, grid, c, b, s); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace qpch

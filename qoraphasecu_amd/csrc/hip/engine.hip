@@ -26,6 +26,7 @@
 #include "misc.hpp"
 #include "passes.hpp"
 #include "plane.hpp"
+#include "chirp_pass.hpp"
 #include "smooth_pass.hpp"
 #include "twopass.hpp"
 #include "blur.hpp"
@@ -581,7 +582,10 @@ void launch_generic(float2* data, long long n_per_start, int batch, int N, long 
 void launch_generic_pass(int N, Mode m, PassArgs a, int starts, hipStream_t s) {
   if (N > dev::kGenericMaxN) throw InvalidArgument("no generic pass for axis length " + std::to_string(N));
   const long long lines = static_cast<long long>(a.outer) * a.inner;
-  const long long lpb = dev::generic_lines_per_block(N);
+  // lengths with a prime factor above 64 run the iteration passes in chirp_pass, whose
+  // tiles hold chirp_lines(N) lines
+  const bool chirp = (m == Mode::kModulus || m == Mode::kUpdate) && chirp_axis(N);
+  const long long lpb = chirp ? chirp_lines(N) : dev::generic_lines_per_block(N);
   const unsigned tiles = static_cast<unsigned>((lines + lpb - 1) / lpb);
   // Strided axes number their blocks XCD-affine, as the line passes do: a block reads a
   // few adjacent columns, and the other pieces of the same 128-byte lines are then read by
@@ -591,11 +595,18 @@ void launch_generic_pass(int N, Mode m, PassArgs a, int starts, hipStream_t s) {
   a.starts = starts;
   a.slots = (a.inner > 1 && !a.no_xcd) ? xcd_slots(starts) : 0;
   const dim3 grid = a.slots > 0 ? dim3(tiles * a.slots) : dim3(tiles, starts);
-  // Smooth lengths (N = R1 R2, R1 a power of two up to 16, R2 in {3, 5, 15}) run the
+  // Smooth lengths (N = R1 R2, R1 a power of two up to 512, R2 in {3, 5, 15}) run the
   // iteration passes with the line in registers (smooth.hpp): same tiles, partials and
   // numbering
   if ((m == Mode::kModulus || m == Mode::kUpdate) && smooth_axis(N)) {
     QPCH_HIP(smooth_launch(N, a.inner > 1, m, a, lines, grid, s));
+    return;
+  }
+  // Bluestein's algorithm with the power-of-two transforms in registers (chirp.hpp); the
+  // tables exist from solver set-up on, so this lookup allocates nothing
+  if (chirp) {
+    const BluesteinTables& t = bluestein_tables(N, false);
+    QPCH_HIP(chirp_launch(N, a.inner > 1, m, a, lines, grid, t.chirp, t.bhat, s));
     return;
   }
   const dev::GenericPlan plan = generic_plan(N);
@@ -617,7 +628,7 @@ template <int EB>
 int axis0_partials(const Dims& d, int radix) {
   const int o = static_cast<int>(d.outer(0)), in = static_cast<int>(d.inner(0));
   if (generic_axis(d.d[0])) {
-    const long long lpb = dev::generic_lines_per_block(d.d[0]);
+    const long long lpb = chirp_axis(d.d[0]) ? chirp_lines(d.d[0]) : dev::generic_lines_per_block(d.d[0]);
     return static_cast<int>((static_cast<long long>(o) * in + lpb - 1) / lpb);
   }
   return pass_partials<EB>(d.d[0], in > 1, o, in, radix);
@@ -938,7 +949,7 @@ struct SolverImpl {
     if constexpr (std::is_same_v<St, float2>) {
       if (twopass_smooth_supported(dims.d[0], dims.d[1])) return true;
       if (persist_generic && generic_axis(dims.d[0]) && generic_axis(dims.d[1]) && !smooth_axis(dims.d[0]) &&
-          !smooth_axis(dims.d[1])) {
+          !smooth_axis(dims.d[1]) && !chirp_axis(dims.d[0]) && !chirp_axis(dims.d[1])) {
         return twopass_generic_supported(generic_plan(dims.d[0]), generic_plan(dims.d[1]));
       }
     }
@@ -1493,7 +1504,7 @@ struct SolverImpl {
     if (!p.amplitude || !p.mask || !p.support) throw InvalidArgument("amplitude, mask and support are required");
     // Bluestein tables of long prime axes now, so that no graph capture ever makes them
     for (int a = 0; a < dims.nd; ++a) {
-      if (needs_bluestein(dims.d[a])) (void)bluestein_tables(dims.d[a], false);
+      if (needs_bluestein(dims.d[a]) || chirp_axis(dims.d[a])) (void)bluestein_tables(dims.d[a], false);
     }
     if (p.lanes < 0 || p.lanes > 8) throw InvalidArgument("lanes must be between 0 and 8");
     if (p.plane < 0 || p.plane > 2) throw InvalidArgument("plane must be 0, 1 or 2");
@@ -2536,7 +2547,8 @@ int qpch_autotune(const qpch_problem* problem, qpch_tuning* out) {
         // (smooth axes run smooth_pass: the persistent launch of square images of a smooth
         // length runs it too, twopass_generic does not)
         if (qpch::twopass_smooth_supported(d.d[0], d.d[1])) return true;
-        return !qpch::smooth_axis(d.d[0]) && !qpch::smooth_axis(d.d[1]) &&
+        return !qpch::smooth_axis(d.d[0]) && !qpch::smooth_axis(d.d[1]) && !qpch::chirp_axis(d.d[0]) &&
+               !qpch::chirp_axis(d.d[1]) &&
                qpch::twopass_generic_supported(qpch::generic_plan(d.d[0]), qpch::generic_plan(d.d[1]));
       }
       return tuned_radix == 16 && qpch::twopass_supported(d.d[0], d.d[1], storage) && !(storage == QPCH_F32 && image);

@@ -35,18 +35,22 @@ void launch_m(bool col, dev::Mode m, const dev::PassArgs& a, long long lines, in
 }  // namespace
 
 int chirp_length(int N) {
-  int m = 1;
+  int m = 256;  // the kernels' smallest transform (a longer one serves as well)
   while (m < 2 * N - 1) m *= 2;
   return m;
 }
 
 bool chirp_axis(int N) {
-  if (N < 2 || N > dev::kGenericMaxN || (N & (N - 1)) == 0 || smooth_axis(N)) return false;
+  if (N < 32 || N > dev::kGenericMaxN || (N & (N - 1)) == 0 || smooth_axis(N)) return false;
   int rest = N;
-  for (int f = 2; f <= 64 && rest > 1; ++f) {
+  for (int f : {2, 3, 5}) {
     while (rest % f == 0) rest /= f;
   }
-  return rest > 1;  // a prime factor above 64 remains
+  // a prime factor of 7 or more: the generic kernel's stage for it is an O(r^2) sum (above 64
+  // a plain DFT per line, or Bluestein through LDS).  Lengths of 2, 3 and 5 alone keep the
+  // generic kernel's short forms, which measured faster (100^2 x 8: 16.1 against 20.8 us per
+  // iteration; 1000 x 64: 24.2 against 25.8), as do lengths below 32.
+  return rest > 1;
 }
 
 int chirp_lines(int N) { return dev::chirp_lines_per_block(chirp_length(N)); }

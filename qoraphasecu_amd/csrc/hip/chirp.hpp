@@ -1,8 +1,9 @@
-// Fused passes along axes whose length N has a prime factor above 64 (up to 2048), by
-// Bluestein's chirp-z transform with the power-of-two transforms of length M = 2^k >= 2N - 1
-// held in registers by the radix-16 line engine (fft_line, fft_engine.hpp).  generic_pass
-// runs the same algorithm with one radix-4/2 stage per LDS round trip (M = 2048: six each
-// way), and a plain O(N^2) DFT per line below 256.
+// Fused passes along axes whose length N (32 to 2048) has a prime factor of 7 or more, by
+// Bluestein's chirp-z transform with the power-of-two transforms of length M = 2^k >=
+// max(256, 2N - 1) held in registers by the radix-16 line engine (fft_line,
+// fft_engine.hpp).  generic_pass runs such factors as O(r^2) stages up to 64, and above as
+// the same algorithm with one radix-4/2 stage per LDS round trip (M = 2048: six each way)
+// or, below 256, a plain O(N^2) DFT per line.
 //
 // A line lives in the engine's layout throughout: thread t < M/16 holds positions
 // p = t + m M/16, m < 16; positions p >= N hold zero.  The length-N DFT (unnormalised,

@@ -8,12 +8,11 @@
 
 namespace qpch {
 
-// The power of two M >= 2N - 1 of Bluestein's algorithm for a length-N transform.
+// The power of two M >= max(256, 2N - 1) of Bluestein's algorithm for a length-N transform.
 int chirp_length(int N);
 
-// Whether an axis of length N runs the modulus and update passes in chirp_pass: N <= 2048,
-// not a power of two, with a prime factor above 64 (the lengths the generic kernel
-// transforms by Bluestein's algorithm or by a plain DFT per line).
+// Whether an axis of length N runs the modulus and update passes in chirp_pass: 32 <= N <=
+// 2048, not a power of two nor a smooth length, with a prime factor of 7 or more.
 bool chirp_axis(int N);
 
 // Lines per work-group of chirp_pass for an axis of length N (its tiles and error partials).

@@ -428,11 +428,10 @@ struct BluesteinTables {
   const void* bhat = nullptr;
 };
 
-int bluestein_length(int n) {
-  int m = 1;
-  while (m < 2 * n - 1) m *= 2;
-  return m;
-}
+// The padded length of Bluestein's algorithm: a power of two >= 2n - 1, and at least 256
+// (the smallest transform of the chirp-z passes; the generic kernel takes Bluestein from
+// 256 on, where 2n - 1 is larger anyway).  The tables of a length serve both kernels.
+int bluestein_length(int n) { return chirp_length(n); }
 
 // In-place forward DFT of a power-of-two length in double (radix 2, twiddles evaluated
 // directly at every use).

@@ -40,12 +40,13 @@ CASES = [
     ((64, 1024), dict(starts=2, storage="f16")),
     ((64, 64), dict(starts=8, storage="f16", mixed=True)),  # fp16 64^2 (fp32 runs whole images)
     ((256, 256), dict(starts=16, batch=4, lanes=4)),  # four persistent launches at once
-    # twopass_generic: both axes not powers of two nor smooth lengths (fp32)
-    ((252, 252), dict(starts=8)),  # 252 = 4 * 3 * 3 * 7
-    ((56, 44), dict(starts=3, batch=1)),
-    ((67, 42), dict(starts=4)),  # a prime axis below 256: one plain DFT per line
-    ((257, 36), dict(starts=2)),  # Bluestein columns
-    ((36, 1021), dict(starts=2)),  # Bluestein rows, one line per work-group
+    # twopass_generic: both axes of primes 2, 3 and 5 only, neither a power of two nor a
+    # smooth length (fp32)
+    ((100, 100), dict(starts=8)),  # 4 * 5 * 5
+    ((36, 50), dict(starts=3, batch=1)),
+    ((90, 150), dict(starts=4)),
+    ((225, 18), dict(starts=2)),  # 18 lines of 225: ragged tiles
+    ((252, 252), dict(starts=8)),  # a factor 7: the chirp-z passes, per pass either way
     # twopass_smooth: square images of a smooth length (smooth.hpp)
     ((240, 240), dict(starts=8)),  # 16 * 15
     ((48, 48), dict(starts=3, batch=1)),  # 16 * 3

@@ -213,6 +213,26 @@ def test_ensemble_of_other_sizes_equals_separate_runs(qpc):
             assert np.array_equal(a.support, b.support)
 
 
+def test_ensemble_of_prime_factor_sizes_equals_separate_runs(qpc):
+    """Shapes with a prime factor of 7 or more (56 x 44 twice, 131 x 64, 30 x 67: the
+    chirp-z passes, Bluestein in registers) beside a smooth one (48 x 40): stacked per
+    shape, each problem's amplitude row and error slot are its own, and the runs equal
+    each problem alone (fp32)."""
+    cases = [synthetic.triangle((56, 44), 12), synthetic.triangle((56, 44), 10), synthetic.triangle((131, 64), 16),
+             synthetic.triangle((30, 67), 8), synthetic.triangle((48, 40), 12)]
+    recipe = qpc.Recipe(qpc.HIO(10), qpc.Shrinkwrap(), qpc.RAAR(10), qpc.detwin(qpc.HIO(5)), qpc.ER(10))
+    many = qpc.reconstruct_many([c[2] for c in cases], supports=[c[1].astype(bool) for c in cases], recipe=recipe,
+                                starts=2, seed=9, backend="hip", variant="batched")
+    assert len(many) == len(cases)
+    for (obj, sup, inten), ens in zip(cases, many):
+        single = qpc.reconstruct(inten, support=sup.astype(bool), recipe=recipe, starts=2, seed=9, backend="hip",
+                                 variant="batched")
+        for a, b in zip(ens, single):
+            np.testing.assert_allclose(a.errors, b.errors, rtol=1e-6)
+            assert rel(a.object, b.object) < 1e-6
+            assert np.array_equal(a.support, b.support)
+
+
 def test_launch_configuration_does_not_change_results(qpc, qpch):
     """Starts per launch, XCD-affine vs plain numbering and launch groups run side by side
     on their own streams only change which work-group does what: the iterates, errors and

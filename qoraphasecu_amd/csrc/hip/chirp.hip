@@ -14,21 +14,24 @@ void launch(const dev::PassArgs& a, long long lines, int n, dim3 grid, const flo
                      n, chirp, bhat);
 }
 
+template <int M, bool COL>
+void launch_mode(dev::Mode m, const dev::PassArgs& a, long long lines, int n, dim3 grid, const float2* chirp,
+                 const float2* bhat, hipStream_t s) {
+  switch (m) {
+    case dev::Mode::kFwd: return launch<M, COL, dev::Mode::kFwd>(a, lines, n, grid, chirp, bhat, s);
+    case dev::Mode::kInv: return launch<M, COL, dev::Mode::kInv>(a, lines, n, grid, chirp, bhat, s);
+    case dev::Mode::kModulus: return launch<M, COL, dev::Mode::kModulus>(a, lines, n, grid, chirp, bhat, s);
+    default: return launch<M, COL, dev::Mode::kUpdate>(a, lines, n, grid, chirp, bhat, s);
+  }
+}
+
 template <int M>
 void launch_m(bool col, dev::Mode m, const dev::PassArgs& a, long long lines, int n, dim3 grid, const float2* chirp,
               const float2* bhat, hipStream_t s) {
   if (col) {
-    if (m == dev::Mode::kModulus) {
-      launch<M, true, dev::Mode::kModulus>(a, lines, n, grid, chirp, bhat, s);
-    } else {
-      launch<M, true, dev::Mode::kUpdate>(a, lines, n, grid, chirp, bhat, s);
-    }
+    launch_mode<M, true>(m, a, lines, n, grid, chirp, bhat, s);
   } else {
-    if (m == dev::Mode::kModulus) {
-      launch<M, false, dev::Mode::kModulus>(a, lines, n, grid, chirp, bhat, s);
-    } else {
-      launch<M, false, dev::Mode::kUpdate>(a, lines, n, grid, chirp, bhat, s);
-    }
+    launch_mode<M, false>(m, a, lines, n, grid, chirp, bhat, s);
   }
 }
 
@@ -57,7 +60,7 @@ int chirp_lines(int N) { return dev::chirp_lines_per_block(chirp_length(N)); }
 
 hipError_t chirp_launch(int N, bool col, dev::Mode m, const dev::PassArgs& a, long long lines, dim3 grid,
                         const void* chirp, const void* bhat, hipStream_t s) {
-  if (!chirp_axis(N) || (m != dev::Mode::kModulus && m != dev::Mode::kUpdate)) return hipErrorInvalidValue;
+  if (!chirp_axis(N) || m == dev::Mode::kGrad) return hipErrorInvalidValue;
   const auto* c = static_cast<const float2*>(chirp);
   const auto* b = static_cast<const float2*>(bhat);
   switch (chirp_length(N)) {

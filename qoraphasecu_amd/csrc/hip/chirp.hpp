@@ -80,7 +80,7 @@ __device__ __forceinline__ void chirp_dft(float2* v, int t, int n, float2* line,
 template <int M, bool COL, Mode MODE>
 __device__ __forceinline__ void chirp_tile(const PassArgs& p, long long lines, int n, const float2* chirp,
                                            const float2* bhat, int start, int tile, float2* lds) {
-  static_assert(MODE == Mode::kModulus || MODE == Mode::kUpdate, "the iteration passes only");
+  static_assert(MODE != Mode::kGrad, "no gradient mode");
   using G = ChirpGeom<M, COL>;
   float2* tw = lds + G::kB * G::kStride;
   float2* ch = tw + M;
@@ -120,7 +120,22 @@ __device__ __forceinline__ void chirp_tile(const PassArgs& p, long long lines, i
     }
     for (int j = tid; j < n; j += G::kThreads) ch[j] = chirp[j];
   };
-  if constexpr (MODE == Mode::kModulus) {
+  if constexpr (MODE == Mode::kFwd || MODE == Mode::kInv) {
+    // the transforms of the prologue and of the 3D line passes: read src (or the wave),
+    // transform, write the wave scaled
+    const float2* in = p.src ? static_cast<const float2*>(p.src) + soff : w;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) v[m] = in[elem(m)];
+    tables();
+    __syncthreads();
+    chirp_dft<M, MODE == Mode::kInv>(v, t, n, line, tw, ch, bh);
+    if (valid) {
+#pragma unroll
+      for (int m = 0; m < 16; ++m) {
+        if (t + m * G::kT < n) w[elem(m)] = {v[m].x * p.out_scale, v[m].y * p.out_scale};
+      }
+    }
+  } else if constexpr (MODE == Mode::kModulus) {
     const long long arow = p.amp_stride * ((p.err_offset + start) / (p.amp_group > 1 ? p.amp_group : 1));
     float amp[16];
 #pragma unroll

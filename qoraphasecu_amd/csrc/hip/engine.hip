@@ -581,9 +581,9 @@ void launch_generic(float2* data, long long n_per_start, int batch, int N, long 
 void launch_generic_pass(int N, Mode m, PassArgs a, int starts, hipStream_t s) {
   if (N > dev::kGenericMaxN) throw InvalidArgument("no generic pass for axis length " + std::to_string(N));
   const long long lines = static_cast<long long>(a.outer) * a.inner;
-  // lengths with a prime factor above 64 run the iteration passes in chirp_pass, whose
-  // tiles hold chirp_lines(N) lines
-  const bool chirp = (m == Mode::kModulus || m == Mode::kUpdate) && chirp_axis(N);
+  // lengths with a prime factor of 7 or more run their passes in chirp_pass, whose tiles
+  // hold chirp_lines(N) lines
+  const bool chirp = m != Mode::kGrad && chirp_axis(N);
   const long long lpb = chirp ? chirp_lines(N) : dev::generic_lines_per_block(N);
   const unsigned tiles = static_cast<unsigned>((lines + lpb - 1) / lpb);
   // Strided axes number their blocks XCD-affine, as the line passes do: a block reads a
@@ -594,10 +594,9 @@ void launch_generic_pass(int N, Mode m, PassArgs a, int starts, hipStream_t s) {
   a.starts = starts;
   a.slots = (a.inner > 1 && !a.no_xcd) ? xcd_slots(starts) : 0;
   const dim3 grid = a.slots > 0 ? dim3(tiles * a.slots) : dim3(tiles, starts);
-  // Smooth lengths (N = R1 R2, R1 a power of two up to 512, R2 in {3, 5, 15}) run the
-  // iteration passes with the line in registers (smooth.hpp): same tiles, partials and
-  // numbering
-  if ((m == Mode::kModulus || m == Mode::kUpdate) && smooth_axis(N)) {
+  // Smooth lengths (N = R1 R2, R1 a power of two up to 512, R2 in {3, 5, 15}) run their
+  // passes with the line in registers (smooth.hpp): same tiles, partials and numbering
+  if (m != Mode::kGrad && smooth_axis(N)) {
     QPCH_HIP(smooth_launch(N, a.inner > 1, m, a, lines, grid, s));
     return;
   }

@@ -42,20 +42,22 @@ void launch(const dev::PassArgs& a, long long lines, dim3 grid, hipStream_t s) {
                      a, lines);
 }
 
+template <int R1, int R2, bool COL>
+void launch_mode(dev::Mode m, const dev::PassArgs& a, long long lines, dim3 grid, hipStream_t s) {
+  switch (m) {
+    case dev::Mode::kFwd: return launch<R1, R2, COL, dev::Mode::kFwd>(a, lines, grid, s);
+    case dev::Mode::kInv: return launch<R1, R2, COL, dev::Mode::kInv>(a, lines, grid, s);
+    case dev::Mode::kModulus: return launch<R1, R2, COL, dev::Mode::kModulus>(a, lines, grid, s);
+    default: return launch<R1, R2, COL, dev::Mode::kUpdate>(a, lines, grid, s);
+  }
+}
+
 template <int R1, int R2>
 void launch_pair(bool col, dev::Mode m, const dev::PassArgs& a, long long lines, dim3 grid, hipStream_t s) {
   if (col) {
-    if (m == dev::Mode::kModulus) {
-      launch<R1, R2, true, dev::Mode::kModulus>(a, lines, grid, s);
-    } else {
-      launch<R1, R2, true, dev::Mode::kUpdate>(a, lines, grid, s);
-    }
+    launch_mode<R1, R2, true>(m, a, lines, grid, s);
   } else {
-    if (m == dev::Mode::kModulus) {
-      launch<R1, R2, false, dev::Mode::kModulus>(a, lines, grid, s);
-    } else {
-      launch<R1, R2, false, dev::Mode::kUpdate>(a, lines, grid, s);
-    }
+    launch_mode<R1, R2, false>(m, a, lines, grid, s);
   }
 }
 
@@ -102,7 +104,7 @@ bool smooth_axis(int N) {
 hipError_t smooth_launch(int N, bool col, dev::Mode m, const dev::PassArgs& a, long long lines, dim3 grid,
                          hipStream_t s) {
   int r1 = 0, r2 = 0;
-  if (!factors(N, r1, r2) || (m != dev::Mode::kModulus && m != dev::Mode::kUpdate)) return hipErrorInvalidValue;
+  if (!factors(N, r1, r2) || m == dev::Mode::kGrad) return hipErrorInvalidValue;
 #define QPCH_SMOOTH_CASE(A, B) \
   if (r1 == A && r2 == B) {    \
     launch_pair<A, B>(col, m, a, lines, grid, s); \

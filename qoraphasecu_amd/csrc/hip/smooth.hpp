@@ -252,7 +252,7 @@ __device__ __forceinline__ void smooth_inv(float2* v, int t, float2* line, const
 template <int R1, int R2, bool COL, Mode M, bool OWN_TW = true, bool COH = false>
 __device__ __forceinline__ void smooth_tile(const PassArgs& p, long long lines, int start, int tile, float2* lds,
                                             float2* tw, float2* twP, double* red, int slot) {
-  static_assert(M == Mode::kModulus || M == Mode::kUpdate, "the iteration passes only");
+  static_assert(M != Mode::kGrad, "no gradient mode");
   using G = SmoothGeom<R1, R2, COL>;
   constexpr int N = G::kN;
   using S = Storage<float2>;
@@ -279,7 +279,34 @@ __device__ __forceinline__ void smooth_tile(const PassArgs& p, long long lines, 
   float2* x = static_cast<float2*>(p.x) + soff;
   float2* line = lds + b * G::kStride;
   float2 v[G::kV];
-  if constexpr (M == Mode::kModulus) {
+  if constexpr (M == Mode::kFwd || M == Mode::kInv) {
+    // the transforms of the prologue and of the 3D line passes: read src (or the wave),
+    // transform, write the wave scaled; forward A -> B, inverse B -> A
+    const float2* in = p.src ? static_cast<const float2*>(p.src) + soff : w;
+    constexpr bool kF = M == Mode::kFwd;
+#pragma unroll
+    for (int m = 0; m < (kF ? G::kA : R2); ++m) {
+      const int q = kF ? G::posA(ta, m) : G::posB(tb, m);
+      v[m] = in[base + static_cast<long long>(q) * inner];
+    }
+    if constexpr (OWN_TW) {
+      smooth_twiddles<N, G::kThreads>(tw);
+      if constexpr (G::kEngine) smooth_twiddles<R1, G::kThreads>(twP);
+    }
+    __syncthreads();
+    if constexpr (kF) {
+      smooth_fwd<R1, R2>(v, t, line, tw, twP);
+    } else {
+      smooth_inv<R1, R2>(v, t, line, tw, twP);
+    }
+    if (valid && (kF ? inB : inA)) {
+#pragma unroll
+      for (int m = 0; m < (kF ? R2 : G::kA); ++m) {
+        const int q = kF ? G::posB(t, m) : G::posA(t, m);
+        w[base + static_cast<long long>(q) * inner] = {v[m].x * p.out_scale, v[m].y * p.out_scale};
+      }
+    }
+  } else if constexpr (M == Mode::kModulus) {
     const long long arow = p.amp_stride * ((p.err_offset + start) / (p.amp_group > 1 ? p.amp_group : 1));
     float amp[R2];
 #pragma unroll

@@ -9,11 +9,11 @@
 
 namespace qpch {
 
-// Whether an axis of length N runs the modulus and update passes in smooth_pass: N = R1 R2
+// Whether an axis of length N runs its fused passes (all but the gradient) in smooth_pass: N = R1 R2
 // <= 2048 with R1 a power of two from 2 to 512 and R2 in {3, 5, 15} (R2 = 15 preferred).
 bool smooth_axis(int N);
 
-// Launches smooth_pass for the modulus (Mode::kModulus) or update (Mode::kUpdate) pass of
+// Launches smooth_pass for a forward, inverse, modulus or update pass (not Mode::kGrad) of
 // an axis of length N with `lines` lines per start, on the grid generic_pass would use
 // (block numbering from a.slots).  col: the axis is strided (inner > 1).  Returns the
 // launch status; hipErrorInvalidValue when no kernel exists.

@@ -3,7 +3,8 @@ with the power-of-two transforms held in registers by the radix-16 line engine) 
 the oracle: one teacher-forced HIO, RAAR and ER iteration from the same starts, error and
 object within 1e-5 relative, for every transform length M (256 to 4096) as a column
 (strided) and as a row axis, composite lengths with a large prime factor, beside
-power-of-two and smooth passes, and as 3D z axes; and RAAR(30) runs, the per-iteration
+power-of-two and smooth passes, as 3D z axes and as 3D y axes (forward and inverse line
+passes); and RAAR(30) runs, the per-iteration
 error within the bar of test_gpu_per_iteration.py (1e-5, or float rounding's measured
 spread)."""
 
@@ -22,7 +23,9 @@ pytestmark = pytest.mark.gpu
 # 252 = 4 * 9 * 7, 700 = 4 * 25 * 7 (M 256 to 2048)
 SHAPES = [(67, 16), (16, 67), (131, 16), (16, 131), (257, 16), (16, 257), (521, 8), (8, 521), (1021, 4), (4, 1021),
           (2039, 2), (2, 2039), (134, 24), (24, 201), (67, 30), (131, 240), (67, 16, 16), (8, 4, 389), (131, 8, 8),
-          (44, 64), (64, 44), (56, 56), (252, 36), (36, 252), (700, 8), (8, 700), (56, 16, 16)]
+          (44, 64), (64, 44), (56, 56), (252, 36), (36, 252), (700, 8), (8, 700), (56, 16, 16),
+          # y axes of 3D shapes (forward and inverse line passes): 56, 67, 131, 1021
+          (16, 56, 44), (8, 67, 32), (4, 131, 24), (2, 1021, 4)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
@@ -49,7 +52,7 @@ def test_chirp_teacher_forced_iteration(qpch, shape, schedule):
             assert rel(xs[k], xo) < 1e-5, (shape, kind, k, rel(xs[k], xo))
 
 
-@pytest.mark.parametrize("shape", [(131, 64), (64, 1021), (67, 16, 16), (56, 44)])
+@pytest.mark.parametrize("shape", [(131, 64), (64, 1021), (67, 16, 16), (56, 44), (8, 67, 32)])
 def test_chirp_per_iteration_error(qpch, shape):
     """RAAR(30) from given starts: the device's error within max(1e-5, 3 x the spread of the
     oracle's float run against its FP64 run) of the oracle's float run at every iteration."""

@@ -2,7 +2,8 @@
 to 512 and R2 in {3, 5, 15}, the line held in registers) against the oracle: one teacher-forced HIO, RAAR
 and ER iteration from the same starts, error and object within 1e-5 relative, on every
 factor pair as a column (strided) axis and as a row axis, beside power-of-two line passes,
-and as the strided z axis of 3D shapes (before the plane pass and before line passes);
+as the strided z axis of 3D shapes (before the plane pass and before line passes), and as
+the y axis of 3D shapes, whose forward and inverse line passes run the same kernels;
 and a 30-iteration RAAR run per shape kind, the per-iteration error within the bar of
 test_gpu_per_iteration.py's 48 x 40 case (1e-5, or float rounding's measured spread)."""
 
@@ -23,7 +24,9 @@ PAIRS = [(6, 10), (10, 6), (30, 12), (12, 30), (20, 60), (60, 20), (24, 40), (40
          (96, 160), (160, 96), (480, 320), (320, 480), (320, 960), (960, 320), (384, 640), (640, 384),
          (1920, 16), (16, 1920), (768, 32), (32, 768), (1280, 16), (16, 1280), (192, 1536), (1536, 192)]
 MIXED = [(64, 240), (240, 64), (30, 16, 16), (24, 64, 64), (16, 24, 40), (120, 8, 32), (96, 16, 16),
-         (16, 32, 480)]
+         (16, 32, 480),
+         # smooth y axes: the forward and inverse line passes between the modulus and update
+         (16, 48, 40), (8, 240, 64), (12, 96, 30), (4, 640, 8)]
 
 
 @pytest.mark.parametrize("shape", PAIRS + MIXED)
@@ -50,7 +53,7 @@ def test_smooth_teacher_forced_iteration(qpch, shape, schedule):
             assert rel(xs[k], xo) < 1e-5, (shape, kind, k, rel(xs[k], xo))
 
 
-@pytest.mark.parametrize("shape", [(240, 240), (80, 48), (30, 16, 16), (480, 96), (1280, 16)])
+@pytest.mark.parametrize("shape", [(240, 240), (80, 48), (30, 16, 16), (480, 96), (1280, 16), (16, 48, 40)])
 def test_smooth_per_iteration_error(qpch, shape):
     """RAAR(30) from given starts: the device's error within max(1e-5, 3 x the spread of the
     oracle's float run against its FP64 run) of the oracle's float run at every iteration."""

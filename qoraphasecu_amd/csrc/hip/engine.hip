@@ -584,7 +584,8 @@ void launch_generic_pass(int N, Mode m, PassArgs a, int starts, hipStream_t s) {
   // lengths with a prime factor of 7 or more run their passes in chirp_pass, whose tiles
   // hold chirp_lines(N) lines
   const bool chirp = m != Mode::kGrad && chirp_axis(N);
-  const long long lpb = chirp ? chirp_lines(N) : dev::generic_lines_per_block(N);
+  const bool smooth = m != Mode::kGrad && smooth_axis(N);
+  const long long lpb = chirp ? chirp_lines(N) : smooth ? smooth_lines(N) : dev::generic_lines_per_block(N);
   const unsigned tiles = static_cast<unsigned>((lines + lpb - 1) / lpb);
   // Strided axes number their blocks XCD-affine, as the line passes do: a block reads a
   // few adjacent columns, and the other pieces of the same 128-byte lines are then read by
@@ -596,7 +597,7 @@ void launch_generic_pass(int N, Mode m, PassArgs a, int starts, hipStream_t s) {
   const dim3 grid = a.slots > 0 ? dim3(tiles * a.slots) : dim3(tiles, starts);
   // Smooth lengths (N = R1 R2, R1 a power of two up to 512, R2 in {3, 5, 15}) run their
   // passes with the line in registers (smooth.hpp): same tiles, partials and numbering
-  if (m != Mode::kGrad && smooth_axis(N)) {
+  if (smooth) {
     QPCH_HIP(smooth_launch(N, a.inner > 1, m, a, lines, grid, s));
     return;
   }
@@ -626,7 +627,9 @@ template <int EB>
 int axis0_partials(const Dims& d, int radix) {
   const int o = static_cast<int>(d.outer(0)), in = static_cast<int>(d.inner(0));
   if (generic_axis(d.d[0])) {
-    const long long lpb = chirp_axis(d.d[0]) ? chirp_lines(d.d[0]) : dev::generic_lines_per_block(d.d[0]);
+    const long long lpb = chirp_axis(d.d[0])    ? chirp_lines(d.d[0])
+                          : smooth_axis(d.d[0]) ? smooth_lines(d.d[0])
+                                                : dev::generic_lines_per_block(d.d[0]);
     return static_cast<int>((static_cast<long long>(o) * in + lpb - 1) / lpb);
   }
   return pass_partials<EB>(d.d[0], in > 1, o, in, radix);
@@ -976,7 +979,7 @@ struct SolverImpl {
       if constexpr (std::is_same_v<St, float2>) {
         if (twopass_smooth_supported(dims.d[0], dims.d[1])) {
           const int N = dims.d[0];
-          q.tiles0 = q.tiles1 = (N + dev::generic_lines_per_block(N) - 1) / dev::generic_lines_per_block(N);
+          q.tiles0 = q.tiles1 = (N + smooth_lines(N) - 1) / smooth_lines(N);
           QPCH_HIP(twopass_smooth_launch(N, p0, p1, q, len, persist_workers, st));
           return true;
         }

@@ -2,13 +2,15 @@
 #include "smooth_pass.hpp"
 
 #include "smooth.hpp"
+#include "smooth_launch.hpp"
 
 #include <algorithm>
 
 namespace qpch {
 namespace {
 
-// The factor pairs: R1 a power of two from 2 to 512 times R2 in {3, 5, 15}, up to 2048.
+// The factor pairs: R1 a power of two from 2 to 512 times R2 in {3, 5, 15}, up to 2048
+// (R2 in {9, 25, 27, 45}: smooth_sq.hip).
 #define QPCH_FOR_EACH_SMOOTH(X)                                                                  \
   X(2, 3) X(2, 5) X(2, 15) X(4, 3) X(4, 5) X(4, 15) X(8, 3) X(8, 5) X(8, 15) X(16, 3) X(16, 5)   \
   X(16, 15) X(32, 3) X(32, 5) X(32, 15) X(64, 3) X(64, 5) X(64, 15) X(128, 3) X(128, 5) X(128, 15)        \
@@ -21,45 +23,31 @@ namespace {
   X(2, 3) X(2, 5) X(2, 15) X(4, 3) X(4, 5) X(4, 15) X(8, 3) X(8, 5) X(8, 15) X(16, 3) X(16, 5)   \
   X(16, 15) X(32, 3) X(32, 5)
 
-bool persistent_pair(int r1, int r2) { return r1 <= 16 || (r1 == 32 && r2 <= 5); }
+bool persistent_pair(int r1, int r2) {
+  return (r2 == 3 || r2 == 5 || r2 == 15) && (r1 <= 16 || (r1 == 32 && r2 <= 5));
+}
 
+// N = R1 R2 with R2 its odd part: a pair with a kernel.  R2 in {3, 5, 15}: R1 from 2 to
+// 512; 9: R1 from 2 to 128; 25 and 27: R1 8 and 16; 45: R1 16.  The pairs left out ran
+// slower than generic_pass (us per iteration of 8 starts, generic / smooth: 100^2 16.1 /
+// 20.1, 108^2 19.2 / 20.9, 360^2 53.3 / 55.2; R2 = 75 at every R1, 600^2 131 / 318): with R1
+// threads of R2 registers in layout B, a small R1 leaves most of a line's threads idle in
+// the radix-R2 stage.
 bool factors(int N, int& r1, int& r2) {
-  for (int r : {15, 5, 3}) {
-    if (N % r) continue;
-    const int q = N / r;
-    if (q >= 2 && q <= 512 && (q & (q - 1)) == 0 && N <= dev::kGenericMaxN) {
-      r1 = q;
-      r2 = r;
-      return true;
-    }
-  }
-  return false;
+  if (N < 2 || N > dev::kGenericMaxN) return false;
+  int q = 1;
+  while ((N / q) % 2 == 0) q *= 2;
+  const int r = N / q;
+  if (q < 2) return false;
+  const bool ok = r == 3 || r == 5 || r == 15 || (r == 9 && q <= 128) || ((r == 25 || r == 27) && (q == 8 || q == 16)) ||
+                  (r == 45 && q == 16);
+  if (!ok) return false;
+  r1 = q;
+  r2 = r;
+  return true;
 }
 
-template <int R1, int R2, bool COL, dev::Mode M>
-void launch(const dev::PassArgs& a, long long lines, dim3 grid, hipStream_t s) {
-  hipLaunchKernelGGL((dev::smooth_pass<R1, R2, COL, M>), grid, dim3(dev::SmoothGeom<R1, R2, COL>::kThreads), 0, s,
-                     a, lines);
-}
-
-template <int R1, int R2, bool COL>
-void launch_mode(dev::Mode m, const dev::PassArgs& a, long long lines, dim3 grid, hipStream_t s) {
-  switch (m) {
-    case dev::Mode::kFwd: return launch<R1, R2, COL, dev::Mode::kFwd>(a, lines, grid, s);
-    case dev::Mode::kInv: return launch<R1, R2, COL, dev::Mode::kInv>(a, lines, grid, s);
-    case dev::Mode::kModulus: return launch<R1, R2, COL, dev::Mode::kModulus>(a, lines, grid, s);
-    default: return launch<R1, R2, COL, dev::Mode::kUpdate>(a, lines, grid, s);
-  }
-}
-
-template <int R1, int R2>
-void launch_pair(bool col, dev::Mode m, const dev::PassArgs& a, long long lines, dim3 grid, hipStream_t s) {
-  if (col) {
-    launch_mode<R1, R2, true>(m, a, lines, grid, s);
-  } else {
-    launch_mode<R1, R2, false>(m, a, lines, grid, s);
-  }
-}
+bool sq_factor(int r2) { return r2 == 9 || r2 == 25 || r2 == 27 || r2 == 45; }
 
 template <int R1, int R2>
 hipError_t launch_twopass(const dev::PassArgs& p0, const dev::PassArgs& p1, const dev::SyncArgs& q, int len,
@@ -101,13 +89,19 @@ bool smooth_axis(int N) {
   return factors(N, r1, r2);
 }
 
+int smooth_lines(int N) {
+  int r1 = 0, r2 = 0;
+  return factors(N, r1, r2) ? dev::smooth_lines_per_block(r1, r2) : 0;
+}
+
 hipError_t smooth_launch(int N, bool col, dev::Mode m, const dev::PassArgs& a, long long lines, dim3 grid,
                          hipStream_t s) {
   int r1 = 0, r2 = 0;
   if (!factors(N, r1, r2) || m == dev::Mode::kGrad) return hipErrorInvalidValue;
+  if (sq_factor(r2)) return smooth_sq_launch(r1, r2, col, m, a, lines, grid, s);
 #define QPCH_SMOOTH_CASE(A, B) \
   if (r1 == A && r2 == B) {    \
-    launch_pair<A, B>(col, m, a, lines, grid, s); \
+    smooth_detail::launch_pair<A, B>(col, m, a, lines, grid, s); \
     return hipGetLastError();  \
   }
   QPCH_FOR_EACH_SMOOTH(QPCH_SMOOTH_CASE)

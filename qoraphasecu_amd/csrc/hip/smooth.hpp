@@ -1,5 +1,6 @@
-// Fused passes along axes of smooth length N = R1 * R2 (R1 a power of two, R2 in {3, 5,
-// 15}), with the line held in registers: a radix-R1 transform (in registers up to 16, the
+// Fused passes along axes of smooth length N = R1 * R2 (R1 a power of two, R2 in {3, 5, 9,
+// 15, 25, 27, 45}; the pairs with a kernel are listed in smooth.hip and smooth_sq.hip),
+// with the line held in registers: a radix-R1 transform (in registers up to 16, the
 // radix-16 line engine from 32 on), one LDS exchange, one in-register radix-R2 stage,
 // where generic_pass (generic.hpp) runs one radix <= 5 stage per LDS round trip and
 // barrier (240 = 16 * 15: two stages instead of four each way).
@@ -26,6 +27,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+#include <utility>
+
 #include "fft_engine.hpp"
 #include "generic.hpp"
 #include "passes.hpp"
@@ -47,22 +51,42 @@ __host__ __device__ constexpr int smooth_pow2_ceil(int v) { return v <= 1 ? 1 : 
 // every R2, 192 = 64 x 3 and 1536 = 512 x 3 were slower than the generic kernel; radix 8
 // for R2 = 3 measured 5-13% slower than 4.
 __host__ __device__ constexpr int smooth_engine_radix(int r2) {
-  return r2 == 15 ? 16 : r2 == 5 ? 8 : 4;
+  return r2 >= 9 ? 16 : r2 == 5 ? 8 : 4;
 }
+// The odd factors R2: 3, 5 and 15, and 9, 25, 27 and 45 (lengths of 2, 3 and 5 with a
+// squared odd prime).  Only R2 <= 15 runs on the line engine (R1 >= 32), whose exchange
+// needs RA + 1 >= R2.
+__host__ __device__ constexpr bool smooth_odd_factor(int r2) {
+  return r2 == 3 || r2 == 5 || r2 == 15 || r2 == 9 || r2 == 25 || r2 == 27 || r2 == 45;
+}
+// Threads per line (layout A's busy threads, or R1, rounded up to a power of two), and
+// lines per work-group: generic_pass's count for R2 in {3, 5, 15} (whole waves for every
+// such pair); for the larger R2, 256 threads (generic_pass's count does not always make
+// whole waves, and with up to 768 threads the radix-45 and -75 stages spilled registers).
+__host__ __device__ constexpr int smooth_threads_per_line(int r1, int r2) {
+  const int ta = r2 * (r1 > 16 ? r1 / smooth_engine_radix(r2) : 1);
+  return smooth_pow2_ceil(ta > r1 ? ta : r1);
+}
+__host__ __device__ constexpr int smooth_lines_per_block(int r1, int r2) {
+  const int t = smooth_threads_per_line(r1, r2);
+  return r2 == 3 || r2 == 5 || r2 == 15 ? generic_lines_per_block(r1 * r2) : (t >= 256 ? 1 : 256 / t);
+}
+
 template <int R1, int R2, bool COL>
 struct SmoothGeom {
   static_assert(R1 >= 2 && R1 <= 512 && (R1 & (R1 - 1)) == 0, "R1: a power of two from 2 to 512");
-  static_assert(R2 == 3 || R2 == 5 || R2 == 15, "R2: 3, 5 or 15");
+  static_assert(smooth_odd_factor(R2), "R2: 3, 5, 9, 15, 25, 27 or 45");
   static constexpr int kN = R1 * R2;
   static_assert(kN <= kGenericMaxN, "the generic kernels' length limit");
   static constexpr bool kEngine = R1 > 16;                     // the line engine for R1
+  static_assert(!kEngine || R2 <= 15, "the line engine's exchange holds R2 <= RA + 1 subsequences");
   static constexpr int kRA = smooth_engine_radix(R2);          // its radix
   static constexpr int kSub = kEngine ? R1 / kRA : 1;          // threads per subsequence (layout A)
   static constexpr int kA = kEngine ? kRA : R1;                // registers in layout A
   static constexpr int kTA = R2 * kSub;                        // threads holding layout A
   static constexpr int kV = kA > R2 ? kA : R2;                 // registers per thread
-  static constexpr int kT = smooth_pow2_ceil(kTA > R1 ? kTA : R1);  // threads per line
-  static constexpr int kB = generic_lines_per_block(kN);       // lines per work-group (as generic_pass)
+  static constexpr int kT = smooth_threads_per_line(R1, R2);  // threads per line
+  static constexpr int kB = smooth_lines_per_block(R1, R2);    // lines per work-group
   static constexpr int kThreads = kB * kT;
   static constexpr int kWaves = (kThreads + 63) / 64;
   // line scratch: R1 <= 16, the line with one pad slot per R1 elements; the engine, one
@@ -129,10 +153,129 @@ __device__ __forceinline__ void dft15(float2* v) {
   }
 }
 
+// sin and cos of x, |x| <= pi, in double at compile time: Taylor series to 30 terms (the
+// last one below 1e-18 of the sum), for the twiddle constants of the stages below.
+constexpr double ce_sin(double x) {
+  double term = x, sum = x;
+  for (int k = 1; k < 30; ++k) {
+    term *= -x * x / static_cast<double>((2 * k) * (2 * k + 1));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double ce_cos(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int k = 1; k < 30; ++k) {
+    term *= -x * x / static_cast<double>((2 * k - 1) * (2 * k));
+    sum += term;
+  }
+  return sum;
+}
+// The angle 2 pi j / R reduced to (-pi, pi].
+constexpr double ce_angle(int j, int R) {
+  const int r = j % R;
+  return 2.0 * 3.14159265358979323846 * static_cast<double>(2 * r > R ? r - R : r) / static_cast<double>(R);
+}
+constexpr int ce_inv_mod(int a, int m) {
+  for (int x = 1; x < m; ++x) {
+    if ((a * x) % m == 1) return x;
+  }
+  return 0;
+}
+
+template <int... I, class F>
+__device__ __forceinline__ void unroll_seq(std::integer_sequence<int, I...>, F&& f) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+// f(integral_constant<int, i>) for i < N: indices usable in constant expressions
+template <int N, class F>
+__device__ __forceinline__ void unroll(F&& f) {
+  unroll_seq(std::make_integer_sequence<int, N>{}, f);
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft_odd(float2* v);
+
+// Cooley-Tukey R = P Q in registers: with n = n1 + P n2 and k = Q k1 + k2,
+//   Y[n1][k2] = DFT_Q over n2 of x[n1 + P n2],  Y[n1][k2] *= W_R^(n1 k2),
+//   X[Q k1 + k2] = DFT_P over n1 of Y[n1][k2];
+// the twiddles are compile-time constants (W_R^j rounded once from double).
+template <int P, int Q, bool INV>
+__device__ __forceinline__ void dft_ct(float2* v) {
+  constexpr int R = P * Q;
+  float2 y[P][Q];
+#pragma unroll
+  for (int n1 = 0; n1 < P; ++n1) {
+#pragma unroll
+    for (int n2 = 0; n2 < Q; ++n2) y[n1][n2] = v[n1 + P * n2];
+    dft_odd<Q, INV>(y[n1]);
+  }
+  unroll<P>([&](auto n1) {
+    unroll<Q>([&](auto k2) {
+      constexpr int j = decltype(n1)::value * decltype(k2)::value;
+      if constexpr (j % R != 0) {
+        constexpr float c = static_cast<float>(ce_cos(ce_angle(j, R)));
+        constexpr float sn = static_cast<float>(ce_sin(ce_angle(j, R)));
+        y[n1][k2] = cmul(y[n1][k2], float2{c, INV ? sn : -sn});
+      }
+    });
+  });
+#pragma unroll
+  for (int k2 = 0; k2 < Q; ++k2) {
+    float2 t[P];
+#pragma unroll
+    for (int n1 = 0; n1 < P; ++n1) t[n1] = y[n1][k2];
+    dft_odd<P, INV>(t);
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) v[Q * k1 + k2] = t[k1];
+  }
+}
+
+// Prime-factor (Good-Thomas) form of N = N1 N2, gcd 1: input n1, n2 at (N2 n1 + N1 n2) mod N,
+// DFT_N1 over n1, DFT_N2 over n2, no twiddles, output k1, k2 at (a k1 + b k2) mod N with
+// a = N2 (N2^-1 mod N1), b = N1 (N1^-1 mod N2) (dft15 is this form for 3 x 5).
+template <int N1, int N2, bool INV>
+__device__ __forceinline__ void dft_pfa(float2* v) {
+  constexpr int N = N1 * N2;
+  constexpr int kA = N2 * ce_inv_mod(N2 % N1, N1), kB = N1 * ce_inv_mod(N1 % N2, N2);
+  static_assert((kA % N1) == 1 && (kA % N2) == 0 && (kB % N2) == 1 && (kB % N1) == 0, "coprime factors");
+  float2 u[N1][N2];
+#pragma unroll
+  for (int n1 = 0; n1 < N1; ++n1) {
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) u[n1][n2] = v[(N2 * n1 + N1 * n2) % N];
+  }
+#pragma unroll
+  for (int n2 = 0; n2 < N2; ++n2) {
+    float2 t[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) t[n1] = u[n1][n2];
+    dft_odd<N1, INV>(t);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) u[k1][n2] = t[k1];
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) dft_odd<N2, INV>(u[k1]);
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) {
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) v[(kA * k1 + kB * k2) % N] = u[k1][k2];
+  }
+}
+
+// In-register DFT of the odd radices of the smooth passes: 3 and 5 (short forms), 15
+// (prime factor 3 x 5), 9 = 3 x 3, 25 = 5 x 5 and 27 = 3 x 9 (Cooley-Tukey), 45 = 9 x 5
+// (prime factor).
 template <int R, bool INV>
 __device__ __forceinline__ void dft_odd(float2* v) {
   if constexpr (R == 15) {
     dft15<INV>(v);
+  } else if constexpr (R == 45) {
+    dft_pfa<9, 5, INV>(v);
+  } else if constexpr (R == 9 || R == 27) {
+    dft_ct<3, R / 3, INV>(v);
+  } else if constexpr (R == 25) {
+    dft_ct<5, 5, INV>(v);
   } else {
     dft_small<R, INV>(v);
   }

@@ -9,9 +9,14 @@
 
 namespace qpch {
 
-// Whether an axis of length N runs its fused passes (all but the gradient) in smooth_pass: N = R1 R2
-// <= 2048 with R1 a power of two from 2 to 512 and R2 in {3, 5, 15} (R2 = 15 preferred).
+// Whether an axis of length N runs its fused passes (all but the gradient) in smooth_pass:
+// N = R1 R2 <= 2048 with R1 a power of two and R2 its odd part: R2 in {3, 5, 15} with R1
+// from 2 to 512, R2 = 9 with R1 from 2 to 128, 25 and 27 with R1 8 or 16, 45 with R1 16.
 bool smooth_axis(int N);
+
+// Lines per work-group of smooth_pass on an axis of length N (smooth_axis(N)): generic_pass's
+// count for R2 in {3, 5, 15}, else as many lines as fill 256 threads.
+int smooth_lines(int N);
 
 // Launches smooth_pass for a forward, inverse, modulus or update pass (not Mode::kGrad) of
 // an axis of length N with `lines` lines per start, on the grid generic_pass would use
@@ -19,6 +24,11 @@ bool smooth_axis(int N);
 // launch status; hipErrorInvalidValue when no kernel exists.
 hipError_t smooth_launch(int N, bool col, dev::Mode m, const dev::PassArgs& a, long long lines, dim3 grid,
                          hipStream_t s);
+
+// smooth_launch for R2 in {9, 25, 27, 45} (smooth_sq.hip): the pair (r1, r2) as
+// smooth_axis factors it.
+hipError_t smooth_sq_launch(int r1, int r2, bool col, dev::Mode m, const dev::PassArgs& a, long long lines, dim3 grid,
+                            hipStream_t s);
 
 // The persistent launch for square images N x N of smooth axis length (twopass_smooth; the
 // lengths R1 R2 with R1 up to 16, and 96 and 160):

@@ -1,5 +1,6 @@
-"""The smooth-length passes (smooth.hpp: axes N = R1 R2 <= 2048 with R1 a power of two from 2
-to 512 and R2 in {3, 5, 15}, the line held in registers) against the oracle: one teacher-forced HIO, RAAR
+"""The smooth-length passes (smooth.hpp: axes N = R1 R2 <= 2048 with R1 a power of two and R2
+in {3, 5, 15} (R1 to 512), 9 (R1 to 128), 25 or 27 (R1 8 or 16) or 45 (R1 16), the line
+held in registers) against the oracle: one teacher-forced HIO, RAAR
 and ER iteration from the same starts, error and object within 1e-5 relative, on every
 factor pair as a column (strided) axis and as a row axis, beside power-of-two line passes,
 as the strided z axis of 3D shapes (before the plane pass and before line passes), and as
@@ -22,11 +23,15 @@ pytestmark = pytest.mark.gpu
 PAIRS = [(6, 10), (10, 6), (30, 12), (12, 30), (20, 60), (60, 20), (24, 40), (40, 24), (120, 48), (48, 120),
          (80, 240), (240, 80),
          (96, 160), (160, 96), (480, 320), (320, 480), (320, 960), (960, 320), (384, 640), (640, 384),
-         (1920, 16), (16, 1920), (768, 32), (32, 768), (1280, 16), (16, 1280), (192, 1536), (1536, 192)]
+         (1920, 16), (16, 1920), (768, 32), (32, 768), (1280, 16), (16, 1280), (192, 1536), (1536, 192),
+         # odd factors with a squared prime: R2 = 9 (18 to 1152), 25 (200, 400), 27 (216, 432),
+         # 45 (720)
+         (18, 200), (200, 18), (36, 216), (216, 36), (72, 400), (400, 72), (144, 432), (432, 144), (288, 720),
+         (720, 288), (576, 16), (16, 576), (1152, 16), (16, 1152)]
 MIXED = [(64, 240), (240, 64), (30, 16, 16), (24, 64, 64), (16, 24, 40), (120, 8, 32), (96, 16, 16),
          (16, 32, 480),
          # smooth y axes: the forward and inverse line passes between the modulus and update
-         (16, 48, 40), (8, 240, 64), (12, 96, 30), (4, 640, 8)]
+         (16, 48, 40), (8, 240, 64), (12, 96, 30), (4, 640, 8), (8, 200, 36), (4, 36, 216)]
 
 
 @pytest.mark.parametrize("shape", PAIRS + MIXED)
@@ -53,7 +58,8 @@ def test_smooth_teacher_forced_iteration(qpch, shape, schedule):
             assert rel(xs[k], xo) < 1e-5, (shape, kind, k, rel(xs[k], xo))
 
 
-@pytest.mark.parametrize("shape", [(240, 240), (80, 48), (30, 16, 16), (480, 96), (1280, 16), (16, 48, 40)])
+@pytest.mark.parametrize("shape", [(240, 240), (80, 48), (30, 16, 16), (480, 96), (1280, 16), (16, 48, 40),
+                                   (200, 200), (720, 72), (576, 36), (432, 16)])
 def test_smooth_per_iteration_error(qpch, shape):
     """RAAR(30) from given starts: the device's error within max(1e-5, 3 x the spread of the
     oracle's float run against its FP64 run) of the oracle's float run at every iteration."""

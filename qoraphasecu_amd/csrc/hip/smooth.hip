@@ -4,8 +4,6 @@
 #include "smooth.hpp"
 #include "smooth_launch.hpp"
 
-#include <algorithm>
-
 namespace qpch {
 namespace {
 
@@ -24,7 +22,8 @@ namespace {
   X(16, 15) X(32, 3) X(32, 5)
 
 bool persistent_pair(int r1, int r2) {
-  return (r2 == 3 || r2 == 5 || r2 == 15) && (r1 <= 16 || (r1 == 32 && r2 <= 5));
+  if (r2 == 3 || r2 == 5 || r2 == 15) return r1 <= 16 || (r1 == 32 && r2 <= 5);
+  return r2 == 9 && r1 <= 8;  // 18, 36 and 72 (smooth_sq.hip)
 }
 
 // N = R1 R2 with R2 its odd part: a pair with a kernel.  R2 in {3, 5, 15}: R1 from 2 to
@@ -49,23 +48,6 @@ bool factors(int N, int& r1, int& r2) {
 
 bool sq_factor(int r2) { return r2 == 9 || r2 == 25 || r2 == 27 || r2 == 45; }
 
-template <int R1, int R2>
-hipError_t launch_twopass(const dev::PassArgs& p0, const dev::PassArgs& p1, const dev::SyncArgs& q, int len,
-                          int workers, hipStream_t s) {
-  constexpr int kThreads = dev::SmoothGeom<R1, R2, true>::kThreads;
-  int per_cu = 0, dev = 0, cus = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::twopass_smooth<R1, R2>, kThreads, 0);
-  if (e == hipSuccess) e = hipGetDevice(&dev);
-  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  if (e != hipSuccess) return e;
-  const int cap = std::max(1, per_cu) * cus / q.xcds * q.xcds;
-  const int items = q.xcds * dev::xcd_starts(p1.starts, 0, q.xcds) * std::max(q.tiles0, q.tiles1);
-  const int w = workers > 0 ? workers : std::max(q.xcds, std::min(items, cap));
-  const long long lines = dev::SmoothGeom<R1, R2, true>::kN;  // per start, both passes (square image)
-  hipLaunchKernelGGL((dev::twopass_smooth<R1, R2>), dim3(w), dim3(kThreads), 0, s, p0, p1, lines, lines, q, len);
-  return hipGetLastError();
-}
-
 }  // namespace
 
 bool twopass_smooth_supported(int ny, int nx) {
@@ -77,8 +59,9 @@ hipError_t twopass_smooth_launch(int N, const dev::PassArgs& p0, const dev::Pass
                                  int len, int workers, hipStream_t s) {
   int r1 = 0, r2 = 0;
   if (!factors(N, r1, r2) || !persistent_pair(r1, r2)) return hipErrorInvalidValue;
+  if (sq_factor(r2)) return twopass_smooth_sq_launch(r1, r2, p0, p1, q, len, workers, s);
 #define QPCH_SMOOTH_CASE(A, B) \
-  if (r1 == A && r2 == B) return launch_twopass<A, B>(p0, p1, q, len, workers, s);
+  if (r1 == A && r2 == B) return smooth_detail::launch_twopass<A, B>(p0, p1, q, len, workers, s);
   QPCH_FOR_EACH_SMOOTH_PERSISTENT(QPCH_SMOOTH_CASE)
 #undef QPCH_SMOOTH_CASE
   return hipErrorInvalidValue;

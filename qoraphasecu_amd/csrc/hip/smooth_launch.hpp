@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "smooth.hpp"
 
 namespace qpch::smooth_detail {
@@ -32,6 +34,25 @@ void launch_pair(bool col, dev::Mode m, const dev::PassArgs& a, long long lines,
   } else {
     launch_mode<R1, R2, false>(m, a, lines, grid, s);
   }
+}
+
+// twopass_smooth<R1, R2> for `len` iterations with `workers` work-groups (0: as many per XCD
+// as the busiest XCD has tiles per pass, at most what fits the device at once).
+template <int R1, int R2>
+hipError_t launch_twopass(const dev::PassArgs& p0, const dev::PassArgs& p1, const dev::SyncArgs& q, int len,
+                          int workers, hipStream_t s) {
+  constexpr int kThreads = dev::SmoothGeom<R1, R2, true>::kThreads;
+  int per_cu = 0, dev = 0, cus = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::twopass_smooth<R1, R2>, kThreads, 0);
+  if (e == hipSuccess) e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  const int cap = std::max(1, per_cu) * cus / q.xcds * q.xcds;
+  const int items = q.xcds * dev::xcd_starts(p1.starts, 0, q.xcds) * std::max(q.tiles0, q.tiles1);
+  const int w = workers > 0 ? workers : std::max(q.xcds, std::min(items, cap));
+  const long long lines = dev::SmoothGeom<R1, R2, true>::kN;  // per start, both passes (square image)
+  hipLaunchKernelGGL((dev::twopass_smooth<R1, R2>), dim3(w), dim3(kThreads), 0, s, p0, p1, lines, lines, q, len);
+  return hipGetLastError();
 }
 
 }  // namespace qpch::smooth_detail

@@ -56,6 +56,9 @@ CASES = [
     ((30, 30), dict(starts=8)),  # 2 * 15: two tiles of 16 lines, the second ragged
     ((160, 160), dict(starts=8)),  # 32 * 5: the power-of-two stage through the line engine
     ((96, 96), dict(starts=8)),  # 32 * 3: the line engine at radix 4
+    ((36, 36), dict(starts=8)),  # 4 * 9 (radix 9 = 3 x 3)
+    ((72, 72), dict(starts=3, batch=1)),  # 8 * 9
+    ((18, 18), dict(starts=8)),  # 2 * 9
     ((480, 480), dict(starts=4)),  # 32 * 15: no persistent kernel, the per-pass launches either way
     ((80, 240), dict(starts=4)),  # smooth, not square: the per-pass launches either way
 ]

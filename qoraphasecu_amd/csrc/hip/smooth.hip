@@ -8,7 +8,7 @@ namespace qpch {
 namespace {
 
 // The factor pairs: R1 a power of two from 2 to 512 times R2 in {3, 5, 15}, up to 2048
-// (R2 in {9, 25, 27, 45}: smooth_sq.hip).
+// (R2 in {9, 25, 27, 45}: smooth_sq.hip; 7, 11, 13, 21, 33, 35, 39: smooth_prime.hip).
 #define QPCH_FOR_EACH_SMOOTH(X)                                                                  \
   X(2, 3) X(2, 5) X(2, 15) X(4, 3) X(4, 5) X(4, 15) X(8, 3) X(8, 5) X(8, 15) X(16, 3) X(16, 5)   \
   X(16, 15) X(32, 3) X(32, 5) X(32, 15) X(64, 3) X(64, 5) X(64, 15) X(128, 3) X(128, 5) X(128, 15)        \
@@ -27,7 +27,8 @@ bool persistent_pair(int r1, int r2) {
 }
 
 // N = R1 R2 with R2 its odd part: a pair with a kernel.  R2 in {3, 5, 15}: R1 from 2 to
-// 512; 9: R1 from 2 to 128; 25 and 27: R1 8 and 16; 45: R1 16.  The pairs left out ran
+// 512; 7: R1 from 2 to 256; 9, 11 and 13: R1 from 2 to 128; 21, 25, 27, 33, 35 and 39: R1
+// 8 and 16; 45: R1 16.  The pairs left out ran
 // slower than generic_pass (us per iteration of 8 starts, generic / smooth: 100^2 16.1 /
 // 20.1, 108^2 19.2 / 20.9, 360^2 53.3 / 55.2; R2 = 75 at every R1, 600^2 131 / 318): with R1
 // threads of R2 registers in layout B, a small R1 leaves most of a line's threads idle in
@@ -38,7 +39,8 @@ bool factors(int N, int& r1, int& r2) {
   while ((N / q) % 2 == 0) q *= 2;
   const int r = N / q;
   if (q < 2) return false;
-  const bool ok = r == 3 || r == 5 || r == 15 || (r == 9 && q <= 128) || ((r == 25 || r == 27) && (q == 8 || q == 16)) ||
+  const bool ok = r == 3 || r == 5 || r == 15 || r == 7 || ((r == 9 || r == 11 || r == 13) && q <= 128) ||
+                  ((r == 25 || r == 27 || r == 21 || r == 33 || r == 35 || r == 39) && (q == 8 || q == 16)) ||
                   (r == 45 && q == 16);
   if (!ok) return false;
   r1 = q;
@@ -47,6 +49,7 @@ bool factors(int N, int& r1, int& r2) {
 }
 
 bool sq_factor(int r2) { return r2 == 9 || r2 == 25 || r2 == 27 || r2 == 45; }
+bool prime_factor(int r2) { return r2 == 7 || r2 == 11 || r2 == 13 || r2 == 21 || r2 == 33 || r2 == 35 || r2 == 39; }
 
 }  // namespace
 
@@ -82,6 +85,7 @@ hipError_t smooth_launch(int N, bool col, dev::Mode m, const dev::PassArgs& a, l
   int r1 = 0, r2 = 0;
   if (!factors(N, r1, r2) || m == dev::Mode::kGrad) return hipErrorInvalidValue;
   if (sq_factor(r2)) return smooth_sq_launch(r1, r2, col, m, a, lines, grid, s);
+  if (prime_factor(r2)) return smooth_prime_launch(r1, r2, col, m, a, lines, grid, s);
 #define QPCH_SMOOTH_CASE(A, B) \
   if (r1 == A && r2 == B) {    \
     smooth_detail::launch_pair<A, B>(col, m, a, lines, grid, s); \

@@ -1,5 +1,6 @@
-// Fused passes along axes of smooth length N = R1 * R2 (R1 a power of two, R2 in {3, 5, 9,
-// 15, 25, 27, 45}; the pairs with a kernel are listed in smooth.hip and smooth_sq.hip),
+// Fused passes along axes of smooth length N = R1 * R2 (R1 a power of two, R2 an odd factor
+// of smooth_odd_factor; the pairs with a kernel are listed in smooth.hip, smooth_sq.hip
+// and smooth_prime.hip),
 // with the line held in registers: a radix-R1 transform (in registers up to 16, the
 // radix-16 line engine from 32 on), one LDS exchange, one in-register radix-R2 stage,
 // where generic_pass (generic.hpp) runs one radix <= 5 stage per LDS round trip and
@@ -51,13 +52,15 @@ __host__ __device__ constexpr int smooth_pow2_ceil(int v) { return v <= 1 ? 1 : 
 // every R2, 192 = 64 x 3 and 1536 = 512 x 3 were slower than the generic kernel; radix 8
 // for R2 = 3 measured 5-13% slower than 4.
 __host__ __device__ constexpr int smooth_engine_radix(int r2) {
-  return r2 >= 9 ? 16 : r2 == 5 ? 8 : 4;
+  return r2 >= 9 ? 16 : r2 >= 5 ? 8 : 4;
 }
-// The odd factors R2: 3, 5 and 15, and 9, 25, 27 and 45 (lengths of 2, 3 and 5 with a
-// squared odd prime).  Only R2 <= 15 runs on the line engine (R1 >= 32), whose exchange
-// needs RA + 1 >= R2.
+// The odd factors R2: 3, 5 and 15; 9, 25, 27 and 45 (lengths of 2, 3 and 5 with a squared
+// odd prime); the primes 7, 11 and 13, and 21, 33, 35 and 39.  Only R2 <= 15 runs on the
+// line engine (R1 >= 32), whose exchange needs RA + 1 >= R2 (7: RA 8, 7 of 8 threads busy;
+// 9, 11, 13: RA 16).
 __host__ __device__ constexpr bool smooth_odd_factor(int r2) {
-  return r2 == 3 || r2 == 5 || r2 == 15 || r2 == 9 || r2 == 25 || r2 == 27 || r2 == 45;
+  return r2 == 3 || r2 == 5 || r2 == 15 || r2 == 9 || r2 == 25 || r2 == 27 || r2 == 45 || r2 == 7 || r2 == 11 ||
+         r2 == 13 || r2 == 21 || r2 == 33 || r2 == 35 || r2 == 39;
 }
 // Threads per line (layout A's busy threads, or R1, rounded up to a power of two), and
 // lines per work-group: generic_pass's count for R2 in {3, 5, 15} (whole waves for every
@@ -75,7 +78,7 @@ __host__ __device__ constexpr int smooth_lines_per_block(int r1, int r2) {
 template <int R1, int R2, bool COL>
 struct SmoothGeom {
   static_assert(R1 >= 2 && R1 <= 512 && (R1 & (R1 - 1)) == 0, "R1: a power of two from 2 to 512");
-  static_assert(smooth_odd_factor(R2), "R2: 3, 5, 9, 15, 25, 27 or 45");
+  static_assert(smooth_odd_factor(R2), "R2: 3, 5, 7, 9, 11, 13, 15, 21, 25, 27, 33, 35, 39 or 45");
   static constexpr int kN = R1 * R2;
   static_assert(kN <= kGenericMaxN, "the generic kernels' length limit");
   static constexpr bool kEngine = R1 > 16;                     // the line engine for R1
@@ -263,15 +266,55 @@ __device__ __forceinline__ void dft_pfa(float2* v) {
   }
 }
 
-// In-register DFT of the odd radices of the smooth passes: 3 and 5 (short forms), 15
-// (prime factor 3 x 5), 9 = 3 x 3, 25 = 5 x 5 and 27 = 3 x 9 (Cooley-Tukey), 45 = 9 x 5
-// (prime factor).
+// DFT of an odd prime P in registers by the symmetric short form (dft_small's for 3 and 5,
+// in general): with s_j = x_j + x_(P-j), d_j = x_j - x_(P-j), j = 1 .. (P-1)/2,
+//   X_0 = x_0 + sum s_j,
+//   X_k, X_(P-k) = x_0 + sum_j cos(2 pi j k / P) s_j  -+ i sum_j sin(2 pi j k / P) d_j
+// (upper signs forward), the constants evaluated at compile time.
+template <int P, bool INV>
+__device__ __forceinline__ void dft_prime(float2* v) {
+  constexpr int H = (P - 1) / 2;
+  const float2 a = v[0];
+  float2 sm[H], df[H];
+  float2 x0 = a;
+#pragma unroll
+  for (int j = 1; j <= H; ++j) {
+    sm[j - 1] = {v[j].x + v[P - j].x, v[j].y + v[P - j].y};
+    df[j - 1] = {v[j].x - v[P - j].x, v[j].y - v[P - j].y};
+    x0 = {x0.x + sm[j - 1].x, x0.y + sm[j - 1].y};
+  }
+  unroll<H>([&](auto kk) {
+    constexpr int k = decltype(kk)::value + 1;
+    float2 re = a, im = {0.0f, 0.0f};
+    unroll<H>([&](auto jj) {
+      constexpr int j = decltype(jj)::value + 1;
+      constexpr float c = static_cast<float>(ce_cos(ce_angle(j * k, P)));
+      constexpr float sn = static_cast<float>(ce_sin(ce_angle(j * k, P)));
+      re = {re.x + c * sm[j - 1].x, re.y + c * sm[j - 1].y};
+      im = {im.x + sn * df[j - 1].x, im.y + sn * df[j - 1].y};
+    });
+    const float2 u = rot<INV>(im);
+    v[k] = {re.x + u.x, re.y + u.y};
+    v[P - k] = {re.x - u.x, re.y - u.y};
+  });
+  v[0] = x0;
+}
+
+// In-register DFT of the odd radices of the smooth passes: 3 and 5 (short forms), 7, 11
+// and 13 (dft_prime), 15 (prime factor 3 x 5), 9 = 3 x 3, 25 = 5 x 5 and 27 = 3 x 9
+// (Cooley-Tukey), 21, 33, 35, 39 and 45 = 9 x 5 (prime factor).
 template <int R, bool INV>
 __device__ __forceinline__ void dft_odd(float2* v) {
   if constexpr (R == 15) {
     dft15<INV>(v);
   } else if constexpr (R == 45) {
     dft_pfa<9, 5, INV>(v);
+  } else if constexpr (R == 7 || R == 11 || R == 13) {
+    dft_prime<R, INV>(v);
+  } else if constexpr (R == 21 || R == 33 || R == 39) {
+    dft_pfa<3, R / 3, INV>(v);
+  } else if constexpr (R == 35) {
+    dft_pfa<5, 7, INV>(v);
   } else if constexpr (R == 9 || R == 27) {
     dft_ct<3, R / 3, INV>(v);
   } else if constexpr (R == 25) {

@@ -11,7 +11,8 @@ namespace qpch {
 
 // Whether an axis of length N runs its fused passes (all but the gradient) in smooth_pass:
 // N = R1 R2 <= 2048 with R1 a power of two and R2 its odd part: R2 in {3, 5, 15} with R1
-// from 2 to 512, R2 = 9 with R1 from 2 to 128, 25 and 27 with R1 8 or 16, 45 with R1 16.
+// from 2 to 512, 7 with R1 from 2 to 256, 9, 11 and 13 with R1 from 2 to 128, 21, 25, 27,
+// 33, 35 and 39 with R1 8 or 16, 45 with R1 16.
 bool smooth_axis(int N);
 
 // Lines per work-group of smooth_pass on an axis of length N (smooth_axis(N)): generic_pass's
@@ -29,6 +30,10 @@ hipError_t smooth_launch(int N, bool col, dev::Mode m, const dev::PassArgs& a, l
 // smooth_axis factors it.
 hipError_t smooth_sq_launch(int r1, int r2, bool col, dev::Mode m, const dev::PassArgs& a, long long lines, dim3 grid,
                             hipStream_t s);
+
+// smooth_launch for R2 in {7, 11, 13, 21, 33, 35, 39} (smooth_prime.hip).
+hipError_t smooth_prime_launch(int r1, int r2, bool col, dev::Mode m, const dev::PassArgs& a, long long lines,
+                               dim3 grid, hipStream_t s);
 
 // twopass_smooth_launch for R2 in {9, 25, 27, 45} (smooth_sq.hip).
 hipError_t twopass_smooth_sq_launch(int r1, int r2, const dev::PassArgs& p0, const dev::PassArgs& p1,

@@ -1,0 +1,32 @@
+// The smooth-length passes (smooth.hpp) whose odd factor R2 holds a prime from 7 to 13: 7,
+// 11 and 13 (dft_prime), and 21, 33, 35 and 39 (prime factor forms), in a translation
+// unit of their own so that it compiles beside smooth.hip.  Without them these lengths
+// run the chirp-z passes (chirp.hpp), whose transforms are at least 256 points long.
+#include "smooth_pass.hpp"
+
+#include "smooth.hpp"
+#include "smooth_launch.hpp"
+
+namespace qpch {
+
+// 7 with R1 up to 256 (from 32 on, the line engine at radix 8), 11 and 13 with R1 up to
+// 128 (radix 16), 21, 33, 35 and 39 with R1 8 and 16.
+#define QPCH_FOR_EACH_SMOOTH_PRIME(X)                                                              \
+  X(2, 7) X(4, 7) X(8, 7) X(16, 7) X(32, 7) X(64, 7) X(128, 7) X(256, 7) X(2, 11) X(4, 11) X(8, 11) \
+  X(16, 11) X(32, 11) X(64, 11) X(128, 11) X(2, 13) X(4, 13) X(8, 13) X(16, 13) X(32, 13) X(64, 13) \
+  X(128, 13) X(8, 21) X(16, 21) X(8, 33) X(16, 33) X(8, 35) X(16, 35) X(8, 39) X(16, 39)
+
+hipError_t smooth_prime_launch(int r1, int r2, bool col, dev::Mode m, const dev::PassArgs& a, long long lines,
+                               dim3 grid, hipStream_t s) {
+  if (m == dev::Mode::kGrad) return hipErrorInvalidValue;
+#define QPCH_SMOOTH_CASE(A, B)                                   \
+  if (r1 == A && r2 == B) {                                      \
+    smooth_detail::launch_pair<A, B>(col, m, a, lines, grid, s); \
+    return hipGetLastError();                                    \
+  }
+  QPCH_FOR_EACH_SMOOTH_PRIME(QPCH_SMOOTH_CASE)
+#undef QPCH_SMOOTH_CASE
+  return hipErrorInvalidValue;
+}
+
+}  // namespace qpch

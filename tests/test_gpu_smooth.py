@@ -1,6 +1,6 @@
 """The smooth-length passes (smooth.hpp: axes N = R1 R2 <= 2048 with R1 a power of two and R2
-in {3, 5, 15} (R1 to 512), 9 (R1 to 128), 25 or 27 (R1 8 or 16) or 45 (R1 16), the line
-held in registers) against the oracle: one teacher-forced HIO, RAAR
+in {3, 5, 15} (R1 to 512), 7 (R1 to 256), 9, 11 or 13 (R1 to 128), 21, 25, 27, 33, 35 or
+39 (R1 8 or 16) or 45 (R1 16), the line held in registers) against the oracle: one teacher-forced HIO, RAAR
 and ER iteration from the same starts, error and object within 1e-5 relative, on every
 factor pair as a column (strided) axis and as a row axis, beside power-of-two line passes,
 as the strided z axis of 3D shapes (before the plane pass and before line passes), and as
@@ -27,11 +27,17 @@ PAIRS = [(6, 10), (10, 6), (30, 12), (12, 30), (20, 60), (60, 20), (24, 40), (40
          # odd factors with a squared prime: R2 = 9 (18 to 1152), 25 (200, 400), 27 (216, 432),
          # 45 (720)
          (18, 200), (200, 18), (36, 216), (216, 36), (72, 400), (400, 72), (144, 432), (432, 144), (288, 720),
-         (720, 288), (576, 16), (16, 576), (1152, 16), (16, 1152)]
+         (720, 288), (576, 16), (16, 576), (1152, 16), (16, 1152),
+         # primes 7, 11, 13 (14 to 1792), and 21, 33, 35, 39 times 8 and 16
+         (14, 22), (22, 14), (28, 26), (26, 28), (56, 44), (44, 56), (112, 52), (52, 112), (224, 88), (88, 224),
+         (448, 104), (104, 448), (896, 16), (16, 896), (1792, 8), (8, 1792), (176, 208), (208, 176), (352, 416),
+         (416, 352), (704, 32), (32, 704), (1408, 8), (8, 1408), (832, 16), (16, 832), (1664, 8), (8, 1664),
+         (168, 264), (264, 168), (336, 280), (280, 336), (528, 312), (312, 528), (560, 120), (120, 560),
+         (624, 64), (64, 624)]
 MIXED = [(64, 240), (240, 64), (30, 16, 16), (24, 64, 64), (16, 24, 40), (120, 8, 32), (96, 16, 16),
          (16, 32, 480),
          # smooth y axes: the forward and inverse line passes between the modulus and update
-         (16, 48, 40), (8, 240, 64), (12, 96, 30), (4, 640, 8), (8, 200, 36), (4, 36, 216)]
+         (16, 48, 40), (8, 240, 64), (12, 96, 30), (4, 640, 8), (8, 200, 36), (4, 36, 216), (8, 56, 44), (4, 224, 13)]
 
 
 @pytest.mark.parametrize("shape", PAIRS + MIXED)
@@ -58,8 +64,13 @@ def test_smooth_teacher_forced_iteration(qpch, shape, schedule):
             assert rel(xs[k], xo) < 1e-5, (shape, kind, k, rel(xs[k], xo))
 
 
+# Radix 7 and 11 run on 56 x 88.  56 x 44 from these starts is ill-conditioned: starts
+# perturbed by one float ulp move the oracle's own float run by up to 1.5e-5 at iteration 8
+# (the smooth kernels land 1.56e-5 from it there, the chirp-z kernels 2.8e-6), where 56 x 88
+# moves by 7e-7.
 @pytest.mark.parametrize("shape", [(240, 240), (80, 48), (30, 16, 16), (480, 96), (1280, 16), (16, 48, 40),
-                                   (200, 200), (720, 72), (576, 36), (432, 16)])
+                                   (200, 200), (720, 72), (576, 36), (432, 16), (56, 88), (448, 104), (1792, 8),
+                                   (336, 26)])
 def test_smooth_per_iteration_error(qpch, shape):
     """RAAR(30) from given starts: the device's error within max(1e-5, 3 x the spread of the
     oracle's float run against its FP64 run) of the oracle's float run at every iteration."""

@@ -23,6 +23,8 @@ namespace {
 
 bool persistent_pair(int r1, int r2) {
   if (r2 == 3 || r2 == 5 || r2 == 15) return r1 <= 16 || (r1 == 32 && r2 <= 5);
+  if (r2 == 7) return r1 <= 8;  // 14, 28, 56 (smooth_prime.hip)
+  if (r2 == 11 || r2 == 13) return r1 <= 4;  // 22, 44, 26, 52
   return r2 == 9 && r1 <= 8;  // 18, 36 and 72 (smooth_sq.hip)
 }
 
@@ -63,6 +65,7 @@ hipError_t twopass_smooth_launch(int N, const dev::PassArgs& p0, const dev::Pass
   int r1 = 0, r2 = 0;
   if (!factors(N, r1, r2) || !persistent_pair(r1, r2)) return hipErrorInvalidValue;
   if (sq_factor(r2)) return twopass_smooth_sq_launch(r1, r2, p0, p1, q, len, workers, s);
+  if (prime_factor(r2)) return twopass_smooth_prime_launch(r1, r2, p0, p1, q, len, workers, s);
 #define QPCH_SMOOTH_CASE(A, B) \
   if (r1 == A && r2 == B) return smooth_detail::launch_twopass<A, B>(p0, p1, q, len, workers, s);
   QPCH_FOR_EACH_SMOOTH_PERSISTENT(QPCH_SMOOTH_CASE)

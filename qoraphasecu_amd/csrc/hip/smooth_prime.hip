@@ -16,6 +16,19 @@ namespace qpch {
   X(16, 11) X(32, 11) X(64, 11) X(128, 11) X(2, 13) X(4, 13) X(8, 13) X(16, 13) X(32, 13) X(64, 13) \
   X(128, 13) X(8, 21) X(16, 21) X(8, 33) X(16, 33) X(8, 35) X(16, 35) X(8, 39) X(16, 39)
 
+// Square images with a persistent launch: 14, 28, 56, 22, 44, 26 and 52 (88 = 8 x 11 and
+// 104 = 8 x 13 held 256 VGPRs and AGPRs besides in the loop over both passes).
+#define QPCH_FOR_EACH_SMOOTH_PRIME_PERSISTENT(X) X(2, 7) X(4, 7) X(8, 7) X(2, 11) X(4, 11) X(2, 13) X(4, 13)
+
+hipError_t twopass_smooth_prime_launch(int r1, int r2, const dev::PassArgs& p0, const dev::PassArgs& p1,
+                                       const dev::SyncArgs& q, int len, int workers, hipStream_t s) {
+#define QPCH_SMOOTH_CASE(A, B) \
+  if (r1 == A && r2 == B) return smooth_detail::launch_twopass<A, B>(p0, p1, q, len, workers, s);
+  QPCH_FOR_EACH_SMOOTH_PRIME_PERSISTENT(QPCH_SMOOTH_CASE)
+#undef QPCH_SMOOTH_CASE
+  return hipErrorInvalidValue;
+}
+
 hipError_t smooth_prime_launch(int r1, int r2, bool col, dev::Mode m, const dev::PassArgs& a, long long lines,
                                dim3 grid, hipStream_t s) {
   if (m == dev::Mode::kGrad) return hipErrorInvalidValue;

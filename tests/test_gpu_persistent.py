@@ -59,6 +59,9 @@ CASES = [
     ((36, 36), dict(starts=8)),  # 4 * 9 (radix 9 = 3 x 3)
     ((72, 72), dict(starts=3, batch=1)),  # 8 * 9
     ((18, 18), dict(starts=8)),  # 2 * 9
+    ((56, 56), dict(starts=8)),  # 8 * 7 (the prime short form)
+    ((44, 44), dict(starts=3, batch=1)),  # 4 * 11
+    ((26, 26), dict(starts=8)),  # 2 * 13
     ((480, 480), dict(starts=4)),  # 32 * 15: no persistent kernel, the per-pass launches either way
     ((80, 240), dict(starts=4)),  # smooth, not square: the per-pass launches either way
 ]

@@ -22,6 +22,7 @@ namespace {
   X(16, 15) X(32, 3) X(32, 5)
 
 bool persistent_pair(int r1, int r2) {
+  if (!dev::smooth_is_pow2(r1) || r2 % 2 == 0) return false;
   if (r2 == 3 || r2 == 5 || r2 == 15) return r1 <= 16 || (r1 == 32 && r2 <= 5);
   if (r2 == 7) return r1 <= 8;  // 14, 28, 56 (smooth_prime.hip)
   if (r2 == 11 || r2 == 13) return r1 <= 4;  // 22, 44, 26, 52
@@ -35,8 +36,27 @@ bool persistent_pair(int r1, int r2) {
 // 20.1, 108^2 19.2 / 20.9, 360^2 53.3 / 55.2; R2 = 75 at every R1, 600^2 131 / 318): with R1
 // threads of R2 registers in layout B, a small R1 leaves most of a line's threads idle in
 // the radix-R2 stage.
+// Lengths of 2, 3 and 5 whose power-of-two part is small beside the odd part take a more
+// even split, with a factor that is not a power of two (smooth_mixed.hip).
+struct MixedPair {
+  int n, r1, r2;
+};
+constexpr MixedPair kMixed[] = {{45, 9, 5},      {50, 10, 5},     {54, 6, 9},      {75, 15, 5},     {90, 6, 15},
+                                {100, 10, 10},   {108, 12, 9},    {125, 25, 5},    {135, 15, 9},    {150, 10, 15},
+                                {180, 12, 15},   {225, 15, 15},   {250, 10, 25},   {270, 18, 15},   {300, 20, 15},
+                                {360, 24, 15},   {375, 25, 15},   {450, 30, 15},   {500, 20, 25},   {540, 36, 15},
+                                {600, 24, 25},   {675, 27, 25},   {750, 30, 25},   {900, 36, 25},   {1000, 40, 25},
+                                {1125, 45, 25},  {1200, 48, 25}};
+
 bool factors(int N, int& r1, int& r2) {
   if (N < 2 || N > dev::kGenericMaxN) return false;
+  for (const MixedPair& m : kMixed) {
+    if (m.n == N) {
+      r1 = m.r1;
+      r2 = m.r2;
+      return true;
+    }
+  }
   int q = 1;
   while ((N / q) % 2 == 0) q *= 2;
   const int r = N / q;
@@ -87,6 +107,7 @@ hipError_t smooth_launch(int N, bool col, dev::Mode m, const dev::PassArgs& a, l
                          hipStream_t s) {
   int r1 = 0, r2 = 0;
   if (!factors(N, r1, r2) || m == dev::Mode::kGrad) return hipErrorInvalidValue;
+  if (!dev::smooth_is_pow2(r1) || r2 % 2 == 0) return smooth_mixed_launch(r1, r2, col, m, a, lines, grid, s);
   if (sq_factor(r2)) return smooth_sq_launch(r1, r2, col, m, a, lines, grid, s);
   if (prime_factor(r2)) return smooth_prime_launch(r1, r2, col, m, a, lines, grid, s);
 #define QPCH_SMOOTH_CASE(A, B) \

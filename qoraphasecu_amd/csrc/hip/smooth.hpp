@@ -62,27 +62,39 @@ __host__ __device__ constexpr bool smooth_odd_factor(int r2) {
   return r2 == 3 || r2 == 5 || r2 == 15 || r2 == 9 || r2 == 25 || r2 == 27 || r2 == 45 || r2 == 7 || r2 == 11 ||
          r2 == 13 || r2 == 21 || r2 == 33 || r2 == 35 || r2 == 39;
 }
+__host__ __device__ constexpr bool smooth_is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+__host__ __device__ constexpr int smooth_odd_part(int v) { return v % 2 == 0 ? smooth_odd_part(v / 2) : v; }
+// A radix one thread transforms in registers (dft_any): a power of two up to 16, an odd
+// factor above, or their product (the prime-factor form).
+__host__ __device__ constexpr bool smooth_reg_radix(int r) {
+  return r >= 2 && r / smooth_odd_part(r) <= 16 && (smooth_odd_part(r) == 1 || smooth_odd_factor(smooth_odd_part(r)));
+}
+// Whether R1 runs on the line engine: powers of two from 32 on.
+__host__ __device__ constexpr bool smooth_engine(int r1) { return r1 > 16 && smooth_is_pow2(r1); }
 // Threads per line (layout A's busy threads, or R1, rounded up to a power of two), and
-// lines per work-group: generic_pass's count for R2 in {3, 5, 15} (whole waves for every
-// such pair); for the larger R2, 256 threads (generic_pass's count does not always make
-// whole waves, and with up to 768 threads the radix-45 and -75 stages spilled registers).
+// lines per work-group: generic_pass's count for a power of two R1 and R2 in {3, 5, 15}
+// (whole waves for every such pair); otherwise 256 threads (generic_pass's count does not
+// always make whole waves, and with up to 768 threads the radix-45 and -75 stages spilled
+// registers).
 __host__ __device__ constexpr int smooth_threads_per_line(int r1, int r2) {
-  const int ta = r2 * (r1 > 16 ? r1 / smooth_engine_radix(r2) : 1);
+  const int ta = r2 * (smooth_engine(r1) ? r1 / smooth_engine_radix(r2) : 1);
   return smooth_pow2_ceil(ta > r1 ? ta : r1);
 }
 __host__ __device__ constexpr int smooth_lines_per_block(int r1, int r2) {
   const int t = smooth_threads_per_line(r1, r2);
-  return r2 == 3 || r2 == 5 || r2 == 15 ? generic_lines_per_block(r1 * r2) : (t >= 256 ? 1 : 256 / t);
+  return smooth_is_pow2(r1) && (r2 == 3 || r2 == 5 || r2 == 15) ? generic_lines_per_block(r1 * r2)
+                                                                 : (t >= 256 ? 1 : 256 / t);
 }
 
 template <int R1, int R2, bool COL>
 struct SmoothGeom {
-  static_assert(R1 >= 2 && R1 <= 512 && (R1 & (R1 - 1)) == 0, "R1: a power of two from 2 to 512");
-  static_assert(smooth_odd_factor(R2), "R2: 3, 5, 7, 9, 11, 13, 15, 21, 25, 27, 33, 35, 39 or 45");
+  static_assert((R1 >= 2 && R1 <= 512 && smooth_is_pow2(R1)) || (smooth_reg_radix(R1) && R1 <= 48),
+                "R1: a power of two from 2 to 512, or a register radix up to 48");
+  static_assert(smooth_reg_radix(R2) && R2 <= 45, "R2: a register radix up to 45");
   static constexpr int kN = R1 * R2;
   static_assert(kN <= kGenericMaxN, "the generic kernels' length limit");
-  static constexpr bool kEngine = R1 > 16;                     // the line engine for R1
-  static_assert(!kEngine || R2 <= 15, "the line engine's exchange holds R2 <= RA + 1 subsequences");
+  static constexpr bool kEngine = smooth_engine(R1);           // the line engine for R1
+  static_assert(!kEngine || (R2 <= 15 && R2 % 2 == 1), "the line engine's exchange holds R2 <= RA + 1 subsequences");
   static constexpr int kRA = smooth_engine_radix(R2);          // its radix
   static constexpr int kSub = kEngine ? R1 / kRA : 1;          // threads per subsequence (layout A)
   static constexpr int kA = kEngine ? kRA : R1;                // registers in layout A
@@ -198,6 +210,8 @@ __device__ __forceinline__ void unroll(F&& f) {
 
 template <int R, bool INV>
 __device__ __forceinline__ void dft_odd(float2* v);
+template <int R, bool INV>
+__device__ __forceinline__ void dft_any(float2* v);
 
 // Cooley-Tukey R = P Q in registers: with n = n1 + P n2 and k = Q k1 + k2,
 //   Y[n1][k2] = DFT_Q over n2 of x[n1 + P n2],  Y[n1][k2] *= W_R^(n1 k2),
@@ -253,12 +267,12 @@ __device__ __forceinline__ void dft_pfa(float2* v) {
     float2 t[N1];
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1) t[n1] = u[n1][n2];
-    dft_odd<N1, INV>(t);
+    dft_any<N1, INV>(t);
 #pragma unroll
     for (int k1 = 0; k1 < N1; ++k1) u[k1][n2] = t[k1];
   }
 #pragma unroll
-  for (int k1 = 0; k1 < N1; ++k1) dft_odd<N2, INV>(u[k1]);
+  for (int k1 = 0; k1 < N1; ++k1) dft_any<N2, INV>(u[k1]);
 #pragma unroll
   for (int k1 = 0; k1 < N1; ++k1) {
 #pragma unroll
@@ -324,6 +338,20 @@ __device__ __forceinline__ void dft_odd(float2* v) {
   }
 }
 
+// In-register DFT of any register radix R = 2^a o (smooth_reg_radix): dft_regs for a power
+// of two, dft_odd for an odd R, and the prime-factor form of 2^a x o otherwise.
+template <int R, bool INV>
+__device__ __forceinline__ void dft_any(float2* v) {
+  constexpr int O = smooth_odd_part(R);
+  if constexpr (O == 1) {
+    dft_regs<R, !INV>(v);
+  } else if constexpr (O == R) {
+    dft_odd<R, INV>(v);
+  } else {
+    dft_pfa<R / O, O, INV>(v);
+  }
+}
+
 // Layout A -> layout B (forward).  t: the thread's index within its line; line: the line's
 // scratch; tw: W_N^j, j < N (forward table); twP: W_R1^j, j < R1 (the engine's table, R1 >=
 // 32).  Every thread of the work-group calls it.
@@ -333,7 +361,7 @@ __device__ __forceinline__ void smooth_fwd(float2* v, int t, float2* line, const
   if constexpr (!G::kEngine) {
     const int i = t;
     if (i < R2) {
-      dft_regs<R1, true>(v);
+      dft_any<R1, false>(v);
 #pragma unroll
       for (int q = 0; q < R1; ++q) line[smooth_pos<R1>(i * R1 + q)] = v[q];
     }
@@ -350,7 +378,7 @@ __device__ __forceinline__ void smooth_fwd(float2* v, int t, float2* line, const
         const float2 u = cmul(v[q], w[q]);
         if (i != 0) v[q] = u;  // W^0 = 1: the input stays as it is
       }
-      dft_odd<R2, false>(v);
+      dft_any<R2, false>(v);
     }
     __syncthreads();
   } else {
@@ -387,7 +415,7 @@ __device__ __forceinline__ void smooth_inv(float2* v, int t, float2* line, const
   if constexpr (!G::kEngine) {
     const int i = t;
     if (i < R1) {
-      dft_odd<R2, true>(v);
+      dft_any<R2, true>(v);
 #pragma unroll
       for (int q = 0; q < R2; ++q) line[smooth_pos<R1>(i * R2 + q)] = v[q];
     }
@@ -404,7 +432,7 @@ __device__ __forceinline__ void smooth_inv(float2* v, int t, float2* line, const
         const float2 u = cmulc(v[q], w[q]);
         if (i != 0) v[q] = u;
       }
-      dft_regs<R1, false>(v);
+      dft_any<R1, true>(v);
     }
     __syncthreads();
   } else {

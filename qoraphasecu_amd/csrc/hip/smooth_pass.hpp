@@ -31,6 +31,11 @@ hipError_t smooth_launch(int N, bool col, dev::Mode m, const dev::PassArgs& a, l
 hipError_t smooth_sq_launch(int r1, int r2, bool col, dev::Mode m, const dev::PassArgs& a, long long lines, dim3 grid,
                             hipStream_t s);
 
+// smooth_launch for the pairs whose R1 is not a power of two or whose R2 is even
+// (smooth_mixed.hip).
+hipError_t smooth_mixed_launch(int r1, int r2, bool col, dev::Mode m, const dev::PassArgs& a, long long lines,
+                               dim3 grid, hipStream_t s);
+
 // smooth_launch for R2 in {7, 11, 13, 21, 33, 35, 39} (smooth_prime.hip).
 hipError_t smooth_prime_launch(int r1, int r2, bool col, dev::Mode m, const dev::PassArgs& a, long long lines,
                                dim3 grid, hipStream_t s);

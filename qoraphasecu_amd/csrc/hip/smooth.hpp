@@ -41,7 +41,8 @@ namespace qpch::dev {
 
 __host__ __device__ constexpr int smooth_pow2_ceil(int v) { return v <= 1 ? 1 : 2 * smooth_pow2_ceil((v + 1) / 2); }
 
-// R1 <= 16: the power-of-two stage is one in-register DFT (layouts A and B above).  R1 >= 32
+// R1 <= 16, or R1 not a power of two (up to 48): the R1 stage is one in-register DFT
+// (dft_any; layouts A and B above).  R1 a power of two >= 32
 // (the line engine): layout A is R2 interleaved subsequences x[i + p R2], p < R1, each
 // transformed by the line engine (fft_line, fft_engine.hpp) at radix RA on R1/RA threads of
 // RA registers: thread (i, i') holds x[i + (i' + m R1/RA) R2], m < RA.  The exchange to

@@ -354,9 +354,10 @@ def cpu_full_run(w, threads):
 def time_next_rows(steps):
     """The rows after the hot path: a non-power-of-two problem (240^2 = 16 x 15, 8 starts,
     RAAR(50)) through the smooth-length kernels, with the oracle's rate on the same
-    problem beside it; a prime row length (64 x 1021, 8 starts, RAAR(50): Bluestein's
-    transform in the generic passes); and the device make_problem (autocorrelation support)
-    and aligned_error at 256^2."""
+    problem beside it; a prime row length (64 x 1021, 8 starts, RAAR(50): the chirp-z
+    passes); 448^2 and 360^2 (8 starts, RAAR(50): the smooth kernels' prime short form and
+    even split); and the device make_problem (autocorrelation support) and aligned_error at
+    256^2."""
     import qoraphasecu_amd as qpc
     from qoraphasecu_amd import _core, workloads
 
@@ -386,6 +387,19 @@ def time_next_rows(steps):
     res["prime_64x1021"] = dict(ms_per_run=wall * 1e3, start_iterations_per_s=8 * 50 / wall)
     log(f"  64 x 1021 (Bluestein rows), 8 starts, RAAR(50): {res['prime_64x1021']['start_iterations_per_s']:.4g} "
         f"start-it/s")
+    # smooth lengths with a prime factor of 7 (448 = 64 x 7) and with an even split whose
+    # first factor is not a power of two (360 = 24 x 15), 8 starts, RAAR(50), all starts in
+    # one launch
+    for n, key, label in ((448, "smooth_448", "448^2 (64 x 7)"), (360, "smooth_360", "360^2 (24 x 15)")):
+        objn, intenn, maskn = workloads.problem_2d(n, seed=1002, photons=1e7, beamstop_radius=6, box=n * 3 // 8)
+        s = _core.Solver(intenn, None, maskn, 8, 0, 0, None, 0.1, "hip", "batched", 10)
+        s.run(ops)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.run(ops)
+        wall = (time.perf_counter() - t0) / steps
+        res[key] = dict(ms_per_run=wall * 1e3, start_iterations_per_s=8 * 50 / wall)
+        log(f"  {label}, 8 starts, RAAR(50): {res[key]['start_iterations_per_s']:.4g} start-it/s")
     obj2, inten2, mask2 = workloads.problem_2d(256, seed=1000, photons=1e7, beamstop_radius=6)
     _core.make_problem(inten2, None, mask2, 0.1)
     t0 = time.perf_counter()

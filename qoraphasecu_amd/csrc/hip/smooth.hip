@@ -30,12 +30,13 @@ bool persistent_pair(int r1, int r2) {
 }
 
 // N = R1 R2 with R2 its odd part: a pair with a kernel.  R2 in {3, 5, 15}: R1 from 2 to
-// 512; 7: R1 from 2 to 256; 9, 11 and 13: R1 from 2 to 128; 21, 25, 27, 33, 35 and 39: R1
-// 8 and 16; 45: R1 16.  The pairs left out ran
+// 512; 7: R1 from 2 to 256; 9, 11 and 13: R1 from 2 to 128; 17 and 19: R1 from 2 to 16;
+// 21, 23, 25, 27, 33, 35, 39 and 49: R1 8 and 16; 45 and 63: R1 16.  The pairs left out ran
 // slower than generic_pass (us per iteration of 8 starts, generic / smooth: 100^2 16.1 /
-// 20.1, 108^2 19.2 / 20.9, 360^2 53.3 / 55.2; R2 = 75 at every R1, 600^2 131 / 318): with R1
-// threads of R2 registers in layout B, a small R1 leaves most of a line's threads idle in
-// the radix-R2 stage.
+// 20.1, 108^2 19.2 / 20.9, 360^2 53.3 / 55.2; R2 = 75 at every R1, 600^2 131 / 318) or than
+// chirp_pass (46^2 20.4 / 22.2, 92^2 20.4 / 21.6, 504^2 83.5 / 108): with R1 threads of R2
+// registers in layout B, a small R1 leaves most of a line's threads idle in the radix-R2
+// stage.
 // Lengths of 2, 3 and 5 whose power-of-two part is small beside the odd part take a more
 // even split, with a factor that is not a power of two (smooth_mixed.hip).
 struct MixedPair {
@@ -62,8 +63,10 @@ bool factors(int N, int& r1, int& r2) {
   const int r = N / q;
   if (q < 2) return false;
   const bool ok = r == 3 || r == 5 || r == 15 || r == 7 || ((r == 9 || r == 11 || r == 13) && q <= 128) ||
-                  ((r == 25 || r == 27 || r == 21 || r == 33 || r == 35 || r == 39) && (q == 8 || q == 16)) ||
-                  (r == 45 && q == 16);
+                  ((r == 17 || r == 19) && q <= 16) ||
+                  ((r == 25 || r == 27 || r == 21 || r == 23 || r == 33 || r == 35 || r == 39 || r == 49) &&
+                   (q == 8 || q == 16)) ||
+                  ((r == 45 || r == 63) && q == 16);
   if (!ok) return false;
   r1 = q;
   r2 = r;
@@ -71,7 +74,10 @@ bool factors(int N, int& r1, int& r2) {
 }
 
 bool sq_factor(int r2) { return r2 == 9 || r2 == 25 || r2 == 27 || r2 == 45; }
-bool prime_factor(int r2) { return r2 == 7 || r2 == 11 || r2 == 13 || r2 == 21 || r2 == 33 || r2 == 35 || r2 == 39; }
+bool prime_factor(int r2) {
+  return r2 == 7 || r2 == 11 || r2 == 13 || r2 == 17 || r2 == 19 || r2 == 23 || r2 == 21 || r2 == 33 || r2 == 35 ||
+         r2 == 39 || r2 == 49 || r2 == 63;
+}
 
 }  // namespace
 

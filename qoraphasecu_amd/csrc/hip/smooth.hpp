@@ -56,12 +56,13 @@ __host__ __device__ constexpr int smooth_engine_radix(int r2) {
   return r2 >= 9 ? 16 : r2 >= 5 ? 8 : 4;
 }
 // The odd factors R2: 3, 5 and 15; 9, 25, 27 and 45 (lengths of 2, 3 and 5 with a squared
-// odd prime); the primes 7, 11 and 13, and 21, 33, 35 and 39.  Only R2 <= 15 runs on the
+// odd prime); the primes 7 to 23, and 21, 33, 35, 39, 49 and 63.  Only R2 <= 15 runs on the
 // line engine (R1 >= 32), whose exchange needs RA + 1 >= R2 (7: RA 8, 7 of 8 threads busy;
 // 9, 11, 13: RA 16).
 __host__ __device__ constexpr bool smooth_odd_factor(int r2) {
   return r2 == 3 || r2 == 5 || r2 == 15 || r2 == 9 || r2 == 25 || r2 == 27 || r2 == 45 || r2 == 7 || r2 == 11 ||
-         r2 == 13 || r2 == 21 || r2 == 33 || r2 == 35 || r2 == 39;
+         r2 == 13 || r2 == 21 || r2 == 33 || r2 == 35 || r2 == 39 || r2 == 17 || r2 == 19 || r2 == 23 || r2 == 49 ||
+         r2 == 63;
 }
 __host__ __device__ constexpr bool smooth_is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 __host__ __device__ constexpr int smooth_odd_part(int v) { return v % 2 == 0 ? smooth_odd_part(v / 2) : v; }
@@ -91,7 +92,7 @@ template <int R1, int R2, bool COL>
 struct SmoothGeom {
   static_assert((R1 >= 2 && R1 <= 512 && smooth_is_pow2(R1)) || (smooth_reg_radix(R1) && R1 <= 48),
                 "R1: a power of two from 2 to 512, or a register radix up to 48");
-  static_assert(smooth_reg_radix(R2) && R2 <= 45, "R2: a register radix up to 45");
+  static_assert(smooth_reg_radix(R2) && R2 <= 63, "R2: a register radix up to 63");
   static constexpr int kN = R1 * R2;
   static_assert(kN <= kGenericMaxN, "the generic kernels' length limit");
   static constexpr bool kEngine = smooth_engine(R1);           // the line engine for R1
@@ -315,17 +316,21 @@ __device__ __forceinline__ void dft_prime(float2* v) {
   v[0] = x0;
 }
 
-// In-register DFT of the odd radices of the smooth passes: 3 and 5 (short forms), 7, 11
-// and 13 (dft_prime), 15 (prime factor 3 x 5), 9 = 3 x 3, 25 = 5 x 5 and 27 = 3 x 9
-// (Cooley-Tukey), 21, 33, 35, 39 and 45 = 9 x 5 (prime factor).
+// In-register DFT of the odd radices of the smooth passes: 3 and 5 (short forms), 7 to 23
+// (dft_prime), 15 (prime factor 3 x 5), 9 = 3 x 3, 25 = 5 x 5, 27 = 3 x 9 and 49 = 7 x 7
+// (Cooley-Tukey), 21, 33, 35, 39, 45 = 9 x 5 and 63 = 9 x 7 (prime factor).
 template <int R, bool INV>
 __device__ __forceinline__ void dft_odd(float2* v) {
   if constexpr (R == 15) {
     dft15<INV>(v);
   } else if constexpr (R == 45) {
     dft_pfa<9, 5, INV>(v);
-  } else if constexpr (R == 7 || R == 11 || R == 13) {
+  } else if constexpr (R == 7 || R == 11 || R == 13 || R == 17 || R == 19 || R == 23) {
     dft_prime<R, INV>(v);
+  } else if constexpr (R == 49) {
+    dft_ct<7, 7, INV>(v);
+  } else if constexpr (R == 63) {
+    dft_pfa<9, 7, INV>(v);
   } else if constexpr (R == 21 || R == 33 || R == 39) {
     dft_pfa<3, R / 3, INV>(v);
   } else if constexpr (R == 35) {

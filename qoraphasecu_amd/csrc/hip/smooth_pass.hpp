@@ -11,8 +11,9 @@ namespace qpch {
 
 // Whether an axis of length N runs its fused passes (all but the gradient) in smooth_pass:
 // N = R1 R2 <= 2048 with R1 a power of two and R2 its odd part: R2 in {3, 5, 15} with R1
-// from 2 to 512, 7 with R1 from 2 to 256, 9, 11 and 13 with R1 from 2 to 128, 21, 25, 27,
-// 33, 35 and 39 with R1 8 or 16, 45 with R1 16.
+// from 2 to 512, 7 with R1 from 2 to 256, 9, 11 and 13 with R1 from 2 to 128, 17 and 19
+// with R1 from 2 to 16, 21, 23, 25, 27, 33, 35, 39 and 49 with R1 8 or 16, 45 and 63 with
+// R1 16, and the mixed pairs (smooth_mixed.hip).
 bool smooth_axis(int N);
 
 // Lines per work-group of smooth_pass on an axis of length N (smooth_axis(N)): generic_pass's
@@ -36,7 +37,7 @@ hipError_t smooth_sq_launch(int r1, int r2, bool col, dev::Mode m, const dev::Pa
 hipError_t smooth_mixed_launch(int r1, int r2, bool col, dev::Mode m, const dev::PassArgs& a, long long lines,
                                dim3 grid, hipStream_t s);
 
-// smooth_launch for R2 in {7, 11, 13, 21, 33, 35, 39} (smooth_prime.hip).
+// smooth_launch for R2 in {7, 11, 13, 17, 19, 21, 23, 33, 35, 39, 49, 63} (smooth_prime.hip).
 hipError_t smooth_prime_launch(int r1, int r2, bool col, dev::Mode m, const dev::PassArgs& a, long long lines,
                                dim3 grid, hipStream_t s);
 

@@ -214,11 +214,11 @@ def test_ensemble_of_other_sizes_equals_separate_runs(qpc):
 
 
 def test_ensemble_of_prime_factor_sizes_equals_separate_runs(qpc):
-    """Shapes with a prime factor of 7 or more (136 x 92 twice, 131 x 64, 30 x 67: the
+    """Shapes with a prime factor of 7 or more (248 x 116 twice, 131 x 64, 30 x 67: the
     chirp-z passes, Bluestein in registers; 56 x 44: the radix-7 and radix-11 smooth
     passes) beside a smooth one (48 x 40): stacked per shape, each problem's amplitude row
     and error slot are its own, and the runs equal each problem alone (fp32)."""
-    cases = [synthetic.triangle((136, 92), 12), synthetic.triangle((136, 92), 10), synthetic.triangle((131, 64), 16),
+    cases = [synthetic.triangle((248, 116), 12), synthetic.triangle((248, 116), 10), synthetic.triangle((131, 64), 16),
              synthetic.triangle((30, 67), 8), synthetic.triangle((48, 40), 12), synthetic.triangle((56, 44), 12)]
     recipe = qpc.Recipe(qpc.HIO(10), qpc.Shrinkwrap(), qpc.RAAR(10), qpc.detwin(qpc.HIO(5)), qpc.ER(10))
     many = qpc.reconstruct_many([c[2] for c in cases], supports=[c[1].astype(bool) for c in cases], recipe=recipe,

@@ -1,6 +1,6 @@
 """The smooth-length passes (smooth.hpp: axes N = R1 R2 <= 2048 with R1 a power of two and R2
-in {3, 5, 15} (R1 to 512), 7 (R1 to 256), 9, 11 or 13 (R1 to 128), 21, 25, 27, 33, 35 or
-39 (R1 8 or 16) or 45 (R1 16), and pairs with R1 not a power of two for other lengths of 2,
+in {3, 5, 15} (R1 to 512), 7 (R1 to 256), 9, 11 or 13 (R1 to 128), 17 or 19 (R1 to 16),
+21, 23, 25, 27, 33, 35, 39 or 49 (R1 8 or 16) or 45 or 63 (R1 16), and pairs with R1 not a power of two for other lengths of 2,
 3 and 5 (100 = 10 x 10, 360 = 24 x 15, ...), the line held in registers) against the oracle: one teacher-forced HIO, RAAR
 and ER iteration from the same starts, error and object within 1e-5 relative, on every
 factor pair as a column (strided) axis and as a row axis, beside power-of-two line passes,
@@ -35,6 +35,9 @@ PAIRS = [(6, 10), (10, 6), (30, 12), (12, 30), (20, 60), (60, 20), (24, 40), (40
          (416, 352), (704, 32), (32, 704), (1408, 8), (8, 1408), (832, 16), (16, 832), (1664, 8), (8, 1664),
          (168, 264), (264, 168), (336, 280), (280, 336), (528, 312), (312, 528), (560, 120), (120, 560),
          (624, 64), (64, 624),
+         # 17, 19 (R1 2 to 16), 23, 49 (R1 8, 16) and 63 (R1 16)
+         (34, 38), (38, 34), (68, 184), (184, 68), (136, 76), (76, 136), (272, 392), (392, 272), (152, 368),
+         (368, 152), (304, 16), (16, 304), (784, 8), (8, 784), (1008, 8), (8, 1008),
          # R1 not a power of two (or R2 even): 45 to 1200
          (45, 50), (50, 45), (54, 75), (75, 54), (90, 100), (100, 90), (108, 125), (125, 108), (135, 150),
          (150, 135), (180, 225), (225, 180), (250, 270), (270, 250), (300, 360), (360, 300), (375, 16), (16, 375),
@@ -76,7 +79,7 @@ def test_smooth_teacher_forced_iteration(qpch, shape, schedule):
 # moves by 7e-7.
 @pytest.mark.parametrize("shape", [(240, 240), (80, 48), (30, 16, 16), (480, 96), (1280, 16), (16, 48, 40),
                                    (200, 200), (720, 72), (576, 36), (432, 16), (56, 88), (448, 104), (1792, 8),
-                                   (336, 26), (100, 100), (360, 180), (1000, 8)])
+                                   (336, 26), (100, 100), (360, 180), (1000, 8), (136, 92), (1008, 8)])
 def test_smooth_per_iteration_error(qpch, shape):
     """RAAR(30) from given starts: the device's error within max(1e-5, 3 x the spread of the
     oracle's float run against its FP64 run) of the oracle's float run at every iteration."""

@@ -1,5 +1,6 @@
 // Fused passes along axes whose length N (32 to 2048) has a prime factor of 7 or more and is
-// not a smooth length (smooth.hpp takes the odd parts 7, 11, 13, 21, 33, 35 and 39), by
+// not a smooth length (smooth.hpp takes odd parts with primes up to 23 where it measured
+// faster), by
 // Bluestein's chirp-z transform with the power-of-two transforms of length M = 2^k >=
 // max(256, 2N - 1) held in registers by the radix-16 line engine (fft_line,
 // fft_engine.hpp).  generic_pass runs such factors as O(r^2) stages up to 64, and above as

@@ -22,7 +22,9 @@ namespace {
   X(16, 15) X(32, 3) X(32, 5)
 
 bool persistent_pair(int r1, int r2) {
-  if (!dev::smooth_is_pow2(r1) || r2 % 2 == 0) return false;
+  if (!dev::smooth_is_pow2(r1) || r2 % 2 == 0) {  // 45, 50, 54, 100 (smooth_mixed.hip)
+    return (r1 == 9 && r2 == 5) || (r1 == 10 && r2 == 5) || (r1 == 6 && r2 == 9) || (r1 == 10 && r2 == 10);
+  }
   if (r2 == 3 || r2 == 5 || r2 == 15) return r1 <= 16 || (r1 == 32 && r2 <= 5);
   if (r2 == 7) return r1 <= 8;  // 14, 28, 56 (smooth_prime.hip)
   if (r2 == 11 || r2 == 13) return r1 <= 4;  // 22, 44, 26, 52
@@ -90,6 +92,7 @@ hipError_t twopass_smooth_launch(int N, const dev::PassArgs& p0, const dev::Pass
                                  int len, int workers, hipStream_t s) {
   int r1 = 0, r2 = 0;
   if (!factors(N, r1, r2) || !persistent_pair(r1, r2)) return hipErrorInvalidValue;
+  if (!dev::smooth_is_pow2(r1) || r2 % 2 == 0) return twopass_smooth_mixed_launch(r1, r2, p0, p1, q, len, workers, s);
   if (sq_factor(r2)) return twopass_smooth_sq_launch(r1, r2, p0, p1, q, len, workers, s);
   if (prime_factor(r2)) return twopass_smooth_prime_launch(r1, r2, p0, p1, q, len, workers, s);
 #define QPCH_SMOOTH_CASE(A, B) \

@@ -14,6 +14,19 @@ namespace qpch {
   X(15, 15) X(10, 25) X(18, 15) X(20, 15) X(24, 15) X(25, 15) X(30, 15) X(20, 25) X(36, 15) X(24, 25) \
   X(27, 25) X(30, 25) X(36, 25) X(40, 25) X(45, 25) X(48, 25)
 
+// Square images with a persistent launch: 45, 50, 54 and 100 (from 75 = 15 x 5 on, the
+// others held 256 VGPRs and AGPRs besides in the loop over both passes).
+#define QPCH_FOR_EACH_SMOOTH_MIXED_PERSISTENT(X) X(9, 5) X(10, 5) X(6, 9) X(10, 10)
+
+hipError_t twopass_smooth_mixed_launch(int r1, int r2, const dev::PassArgs& p0, const dev::PassArgs& p1,
+                                       const dev::SyncArgs& q, int len, int workers, hipStream_t s) {
+#define QPCH_SMOOTH_CASE(A, B) \
+  if (r1 == A && r2 == B) return smooth_detail::launch_twopass<A, B>(p0, p1, q, len, workers, s);
+  QPCH_FOR_EACH_SMOOTH_MIXED_PERSISTENT(QPCH_SMOOTH_CASE)
+#undef QPCH_SMOOTH_CASE
+  return hipErrorInvalidValue;
+}
+
 hipError_t smooth_mixed_launch(int r1, int r2, bool col, dev::Mode m, const dev::PassArgs& a, long long lines,
                                dim3 grid, hipStream_t s) {
   if (m == dev::Mode::kGrad) return hipErrorInvalidValue;

@@ -19,8 +19,9 @@ namespace qpch {
   X(128, 13) X(8, 21) X(16, 21) X(8, 33) X(16, 33) X(8, 35) X(16, 35) X(8, 39) X(16, 39) X(2, 17) X(4, 17) \
   X(8, 17) X(16, 17) X(2, 19) X(4, 19) X(8, 19) X(16, 19) X(8, 23) X(16, 23) X(8, 49) X(16, 49) X(16, 63)
 
-// Square images with a persistent launch: 14, 28, 56, 22, 44, 26 and 52 (88 = 8 x 11 and
-// 104 = 8 x 13 held 256 VGPRs and AGPRs besides in the loop over both passes).
+// Square images with a persistent launch: 14, 28, 56, 22, 44, 26 and 52 (88 = 8 x 11,
+// 104 = 8 x 13 and the radix-17 and -19 squares held 256 VGPRs and AGPRs besides in the
+// loop over both passes).
 #define QPCH_FOR_EACH_SMOOTH_PRIME_PERSISTENT(X) X(2, 7) X(4, 7) X(8, 7) X(2, 11) X(4, 11) X(2, 13) X(4, 13)
 
 hipError_t twopass_smooth_prime_launch(int r1, int r2, const dev::PassArgs& p0, const dev::PassArgs& p1,

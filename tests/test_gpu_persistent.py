@@ -42,12 +42,14 @@ CASES = [
     ((256, 256), dict(starts=16, batch=4, lanes=4)),  # four persistent launches at once
     # twopass_generic: both axes of primes 2, 3 and 5 only, neither a power of two nor a
     # smooth length (fp32)
-    ((125, 125), dict(starts=8)),  # 5 * 5 * 5
-    ((45, 81), dict(starts=3, batch=1)),
-    ((250, 162), dict(starts=4)),  # 2 * 125, 2 * 81
-    ((225, 27), dict(starts=2)),  # 27 lines of 225: ragged tiles
+    ((81, 81), dict(starts=8)),  # 3^4
+    ((27, 162), dict(starts=3, batch=1)),
+    ((243, 324), dict(starts=4)),  # 3^5, 4 * 81
+    ((405, 27), dict(starts=2)),  # 27 lines of 405: ragged tiles
     ((252, 252), dict(starts=8)),  # a factor 7: the chirp-z passes, per pass either way
-    ((100, 100), dict(starts=8)),  # 4 * 5 * 5
+    ((125, 125), dict(starts=8)),  # 25 * 5: smooth passes, per pass either way
+    ((100, 100), dict(starts=8)),  # 10 * 10: twopass_smooth
+    ((54, 54), dict(starts=3, batch=1)),  # 6 * 9: twopass_smooth
     ((216, 216), dict(starts=8)),  # 8 * 27: smooth passes, per pass either way
     # twopass_smooth: square images of a smooth length (smooth.hpp)
     ((240, 240), dict(starts=8)),  # 16 * 15

@@ -673,14 +673,20 @@ void device_fft(float2* data, const Dims& d, int batch, bool forward, float scal
     const float sc = last_axis ? scale : 1.0f;
     scaled = scaled || last_axis;
     const long long inner = d.inner(a), outer = d.outer(a);
-    if (is_pow2(N) && N <= kMaxFused) {
+    if ((is_pow2(N) && N <= kMaxFused) || smooth_axis(N)) {
+      // power-of-two lines in line_pass; smooth lengths in smooth_pass (tables made in the
+      // kernel: nothing is allocated, so the no-allocation entry points may take this path)
       PassArgs p{};
       p.outer = static_cast<int>(outer);
       p.inner = static_cast<int>(inner);
       p.n = d.n;
       p.w = data;
       p.out_scale = sc;
-      launch_pass<float2>(N, inner > 1, forward ? Mode::kFwd : Mode::kInv, p, batch, s);
+      if (is_pow2(N)) {
+        launch_pass<float2>(N, inner > 1, forward ? Mode::kFwd : Mode::kInv, p, batch, s);
+      } else {
+        launch_generic_pass(N, forward ? Mode::kFwd : Mode::kInv, p, batch, s);
+      }
     } else {
       launch_generic(data, d.n, batch, N, outer, inner, forward, sc, s);
     }

@@ -49,7 +49,18 @@ constexpr MixedPair kMixed[] = {{45, 9, 5},      {50, 10, 5},     {54, 6, 9},   
                                 {180, 12, 15},   {225, 15, 15},   {250, 10, 25},   {270, 18, 15},   {300, 20, 15},
                                 {360, 24, 15},   {375, 25, 15},   {450, 30, 15},   {500, 20, 25},   {540, 36, 15},
                                 {600, 24, 25},   {675, 27, 25},   {750, 30, 25},   {900, 36, 25},   {1000, 40, 25},
-                                {1125, 45, 25},  {1200, 48, 25}};
+                                {1125, 45, 25},  {1200, 48, 25},
+                                // lengths with a prime from 7 to 13 (otherwise chirp_pass)
+                                {42, 6, 7},      {98, 14, 7},     {132, 12, 11},   {196, 28, 7},    {210, 14, 15},
+                                {252, 28, 9},    {315, 35, 9},    {390, 30, 13},   {420, 28, 15},   {441, 21, 21},
+                                {462, 42, 11},   {700, 28, 25},
+                                {84, 12, 7}, {70, 10, 7}, {66, 11, 6}, {78, 13, 6}, {110, 11, 10},
+                                {126, 14, 9}, {130, 13, 10}, {140, 14, 10}, {154, 14, 11}, {156, 13, 12},
+                                {182, 14, 13}, {198, 18, 11}, {220, 20, 11}, {234, 18, 13}, {260, 20, 13},
+                                {308, 22, 14}, {364, 26, 14}, {440, 22, 20}, {468, 26, 18}, {504, 24, 21},
+                                {520, 26, 20}, {588, 28, 21}, {616, 28, 22}, {660, 30, 22}, {672, 28, 24},
+                                {728, 28, 26}, {780, 30, 26}, {840, 30, 28}, {880, 40, 22}, {924, 33, 28},
+                                {980, 35, 28}};
 
 bool factors(int N, int& r1, int& r2) {
   if (N < 2 || N > dev::kGenericMaxN) return false;

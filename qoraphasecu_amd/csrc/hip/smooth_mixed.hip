@@ -37,7 +37,7 @@ hipError_t smooth_mixed_launch(int r1, int r2, bool col, dev::Mode m, const dev:
   }
   QPCH_FOR_EACH_SMOOTH_MIXED(QPCH_SMOOTH_CASE)
 #undef QPCH_SMOOTH_CASE
-  return hipErrorInvalidValue;
+  return smooth_mixed7_launch(r1, r2, col, m, a, lines, grid, s);  // the pairs with a prime from 7 to 13
 }
 
 }  // namespace qpch

@@ -37,6 +37,11 @@ hipError_t smooth_sq_launch(int r1, int r2, bool col, dev::Mode m, const dev::Pa
 hipError_t smooth_mixed_launch(int r1, int r2, bool col, dev::Mode m, const dev::PassArgs& a, long long lines,
                                dim3 grid, hipStream_t s);
 
+// smooth_mixed_launch for the mixed pairs of lengths with a prime from 7 to 13
+// (smooth_mixed7.hip).
+hipError_t smooth_mixed7_launch(int r1, int r2, bool col, dev::Mode m, const dev::PassArgs& a, long long lines,
+                                dim3 grid, hipStream_t s);
+
 // smooth_launch for R2 in {7, 11, 13, 17, 19, 21, 23, 33, 35, 39, 49, 63} (smooth_prime.hip).
 hipError_t smooth_prime_launch(int r1, int r2, bool col, dev::Mode m, const dev::PassArgs& a, long long lines,
                                dim3 grid, hipStream_t s);

@@ -20,10 +20,10 @@ pytestmark = pytest.mark.gpu
 
 # N -> M: 67 -> 256, 131 -> 512, 257 -> 1024, 521 and 1021 -> 2048, 2039 -> 4096;
 # 134 = 2 * 67 and 201 = 3 * 67 (M 512); factors 7 to 64: 116 = 4 * 29, 248 = 8 * 31,
-# 252 = 4 * 9 * 7, 700 = 4 * 25 * 7 (M 256 to 2048)
+# 532 = 4 * 7 * 19, 868 = 4 * 7 * 31 (M 256 to 2048)
 SHAPES = [(67, 16), (16, 67), (131, 16), (16, 131), (257, 16), (16, 257), (521, 8), (8, 521), (1021, 4), (4, 1021),
           (2039, 2), (2, 2039), (134, 24), (24, 201), (67, 30), (131, 240), (67, 16, 16), (8, 4, 389), (131, 8, 8),
-          (116, 64), (64, 116), (248, 248), (252, 36), (36, 252), (700, 8), (8, 700), (248, 16, 16),
+          (116, 64), (64, 116), (248, 248), (532, 36), (36, 532), (868, 8), (8, 868), (248, 16, 16),
           # y axes of 3D shapes (forward and inverse line passes): 248, 67, 131, 1021
           (16, 248, 116), (8, 67, 32), (4, 131, 24), (2, 1021, 4)]
 

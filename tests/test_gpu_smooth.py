@@ -42,7 +42,15 @@ PAIRS = [(6, 10), (10, 6), (30, 12), (12, 30), (20, 60), (60, 20), (24, 40), (40
          (45, 50), (50, 45), (54, 75), (75, 54), (90, 100), (100, 90), (108, 125), (125, 108), (135, 150),
          (150, 135), (180, 225), (225, 180), (250, 270), (270, 250), (300, 360), (360, 300), (375, 16), (16, 375),
          (450, 32), (32, 450), (500, 16), (16, 500), (540, 16), (16, 540), (600, 16), (16, 600), (675, 8), (8, 675),
-         (750, 8), (8, 750), (900, 8), (8, 900), (1000, 8), (8, 1000), (1125, 4), (4, 1125), (1200, 4), (4, 1200)]
+         (750, 8), (8, 750), (900, 8), (8, 900), (1000, 8), (8, 1000), (1125, 4), (4, 1125), (1200, 4), (4, 1200),
+         # even splits of lengths with a prime from 7 to 13 (42 to 980)
+         (42, 98), (98, 42), (132, 196), (196, 132), (210, 252), (252, 210), (315, 390), (390, 315), (420, 441),
+         (441, 420), (462, 16), (16, 462), (700, 8), (8, 700), (84, 70), (70, 84), (66, 78), (78, 66), (110, 126),
+         (126, 110), (130, 140), (140, 130), (154, 156), (156, 154), (182, 198), (198, 182), (220, 234), (234, 220),
+         (260, 308), (308, 260), (364, 16), (16, 364), (440, 16), (16, 440), (468, 16), (16, 468), (504, 16),
+         (16, 504), (520, 8), (8, 520), (588, 8), (8, 588), (616, 8), (8, 616), (660, 8), (8, 660), (672, 8),
+         (8, 672), (728, 8), (8, 728), (780, 8), (8, 780), (840, 8), (8, 840), (880, 8), (8, 880), (924, 8),
+         (8, 924), (980, 8), (8, 980)]
 MIXED = [(64, 240), (240, 64), (30, 16, 16), (24, 64, 64), (16, 24, 40), (120, 8, 32), (96, 16, 16),
          (16, 32, 480),
          # smooth y axes: the forward and inverse line passes between the modulus and update
@@ -79,7 +87,8 @@ def test_smooth_teacher_forced_iteration(qpch, shape, schedule):
 # moves by 7e-7.
 @pytest.mark.parametrize("shape", [(240, 240), (80, 48), (30, 16, 16), (480, 96), (1280, 16), (16, 48, 40),
                                    (200, 200), (720, 72), (576, 36), (432, 16), (56, 88), (448, 104), (1792, 8),
-                                   (336, 26), (100, 100), (360, 180), (1000, 8), (136, 92), (1008, 8)])
+                                   (336, 26), (100, 100), (360, 180), (1000, 8), (136, 92), (1008, 8), (420, 196),
+                                   (840, 8)])
 def test_smooth_per_iteration_error(qpch, shape):
     """RAAR(30) from given starts: the device's error within max(1e-5, 3 x the spread of the
     oracle's float run against its FP64 run) of the oracle's float run at every iteration."""

@@ -87,7 +87,8 @@ typedef struct {
   int persist;         /* 0 = 2D chunks of iterations as one persistent launch where a kernel
                           exists for power-of-two axes (64 to 1024, radix 16) and for square
                           fp32 images of a smooth length (R1 R2, R1 in 2..16 a power of two,
-                          R2 in {3, 5, 15}; 96, 160; 18, 36, 72; 14, 28, 56, 22, 44, 26, 52),
+                          R2 in {3, 5, 15}; 96, 160; 18, 36, 72; 14, 28, 56, 22, 44, 26, 52;
+                          45, 50, 54, 100, 42, 70),
                           1 = also for the fp32 images whose two axes run the generic kernel,
                           2 = a launch per pass; every choice is the same arithmetic, bit for
                           bit */

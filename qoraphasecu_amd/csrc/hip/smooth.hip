@@ -22,8 +22,9 @@ namespace {
   X(16, 15) X(32, 3) X(32, 5)
 
 bool persistent_pair(int r1, int r2) {
-  if (!dev::smooth_is_pow2(r1) || r2 % 2 == 0) {  // 45, 50, 54, 100 (smooth_mixed.hip)
-    return (r1 == 9 && r2 == 5) || (r1 == 10 && r2 == 5) || (r1 == 6 && r2 == 9) || (r1 == 10 && r2 == 10);
+  if (!dev::smooth_is_pow2(r1) || r2 % 2 == 0) {  // 45, 50, 54, 100 (smooth_mixed.hip), 42, 70 (smooth_mixed7.hip)
+    return (r1 == 9 && r2 == 5) || (r1 == 10 && r2 == 5) || (r1 == 6 && r2 == 9) || (r1 == 10 && r2 == 10) ||
+           (r1 == 6 && r2 == 7) || (r1 == 10 && r2 == 7);
   }
   if (r2 == 3 || r2 == 5 || r2 == 15) return r1 <= 16 || (r1 == 32 && r2 <= 5);
   if (r2 == 7) return r1 <= 8;  // 14, 28, 56 (smooth_prime.hip)
@@ -53,14 +54,13 @@ constexpr MixedPair kMixed[] = {{45, 9, 5},      {50, 10, 5},     {54, 6, 9},   
                                 // lengths with a prime from 7 to 13 (otherwise chirp_pass)
                                 {42, 6, 7},      {98, 14, 7},     {132, 12, 11},   {196, 28, 7},    {210, 14, 15},
                                 {252, 28, 9},    {315, 35, 9},    {390, 30, 13},   {420, 28, 15},   {441, 21, 21},
-                                {462, 42, 11},   {700, 28, 25},
-                                {84, 12, 7}, {70, 10, 7}, {66, 11, 6}, {78, 13, 6}, {110, 11, 10},
-                                {126, 14, 9}, {130, 13, 10}, {140, 14, 10}, {154, 14, 11}, {156, 13, 12},
-                                {182, 14, 13}, {198, 18, 11}, {220, 20, 11}, {234, 18, 13}, {260, 20, 13},
-                                {308, 22, 14}, {364, 26, 14}, {440, 22, 20}, {468, 26, 18}, {504, 24, 21},
-                                {520, 26, 20}, {588, 28, 21}, {616, 28, 22}, {660, 30, 22}, {672, 28, 24},
-                                {728, 28, 26}, {780, 30, 26}, {840, 30, 28}, {880, 40, 22}, {924, 33, 28},
-                                {980, 35, 28}};
+                                {462, 42, 11},   {700, 28, 25},   {84, 12, 7},     {70, 10, 7},     {66, 11, 6},
+                                {78, 13, 6},     {110, 11, 10},   {126, 14, 9},    {130, 13, 10},   {140, 14, 10},
+                                {154, 14, 11},   {156, 13, 12},   {182, 14, 13},   {198, 18, 11},   {220, 20, 11},
+                                {234, 18, 13},   {260, 20, 13},   {308, 22, 14},   {364, 26, 14},   {440, 22, 20},
+                                {468, 26, 18},   {504, 24, 21},   {520, 26, 20},   {588, 28, 21},   {616, 28, 22},
+                                {660, 30, 22},   {672, 28, 24},   {728, 28, 26},   {780, 30, 26},   {840, 30, 28},
+                                {880, 40, 22},   {924, 33, 28},   {980, 35, 28}};
 
 bool factors(int N, int& r1, int& r2) {
   if (N < 2 || N > dev::kGenericMaxN) return false;

@@ -24,7 +24,7 @@ hipError_t twopass_smooth_mixed_launch(int r1, int r2, const dev::PassArgs& p0, 
   if (r1 == A && r2 == B) return smooth_detail::launch_twopass<A, B>(p0, p1, q, len, workers, s);
   QPCH_FOR_EACH_SMOOTH_MIXED_PERSISTENT(QPCH_SMOOTH_CASE)
 #undef QPCH_SMOOTH_CASE
-  return hipErrorInvalidValue;
+  return twopass_smooth_mixed7_launch(r1, r2, p0, p1, q, len, workers, s);
 }
 
 hipError_t smooth_mixed_launch(int r1, int r2, bool col, dev::Mode m, const dev::PassArgs& a, long long lines,

@@ -15,6 +15,19 @@ namespace qpch {
   X(26, 18) X(24, 21) X(26, 20) X(28, 21) X(28, 22) X(30, 22) X(28, 24) X(28, 26) X(30, 26) X(30, 28) \
   X(40, 22) X(33, 28) X(35, 28)
 
+// Square images with a persistent launch: 42 and 70 (66, 78 and 84 held 256 VGPRs and
+// AGPRs besides in the loop over both passes).
+#define QPCH_FOR_EACH_SMOOTH_MIXED7_PERSISTENT(X) X(6, 7) X(10, 7)
+
+hipError_t twopass_smooth_mixed7_launch(int r1, int r2, const dev::PassArgs& p0, const dev::PassArgs& p1,
+                                        const dev::SyncArgs& q, int len, int workers, hipStream_t s) {
+#define QPCH_SMOOTH_CASE(A, B) \
+  if (r1 == A && r2 == B) return smooth_detail::launch_twopass<A, B>(p0, p1, q, len, workers, s);
+  QPCH_FOR_EACH_SMOOTH_MIXED7_PERSISTENT(QPCH_SMOOTH_CASE)
+#undef QPCH_SMOOTH_CASE
+  return hipErrorInvalidValue;
+}
+
 hipError_t smooth_mixed7_launch(int r1, int r2, bool col, dev::Mode m, const dev::PassArgs& a, long long lines,
                                 dim3 grid, hipStream_t s) {
   if (m == dev::Mode::kGrad) return hipErrorInvalidValue;

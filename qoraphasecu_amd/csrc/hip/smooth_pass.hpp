@@ -46,7 +46,11 @@ hipError_t smooth_mixed7_launch(int r1, int r2, bool col, dev::Mode m, const dev
 hipError_t smooth_prime_launch(int r1, int r2, bool col, dev::Mode m, const dev::PassArgs& a, long long lines,
                                dim3 grid, hipStream_t s);
 
-// twopass_smooth_launch for the mixed pairs (smooth_mixed.hip).
+// twopass_smooth_launch for the mixed pairs of smooth_mixed7.hip.
+hipError_t twopass_smooth_mixed7_launch(int r1, int r2, const dev::PassArgs& p0, const dev::PassArgs& p1,
+                                        const dev::SyncArgs& q, int len, int workers, hipStream_t s);
+
+// twopass_smooth_launch for the mixed pairs (smooth_mixed.hip, then smooth_mixed7.hip).
 hipError_t twopass_smooth_mixed_launch(int r1, int r2, const dev::PassArgs& p0, const dev::PassArgs& p1,
                                        const dev::SyncArgs& q, int len, int workers, hipStream_t s);
 
@@ -60,7 +64,7 @@ hipError_t twopass_smooth_sq_launch(int r1, int r2, const dev::PassArgs& p0, con
 
 // The persistent launch for square images N x N of smooth axis length (twopass_smooth; the
 // lengths R1 R2 with R2 in {3, 5, 15} and R1 up to 16, 96, 160, and 18, 36, 72, 14, 28,
-// 56, 22, 44, 26, 52, 45, 50, 54 and 100):
+// 56, 22, 44, 26, 52, 45, 50, 54, 100, 42 and 70):
 // whether a kernel exists, and the launch of `len` iterations with `workers` work-groups
 // (0: as many per XCD as the busiest XCD has tiles per pass, at most what fits the device
 // at once).  q.tiles0 / q.tiles1: the tiles per start of the two passes.

@@ -50,6 +50,8 @@ CASES = [
     ((125, 125), dict(starts=8)),  # 25 * 5: smooth passes, per pass either way
     ((100, 100), dict(starts=8)),  # 10 * 10: twopass_smooth
     ((54, 54), dict(starts=3, batch=1)),  # 6 * 9: twopass_smooth
+    ((70, 70), dict(starts=8)),  # 10 * 7: twopass_smooth
+    ((42, 42), dict(starts=3, batch=1)),  # 6 * 7
     ((216, 216), dict(starts=8)),  # 8 * 27: smooth passes, per pass either way
     # twopass_smooth: square images of a smooth length (smooth.hpp)
     ((240, 240), dict(starts=8)),  # 16 * 15

@@ -51,6 +51,9 @@ constexpr MixedPair kMixed[] = {{45, 9, 5},      {50, 10, 5},     {54, 6, 9},   
                                 {360, 24, 15},   {375, 25, 15},   {450, 30, 15},   {500, 20, 25},   {540, 36, 15},
                                 {600, 24, 25},   {675, 27, 25},   {750, 30, 25},   {900, 36, 25},   {1000, 40, 25},
                                 {1125, 45, 25},  {1200, 48, 25},
+                                {81, 9, 9},      {162, 18, 9},    {324, 18, 18},   {648, 36, 18},   {729, 27, 27},
+                                {972, 36, 27},   {1296, 36, 36},  {625, 25, 25},   {1350, 45, 30},  {1440, 48, 30},
+                                {1800, 45, 40},  {2025, 45, 45},
                                 // lengths with a prime from 7 to 13 (otherwise chirp_pass)
                                 {42, 6, 7},      {98, 14, 7},     {132, 12, 11},   {196, 28, 7},    {210, 14, 15},
                                 {252, 28, 9},    {315, 35, 9},    {390, 30, 13},   {420, 28, 15},   {441, 21, 21},

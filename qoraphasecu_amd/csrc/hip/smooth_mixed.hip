@@ -12,7 +12,8 @@ namespace qpch {
 #define QPCH_FOR_EACH_SMOOTH_MIXED(X)                                                                  \
   X(9, 5) X(10, 5) X(6, 9) X(15, 5) X(6, 15) X(10, 10) X(12, 9) X(25, 5) X(15, 9) X(10, 15) X(12, 15)  \
   X(15, 15) X(10, 25) X(18, 15) X(20, 15) X(24, 15) X(25, 15) X(30, 15) X(20, 25) X(36, 15) X(24, 25) \
-  X(27, 25) X(30, 25) X(36, 25) X(40, 25) X(45, 25) X(48, 25)
+  X(27, 25) X(30, 25) X(36, 25) X(40, 25) X(45, 25) X(48, 25) X(9, 9) X(18, 9) X(18, 18) X(36, 18) X(27, 27)  \
+  X(36, 27) X(36, 36) X(25, 25) X(45, 30) X(48, 30) X(45, 40) X(45, 45)
 
 // Square images with a persistent launch: 45, 50, 54 and 100 (from 75 = 15 x 5 on, the
 // others held 256 VGPRs and AGPRs besides in the loop over both passes).

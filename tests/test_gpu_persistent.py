@@ -42,9 +42,9 @@ CASES = [
     ((256, 256), dict(starts=16, batch=4, lanes=4)),  # four persistent launches at once
     # twopass_generic: both axes of primes 2, 3 and 5 only, neither a power of two nor a
     # smooth length (fp32)
-    ((81, 81), dict(starts=8)),  # 3^4
-    ((27, 162), dict(starts=3, batch=1)),
-    ((243, 324), dict(starts=4)),  # 3^5, 4 * 81
+    ((243, 243), dict(starts=8)),  # 3^5
+    ((27, 486), dict(starts=3, batch=1)),
+    ((405, 243), dict(starts=4)),  # 5 * 81, 3^5
     ((405, 27), dict(starts=2)),  # 27 lines of 405: ragged tiles
     ((252, 252), dict(starts=8)),  # a factor 7: the chirp-z passes, per pass either way
     ((125, 125), dict(starts=8)),  # 25 * 5: smooth passes, per pass either way

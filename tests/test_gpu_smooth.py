@@ -43,6 +43,9 @@ PAIRS = [(6, 10), (10, 6), (30, 12), (12, 30), (20, 60), (60, 20), (24, 40), (40
          (150, 135), (180, 225), (225, 180), (250, 270), (270, 250), (300, 360), (360, 300), (375, 16), (16, 375),
          (450, 32), (32, 450), (500, 16), (16, 500), (540, 16), (16, 540), (600, 16), (16, 600), (675, 8), (8, 675),
          (750, 8), (8, 750), (900, 8), (8, 900), (1000, 8), (8, 1000), (1125, 4), (4, 1125), (1200, 4), (4, 1200),
+         (81, 162), (162, 81), (324, 648), (648, 324), (729, 16), (16, 729), (972, 8), (8, 972), (1296, 4),
+         (4, 1296), (625, 8), (8, 625), (1350, 4), (4, 1350), (1440, 4), (4, 1440), (1800, 2), (2, 1800), (2025, 2),
+         (2, 2025),
          # even splits of lengths with a prime from 7 to 13 (42 to 980)
          (42, 98), (98, 42), (132, 196), (196, 132), (210, 252), (252, 210), (315, 390), (390, 315), (420, 441),
          (441, 420), (462, 16), (16, 462), (700, 8), (8, 700), (84, 70), (70, 84), (66, 78), (78, 66), (110, 126),

@@ -63,7 +63,16 @@ constexpr MixedPair kMixed[] = {{45, 9, 5},      {50, 10, 5},     {54, 6, 9},   
                                 {234, 18, 13},   {260, 20, 13},   {308, 22, 14},   {364, 26, 14},   {440, 22, 20},
                                 {468, 26, 18},   {504, 24, 21},   {520, 26, 20},   {588, 28, 21},   {616, 28, 22},
                                 {660, 30, 22},   {672, 28, 24},   {728, 28, 26},   {780, 30, 26},   {840, 30, 28},
-                                {880, 40, 22},   {924, 33, 28},   {980, 35, 28}};
+                                {880, 40, 22},   {924, 33, 28},   {980, 35, 28},
+                                // more lengths with primes up to 13, odd ones among them (smooth_mixed13.hip)
+                                {63, 9, 7},      {77, 11, 7},     {91, 13, 7},     {99, 11, 9},     {105, 15, 7},
+                                {117, 13, 9},    {121, 11, 11},   {143, 13, 11},   {147, 21, 7},    {165, 15, 11},
+                                {169, 13, 13},   {175, 25, 7},    {189, 21, 9},    {195, 15, 13},   {231, 21, 11},
+                                {242, 22, 11},   {245, 35, 7},    {273, 21, 13},   {275, 25, 11},   {286, 22, 13},
+                                {294, 21, 14},   {297, 27, 11},   {325, 25, 13},   {330, 22, 15},   {338, 26, 13},
+                                {350, 25, 14},   {351, 27, 13},   {363, 33, 11},   {378, 21, 18},   {385, 35, 11},
+                                {396, 22, 18},   {429, 33, 13},   {455, 35, 13},   {484, 22, 22},   {490, 35, 14},
+                                {495, 33, 15},   {507, 39, 13}};
 
 bool factors(int N, int& r1, int& r2) {
   if (N < 2 || N > dev::kGenericMaxN) return false;

@@ -38,7 +38,7 @@ hipError_t smooth_mixed7_launch(int r1, int r2, bool col, dev::Mode m, const dev
   }
   QPCH_FOR_EACH_SMOOTH_MIXED7(QPCH_SMOOTH_CASE)
 #undef QPCH_SMOOTH_CASE
-  return hipErrorInvalidValue;
+  return smooth_mixed13_launch(r1, r2, col, m, a, lines, grid, s);  // smooth_mixed13.hip
 }
 
 }  // namespace qpch

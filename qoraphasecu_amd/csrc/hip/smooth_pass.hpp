@@ -42,6 +42,10 @@ hipError_t smooth_mixed_launch(int r1, int r2, bool col, dev::Mode m, const dev:
 hipError_t smooth_mixed7_launch(int r1, int r2, bool col, dev::Mode m, const dev::PassArgs& a, long long lines,
                                 dim3 grid, hipStream_t s);
 
+// smooth_mixed7_launch for the further mixed pairs of smooth_mixed13.hip.
+hipError_t smooth_mixed13_launch(int r1, int r2, bool col, dev::Mode m, const dev::PassArgs& a, long long lines,
+                                 dim3 grid, hipStream_t s);
+
 // smooth_launch for R2 in {7, 11, 13, 17, 19, 21, 23, 33, 35, 39, 49, 63} (smooth_prime.hip).
 hipError_t smooth_prime_launch(int r1, int r2, bool col, dev::Mode m, const dev::PassArgs& a, long long lines,
                                dim3 grid, hipStream_t s);

@@ -46,6 +46,12 @@ PAIRS = [(6, 10), (10, 6), (30, 12), (12, 30), (20, 60), (60, 20), (24, 40), (40
          (81, 162), (162, 81), (324, 648), (648, 324), (729, 16), (16, 729), (972, 8), (8, 972), (1296, 4),
          (4, 1296), (625, 8), (8, 625), (1350, 4), (4, 1350), (1440, 4), (4, 1440), (1800, 2), (2, 1800), (2025, 2),
          (2, 2025),
+         # more lengths with primes up to 13 (63 to 507)
+         (63, 77), (77, 63), (91, 99), (99, 91), (105, 117), (117, 105), (121, 143), (143, 121), (147, 165),
+         (165, 147), (169, 175), (175, 169), (189, 195), (195, 189), (231, 242), (242, 231), (245, 273), (273, 245),
+         (275, 286), (286, 275), (294, 297), (297, 294), (325, 330), (330, 325), (338, 350), (350, 338), (351, 363),
+         (363, 351), (378, 385), (385, 378), (396, 429), (429, 396), (455, 16), (16, 455), (484, 16), (16, 484),
+         (490, 16), (16, 490), (495, 16), (16, 495), (507, 16), (16, 507),
          # even splits of lengths with a prime from 7 to 13 (42 to 980)
          (42, 98), (98, 42), (132, 196), (196, 132), (210, 252), (252, 210), (315, 390), (390, 315), (420, 441),
          (441, 420), (462, 16), (16, 462), (700, 8), (8, 700), (84, 70), (70, 84), (66, 78), (78, 66), (110, 126),

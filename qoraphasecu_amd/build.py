@@ -62,12 +62,12 @@ def core_ext() -> str:
     return os.path.join(HERE, "_core" + sysconfig.get_config_var("EXT_SUFFIX"))
 
 
-ENGINE_SOURCES = ["engine.hip", "twopass.hip", "smooth.hip", "smooth_sq.hip", "smooth_prime.hip", "smooth_mixed.hip", "smooth_mixed7.hip", "smooth_mixed13.hip", "chirp.hip"]
+ENGINE_SOURCES = ["engine.hip", "twopass.hip", "smooth.hip", "smooth_sq.hip", "smooth_prime.hip", "smooth_mixed.hip", "smooth_mixed7.hip", "smooth_mixed13.hip", "smooth_mixed1k.hip", "chirp.hip"]
 # smooth.hip: multiply-adds fused only within one source expression (decided by the front
 # end), so that the per-pass and the persistent kernels, which inline the same tile code
 # into different contexts, round alike.  With the default (fused wherever the back end
 # finds a pair) the two differed in the last bits on 120 = 8 * 15 and 30 = 2 * 15.
-SOURCE_FLAGS = {"smooth.hip": ["-ffp-contract=on"], "smooth_sq.hip": ["-ffp-contract=on"], "smooth_prime.hip": ["-ffp-contract=on"], "smooth_mixed.hip": ["-ffp-contract=on"], "smooth_mixed7.hip": ["-ffp-contract=on"], "smooth_mixed13.hip": ["-ffp-contract=on"], "chirp.hip": ["-ffp-contract=on"]}
+SOURCE_FLAGS = {"smooth.hip": ["-ffp-contract=on"], "smooth_sq.hip": ["-ffp-contract=on"], "smooth_prime.hip": ["-ffp-contract=on"], "smooth_mixed.hip": ["-ffp-contract=on"], "smooth_mixed7.hip": ["-ffp-contract=on"], "smooth_mixed13.hip": ["-ffp-contract=on"], "smooth_mixed1k.hip": ["-ffp-contract=on"], "chirp.hip": ["-ffp-contract=on"]}
 
 
 def build_engine() -> str:

@@ -72,7 +72,14 @@ constexpr MixedPair kMixed[] = {{45, 9, 5},      {50, 10, 5},     {54, 6, 9},   
                                 {294, 21, 14},   {297, 27, 11},   {325, 25, 13},   {330, 22, 15},   {338, 26, 13},
                                 {350, 25, 14},   {351, 27, 13},   {363, 33, 11},   {378, 21, 18},   {385, 35, 11},
                                 {396, 22, 18},   {429, 33, 13},   {455, 35, 13},   {484, 22, 22},   {490, 35, 14},
-                                {495, 33, 15},   {507, 39, 13}};
+                                {495, 33, 15},   {507, 39, 13},
+                                // 525 to 1014 with primes up to 13 (smooth_mixed1k.hip)
+                                {525, 25, 21},   {546, 26, 21},   {550, 25, 22},   {567, 27, 21},   {572, 26, 22},
+                                {585, 39, 15},   {594, 27, 22},   {630, 30, 21},   {650, 26, 25},   {676, 26, 26},
+                                {693, 33, 21},   {702, 27, 26},   {726, 33, 22},   {735, 35, 21},   {756, 28, 27},
+                                {770, 35, 22},   {792, 33, 24},   {819, 39, 21},   {825, 33, 25},   {858, 33, 26},
+                                {875, 35, 25},   {882, 42, 21},   {891, 33, 27},   {910, 35, 26},   {936, 36, 26},
+                                {945, 35, 27},   {968, 44, 22},   {975, 39, 25},   {990, 33, 30},   {1014, 39, 26}};
 
 bool factors(int N, int& r1, int& r2) {
   if (N < 2 || N > dev::kGenericMaxN) return false;

@@ -25,7 +25,7 @@ hipError_t smooth_mixed13_launch(int r1, int r2, bool col, dev::Mode m, const de
   }
   QPCH_FOR_EACH_SMOOTH_MIXED13(QPCH_SMOOTH_CASE)
 #undef QPCH_SMOOTH_CASE
-  return hipErrorInvalidValue;
+  return smooth_mixed1k_launch(r1, r2, col, m, a, lines, grid, s);  // smooth_mixed1k.hip
 }
 
 }  // namespace qpch

@@ -42,6 +42,10 @@ hipError_t smooth_mixed_launch(int r1, int r2, bool col, dev::Mode m, const dev:
 hipError_t smooth_mixed7_launch(int r1, int r2, bool col, dev::Mode m, const dev::PassArgs& a, long long lines,
                                 dim3 grid, hipStream_t s);
 
+// smooth_mixed13_launch for the pairs of smooth_mixed1k.hip (525 to 1014).
+hipError_t smooth_mixed1k_launch(int r1, int r2, bool col, dev::Mode m, const dev::PassArgs& a, long long lines,
+                                 dim3 grid, hipStream_t s);
+
 // smooth_mixed7_launch for the further mixed pairs of smooth_mixed13.hip.
 hipError_t smooth_mixed13_launch(int r1, int r2, bool col, dev::Mode m, const dev::PassArgs& a, long long lines,
                                  dim3 grid, hipStream_t s);

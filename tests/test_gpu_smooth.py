@@ -46,6 +46,13 @@ PAIRS = [(6, 10), (10, 6), (30, 12), (12, 30), (20, 60), (60, 20), (24, 40), (40
          (81, 162), (162, 81), (324, 648), (648, 324), (729, 16), (16, 729), (972, 8), (8, 972), (1296, 4),
          (4, 1296), (625, 8), (8, 625), (1350, 4), (4, 1350), (1440, 4), (4, 1440), (1800, 2), (2, 1800), (2025, 2),
          (2, 2025),
+         # 525 to 1014 with primes up to 13
+         (525, 8), (8, 525), (546, 8), (8, 546), (550, 8), (8, 550), (567, 8), (8, 567), (572, 8), (8, 572),
+         (585, 8), (8, 585), (594, 8), (8, 594), (630, 8), (8, 630), (650, 8), (8, 650), (676, 8), (8, 676),
+         (693, 8), (8, 693), (702, 8), (8, 702), (726, 8), (8, 726), (735, 8), (8, 735), (756, 8), (8, 756),
+         (770, 8), (8, 770), (792, 8), (8, 792), (819, 8), (8, 819), (825, 8), (8, 825), (858, 8), (8, 858),
+         (875, 8), (8, 875), (882, 8), (8, 882), (891, 8), (8, 891), (910, 8), (8, 910), (936, 8), (8, 936),
+         (945, 8), (8, 945), (968, 8), (8, 968), (975, 8), (8, 975), (990, 8), (8, 990), (1014, 8), (8, 1014),
          # more lengths with primes up to 13 (63 to 507)
          (63, 77), (77, 63), (91, 99), (99, 91), (105, 117), (117, 105), (121, 143), (143, 121), (147, 165),
          (165, 147), (169, 175), (175, 169), (189, 195), (195, 189), (231, 242), (242, 231), (245, 273), (273, 245),
